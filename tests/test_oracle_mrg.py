@@ -1,0 +1,147 @@
+"""Pins the oracle's MRG, skip powers, seeds and random numbers by routes
+independent of the oracle's C arithmetic (Python big integers, square-and-
+multiply matrix powers, closed forms, the reference table's digest)."""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+P = 2**31 - 1
+X = 107374182
+Y = 104480
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "mrg_skip_table.json")
+
+
+def py_steps(seed, k):
+    """x_n = (X x_{n-1} + Y x_{n-5}) mod P on the state (z1..z5) = newest first."""
+    z = [int(v) for v in seed]
+    out = []
+    for _ in range(k):
+        nz = (X * z[0] + Y * z[4]) % P
+        z = [nz] + z[:4]
+        out.append(nz)
+    return z, out
+
+
+def mat_mul(a, b):
+    return [[sum(a[i][k] * b[k][j] for k in range(5)) % P for j in range(5)] for i in range(5)]
+
+
+def mat_pow(e):
+    m = [[X, 0, 0, 0, Y], [1, 0, 0, 0, 0], [0, 1, 0, 0, 0], [0, 0, 1, 0, 0], [0, 0, 0, 1, 0]]
+    r = [[int(i == j) for j in range(5)] for i in range(5)]
+    while e:
+        if e & 1:
+            r = mat_mul(r, m)
+        m = mat_mul(m, m)
+        e >>= 1
+    return r
+
+
+def apply(m, z):
+    return [sum(m[i][k] * z[k] for k in range(5)) % P for i in range(5)]
+
+
+def test_seed_closed_form(oracle):
+    # make_mrg_seed(2, 3): low 30 bits + 1, next 30 bits + 1, top nibbles + 1.
+    assert list(oracle.mrg_seed(2, 3)) == [3, 1, 4, 1, 1]
+
+
+def test_seed_general(oracle):
+    s1, s2 = (5 << 30) | 7 | (1 << 62), (9 << 30) | 11 | (3 << 61)
+    got = list(oracle.mrg_seed(s1, s2))
+    assert got[0] == 8 and got[2] == 12
+    assert got[1] == ((s1 >> 30) & (2**30 - 1)) + 1
+    assert got[3] == ((s2 >> 30) & (2**30 - 1)) + 1
+    assert got[4] == (s2 >> 60) * 16 + (s1 >> 60) + 1
+
+
+def test_step_matches_bigint_recurrence(oracle):
+    seed = oracle.mrg_seed(2, 3)
+    st = oracle.mrg_state(seed)
+    _, ref = py_steps(seed, 2000)
+    got = [oracle.lib().o_mrg_next(st) for _ in range(2000)]
+    assert got == ref
+
+
+@pytest.mark.parametrize("hi,mid,lo", [(0, 0, 1), (0, 0, 255), (0, 0, 0x1234567), (0, 1, 0),
+                                       (0, 65535, 0), (0, 2**33 + 77, 0), (2, 0, 2 * 41),
+                                       (50, 7, 0), (0, 2**63 + 5, 2**40 + 3)])
+def test_skip_equals_matrix_power(oracle, hi, mid, lo):
+    seed = oracle.mrg_seed(2, 3)
+    st = oracle.mrg_state(seed)
+    oracle.lib().o_mrg_skip(st, hi, mid, lo)
+    ref = apply(mat_pow(hi * 2**128 + mid * 2**64 + lo), [int(v) for v in seed])
+    assert list(st.z) == ref
+
+
+def test_table_small_powers_by_brute_force(oracle):
+    t = oracle.skip_table()
+    seed = [3, 1, 4, 1, 1]
+    for k in range(0, 256, 17):
+        m = oracle.mrg_power(0, k).astype(object)
+        z = apply(m.tolist(), seed)
+        ref, _ = py_steps(seed, k)
+        assert z == ref
+    assert t.shape == (24, 256, 9)
+
+
+def test_table_composition(oracle):
+    for b in range(23):
+        a = oracle.mrg_power(b, 1).astype(object).tolist()
+        r = a
+        for _ in range(8):
+            r = mat_mul(r, r)
+        assert r == oracle.mrg_power(b + 1, 1).astype(object).tolist()
+    for b in (0, 8, 11, 23):
+        a = oracle.mrg_power(b, 1).astype(object).tolist()
+        acc = [[int(i == j) for j in range(5)] for i in range(5)]
+        for v in range(1, 256):
+            acc = mat_mul(acc, a)
+            if v in (2, 3, 128, 255):
+                assert acc == oracle.mrg_power(b, v).astype(object).tolist()
+
+
+def test_nine_word_encoding_reconstructs_full_matrix(oracle):
+    """The table's 9 words {s,t,u,v,w,a,b,c,d} (the layout that
+    graph_generation.cu:101-112 applies) rebuild the full power of M."""
+    t = oracle.skip_table()
+    for b, v in [(0, 1), (0, 77), (8, 200), (11, 3), (16, 250), (23, 255)]:
+        s, tt, u, vv, w, a, bb, c, d = (int(x) for x in t[b, v])
+        full = [[d, Y * s, Y * a, Y * bb, Y * c],
+                [c, w, Y * s, Y * a, Y * bb],
+                [bb, vv, w, Y * s, Y * a],
+                [a, u, vv, w, Y * s],
+                [s, tt, u, vv, w]]
+        full = [[x % P for x in r] for r in full]
+        assert full == oracle.mrg_power(b, v).astype(object).tolist()
+
+
+def test_table_digest_matches_reference_table(oracle):
+    doc = json.load(open(GOLDEN))
+    t = oracle.skip_table()
+    blob = t.astype("<u4").tobytes()
+    assert list(t.shape) == doc["shape"]
+    assert hashlib.sha256(blob).hexdigest() == doc["sha256"]
+
+
+def test_random_numbers(oracle):
+    seed = [3, 1, 4, 1, 1]
+    for pos in (0, 1, 2, 63, 1000):
+        z = apply(mat_pow(2 * 2**128 + 2 * pos), seed)
+        _, us = py_steps(z, 4)
+        ref = [u * (1.0 / P) for u in us]
+        assert list(oracle.random_numbers(4, 2, 3, pos)) == ref
+
+
+def test_weight_is_rn_f32_times_two_pow_minus_31(oracle):
+    seed = oracle.mrg_seed(2, 3)
+    st = oracle.mrg_state(seed)
+    _, us = py_steps(seed, 500)
+    ws = [oracle.lib().o_mrg_weight(st) for _ in range(500)]
+    for u, w in zip(us, ws):
+        assert np.float32(w) == np.float32(np.float32(u) * np.float32(2.0**-31))
+    # u >= 2^31 - 64 rounds to 2^31 in f32, so w == 1.0 exactly is possible.
+    assert np.float32(np.float32(2**31 - 2) * np.float32(2.0**-31)) == np.float32(1.0)
