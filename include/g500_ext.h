@@ -1,0 +1,69 @@
+/*
+ * Library entry points beyond the reference headers (src/ *.h).
+ *
+ * The first group restates helpers that are `static` in the reference's
+ * runner (examples/graph500_runner.cu) so that any host program can validate
+ * and count like the runner does. The second group is library control
+ * (error mode, timing, host<->device copies, tuning knobs) used by the
+ * runner, the tests and the benchmark.
+ */
+#ifndef G500_EXT_H
+#define G500_EXT_H
+#include "traversal.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* graph500_runner.cu:43-74: up to max_roots distinct non-isolated roots drawn
+ * from make_random_numbers(2, seed1, seed2, counter). Returns the count. */
+i32 generate_bfs_roots(Worker_State *ws, const Oned_Graph *g, u64 seed1, u64 seed2, u64 *roots,
+                       i32 max_roots);
+/* graph500_runner.cu:171-201: sum over vertices with pred != -1 of the CSR
+ * entries u <= v (the TEPS numerator). pred is a device pointer. */
+u64 oned_graph_edge_count(Worker_State *ws, const Oned_Graph *g, const i64 *pred);
+/* graph500_runner.cu:84-169, :203-255: Graph500 rules 1-7 on bfs->pred
+ * (depths derived with bfs_compute_dist_from_pred). Returns true when no
+ * error is found; *errors_out (may be NULL) receives the error count. */
+b32 oned_graph_bfs_validate(Worker_State *ws, const Oned_Graph *g, Bfs_State *bfs, u64 root,
+                            u64 *errors_out);
+/* graph500_runner.cu:22-41: min, q1, median, q3, max, mean, stddev. */
+void get_statistics(const f64 *x, i32 n, f64 r[7]);
+
+/* ---- library control --------------------------------------------------- */
+/* G500_ERRORS_ABORT (default): fatal errors print and exit like the
+ * reference. G500_ERRORS_RETURN: they set a status and the entry point
+ * returns early (used by tests and serving code). */
+#define G500_ERRORS_ABORT 0
+#define G500_ERRORS_RETURN 1
+void g500_set_error_mode(i32 mode);
+i32 g500_status(void);
+const char *g500_last_error(void);
+void g500_clear_error(void);
+
+/* Device time in milliseconds of the last oned_graph_bfs_run /
+ * oned_graph_sssp_run / generate_kronecker_range /
+ * oned_graph_from_tuple_graph, measured with HIP events on the library's
+ * stream around its kernels. */
+f64 g500_last_run_ms(void);
+
+/* Blocking copies between host memory and the library's device memory. */
+void g500_copy_to_host(void *dst, const void *src_device, u64 bytes);
+void g500_copy_to_device(void *dst_device, const void *src, u64 bytes);
+
+/* Tuning knobs (names: "bfs_alpha", "bfs_beta", "bfs_block", "bfs_hub_degree",
+ * "sssp_block", "gen_blocks"). Returns 0 on success, -1 for an unknown name. */
+i32 g500_set_param(const char *name, f64 value);
+i32 g500_get_param(const char *name, f64 *value);
+
+/* Per-level log of the last BFS run: mode (0 top-down, 1 bottom-up) and
+ * frontier size of each level. Returns the number of levels. */
+i32 g500_bfs_level_log(const Bfs_State *bfs, i32 *modes, u64 *frontier, i32 max_levels);
+/* Bucket / light-iteration counts of the last SSSP run. */
+void g500_sssp_run_stats(const Sssp_State *sssp, u64 *buckets, u64 *light_iterations,
+                         u64 *relaxations);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* G500_EXT_H */

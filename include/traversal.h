@@ -1,0 +1,96 @@
+/*
+ * CSR graph, BFS and delta-stepping SSSP of the MI355X Graph500 library.
+ *
+ * Replaces src/traversal.h:8-54 and src/traversal.cu:1-721 (single rank).
+ * All pointers inside the structs are device pointers into the worker's
+ * device arena; *_destroy and oned_graph_free only zero the struct
+ * (traversal.cu:565-568, :713-721); memory returns with worker_finalize.
+ * Every entry point is blocking: it returns after the device work is done.
+ *
+ * Layout deltas from the reference (the fields the reference runner reads
+ * keep their names and meaning):
+ *   - Oned_Graph.column is u32 (32-bit vertex ids, SCALE <= 32) instead of
+ *     u64; row offsets stay u64. Each row is stored in ascending
+ *     (weight, column) order, so the CSR is deterministic (the reference's
+ *     in-row order depends on atomic arrival, traversal.cu:51).
+ *   - Bfs_State keeps pred/dist/max_nlocalverts; the frontier counters and
+ *     grid-barrier words (traversal.h:23-26) live in the device control block
+ *     `ctl`; frontiers are u32 queues plus bitmaps for bottom-up levels.
+ *   - Sssp_State keeps pred/dist/max_nlocalverts; the working labels are one
+ *     packed u64 per vertex (f32 distance bits << 32 | parent), updated with
+ *     a compare-and-swap loop instead of dist_bits + pred_lock
+ *     (traversal.cu:211-240); delta is a field, 0.1f by default
+ *     (traversal.cu:645).
+ */
+#ifndef G500_TRAVERSAL_H
+#define G500_TRAVERSAL_H
+#include "base.h"
+#include "graph_generation.h"
+#include "worker.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* traversal.h:8-16 */
+typedef struct Oned_Graph Oned_Graph;
+struct Oned_Graph {
+    u64 nlocalverts;
+    u64 nlocaledges;
+    u64 nglobalverts, notisolated;
+    u64 *rowstarts; /* [nlocalverts + 1] */
+    u32 *column;    /* [nlocaledges] */
+    f32 *weights;   /* [nlocaledges] */
+};
+
+/* traversal.h:18-27 */
+typedef struct Bfs_State Bfs_State;
+struct Bfs_State {
+    i64 *pred;             /* [n] parent; -1 unreached; root -> root */
+    i64 *dist;             /* [n] depth, allocated by bfs_compute_dist_from_pred */
+    u32 *frontier[2];      /* [n] vertex queues */
+    u64 *frontier_bits[2]; /* [ceil(n/64)] frontier bitmaps */
+    u64 *visited;          /* [ceil(n/64)] */
+    u32 *hubs;             /* [n] queue of high-degree frontier vertices */
+    void *ctl;             /* device control block */
+    u64 max_nlocalverts;
+};
+
+/* traversal.h:30-41 */
+typedef struct Sssp_State Sssp_State;
+struct Sssp_State {
+    i64 *pred;          /* [n] parent; -1 unreached; root -> root */
+    f32 *dist;          /* [n] distance; -1.0f unreached */
+    u64 *dist_pred;     /* [n] packed working labels */
+    u32 *frontier;      /* [6n] light-phase queues: 2 buffers x 3 work tiers */
+    u32 *members;       /* [6n] bucket-member logs: 3 work tiers x 2n */
+    u32 *split;         /* [n] number of light entries (w < split_delta) per row */
+    u32 *stamp;         /* [n] light-queue de-duplication */
+    u32 *mstamp;        /* [n] bucket-member de-duplication */
+    void *ctl;          /* device control block */
+    f32 delta;          /* bucket width (default 0.1f) */
+    f32 split_delta;    /* delta that `split` was computed for (0: none yet) */
+    u64 max_nlocalverts;
+};
+
+/* traversal.h:43 */
+Oned_Graph oned_graph_from_tuple_graph(Worker_State *ws, Tuple_Graph *tg, u64 nglobalverts);
+/* traversal.h:44 */
+b32 oned_graph_is_vertex_isolated(Worker_State *ws, const Oned_Graph *g, u64 global_vertex);
+/* traversal.h:45-49 */
+Bfs_State oned_graph_bfs_create(Worker_State *ws, const Oned_Graph *g);
+void oned_graph_bfs_clear(Worker_State *ws, Bfs_State *bfs);
+void oned_graph_bfs_run(Worker_State *ws, const Oned_Graph *g, Bfs_State *bfs, u64 root);
+i64 *bfs_compute_dist_from_pred(Worker_State *ws, const Oned_Graph *g, Bfs_State *bfs, u64 root);
+void oned_graph_bfs_destroy(Bfs_State *bfs);
+/* traversal.h:50-54 */
+Sssp_State oned_graph_sssp_create(Worker_State *ws, const Oned_Graph *g);
+void oned_graph_sssp_clear(Worker_State *ws, Sssp_State *sssp);
+void oned_graph_sssp_run(Worker_State *ws, const Oned_Graph *g, Sssp_State *sssp, u64 root);
+void oned_graph_sssp_destroy(Sssp_State *sssp);
+void oned_graph_free(Oned_Graph *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* G500_TRAVERSAL_H */

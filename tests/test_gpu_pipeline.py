@@ -1,0 +1,260 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU oracle.
+
+Bars: generator edges/weights and CSR arrays bit-exact; BFS depths bit-exact
+with parent trees passing the Graph500 rules (GPU validator and oracle
+validator); validation error counts and depth arrays equal on corrupted
+parent arrays; SSSP FP32 distances bit-exact against the oracle's FP32
+Dijkstra with the parent certificate holding; roots and TEPS edge counts equal.
+"""
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import oracle_graph
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sess():
+    import g500
+    s = g500.Session(arena_bytes=48 << 30)
+    yield s
+    s.close()
+
+
+@pytest.fixture
+def arena(sess):
+    m = sess.mark()
+    yield sess
+    sess.release(m)
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+# ------------------------------------------------------------------ generator
+@pytest.mark.parametrize("scale,ef,s1,s2", [(1, 16, 2, 3), (5, 16, 2, 3), (10, 16, 2, 3), (12, 16, 2, 3),
+                                            (12, 4, 2, 3), (11, 32, 7, 11)])
+def test_generator_bitexact(arena, oracle, scale, ef, s1, s2):
+    say("generator", scale, ef, s1, s2)
+    tg = arena.generate(scale, ef, s1, s2)
+    e, w = arena.tuples_to_host(tg)
+    eo, wo = oracle.kronecker(scale, ef, s1, s2)
+    assert np.array_equal(e, eo)
+    assert np.array_equal(w.view(np.uint32), wo.view(np.uint32))
+
+
+@pytest.mark.parametrize("scale,start,end", [(16, 1000, 70001), (20, 2**20 - 300, 2**20 + 5000),
+                                             (28, 2**33 + 17, 2**33 + 3000), (26, 2**30 - 999, 2**30)])
+def test_generator_ranges(arena, oracle, scale, start, end):
+    say("generator range", scale, start, end)
+    tg = arena.generate(scale, 16, start=start, end=end)
+    e, w = arena.tuples_to_host(tg)
+    eo, wo = oracle.kronecker(scale, 16, start=start, end=end)
+    assert np.array_equal(e, eo)
+    assert np.array_equal(w.view(np.uint32), wo.view(np.uint32))
+
+
+# ------------------------------------------------------------------------ CSR
+def check_csr(sess, oracle, n, edges, weights):
+    tg = sess.upload_tuples(edges, weights)
+    g = sess.csr(tg, n)
+    rs, col, w = sess.csr_to_host(g)
+    ref = oracle.csr(n, edges, weights)
+    assert np.array_equal(rs, ref.rowstarts)
+    assert np.array_equal(col, ref.column)
+    assert np.array_equal(w.view(np.uint32), ref.weights.view(np.uint32))
+    deg = np.diff(rs.astype(np.int64))
+    assert g.notisolated == int((deg > 0).sum())
+    return g
+
+
+@pytest.mark.parametrize("scale,ef", [(1, 1), (6, 4), (10, 16), (12, 16), (16, 16)])
+def test_csr_matches_oracle(arena, oracle, scale, ef):
+    say("csr", scale, ef)
+    e, w, _ = oracle_graph(scale, ef)
+    check_csr(arena, oracle, 1 << scale, e, w)
+
+
+def test_csr_from_generated_tuples(arena, oracle):
+    tg = arena.generate(14, 16)
+    g = arena.csr(tg, 1 << 14)
+    rs, col, w = arena.csr_to_host(g)
+    _, _, ref = oracle_graph(14, 16)
+    assert np.array_equal(rs, ref.rowstarts) and np.array_equal(col, ref.column)
+    assert np.array_equal(w.view(np.uint32), ref.weights.view(np.uint32))
+
+
+def test_csr_edge_cases(arena, oracle):
+    rng = np.random.default_rng(5)
+    cases = []
+    cases.append((10, np.zeros((0, 2), np.uint64), np.zeros(0, np.float32)))                      # no edges
+    cases.append((10, np.array([[3, 3], [7, 7]], np.uint64), np.array([0.5, 0.25], np.float32)))   # loops only
+    # a star of 20000 leaves (radix tier), rows of 70 (block tier) and 40 (wave tier)
+    n = 50000
+    star = np.stack([np.zeros(20000, np.uint64), np.arange(1, 20001, dtype=np.uint64)], 1)
+    mid = np.stack([np.full(70 * 30, 0, np.uint64) + np.repeat(np.arange(30000, 30030, dtype=np.uint64), 70),
+                    rng.integers(0, n, 70 * 30).astype(np.uint64)], 1)
+    small = np.stack([np.repeat(np.arange(40000, 40100, dtype=np.uint64), 40),
+                      rng.integers(0, n, 4000).astype(np.uint64)], 1)
+    e = np.concatenate([star, mid, small, star[:500]])                                        # duplicates
+    w = rng.random(len(e)).astype(np.float32)
+    w[:7] = 0.0
+    w[7:20] = 1.0
+    w[-500:] = w[:500]                                                                          # tied duplicates
+    cases.append((n, e, w))
+    # 3 vertices, n not a multiple of 64
+    cases.append((3, np.array([[0, 1], [1, 2], [2, 0], [0, 1]], np.uint64), np.array([.1, .2, .3, .1], np.float32)))
+    for n, e, w in cases:
+        say("csr case", n, len(e))
+        check_csr(arena, oracle, n, e, w)
+
+
+# ------------------------------------------------------------------------ BFS
+def bfs_check(sess, oracle, g, ref, root):
+    sess.bfs(g, sess.bfs_state, root)
+    pred = sess.bfs_pred(sess.bfs_state)
+    _, depth_ref = oracle.bfs(ref, root)
+    assert oracle.bfs_validate(ref, pred, root) == 0
+    ok, nerr = sess.bfs_validate(g, sess.bfs_state, root)
+    assert ok and nerr == 0
+    depth = sess.bfs_depth(g, sess.bfs_state, root)
+    assert np.array_equal(depth, depth_ref)
+    assert sess.edge_count(g, sess.bfs_state.pred) == oracle.edge_count(ref, pred)
+    return pred
+
+
+@pytest.mark.parametrize("scale", [10, 12, 16])
+def test_bfs_depths_and_rules(arena, oracle, scale):
+    e, w, ref = oracle_graph(scale)
+    g = check_csr(arena, oracle, 1 << scale, e, w)
+    arena.bfs_state = arena.bfs_create(g)
+    roots = oracle.roots(ref)
+    assert np.array_equal(arena.roots(g), roots)
+    for r in roots[:8]:
+        say("bfs", scale, int(r))
+        bfs_check(arena, oracle, g, ref, int(r))
+
+
+# Beamer's rule: top-down -> bottom-up when m_f > m_u / alpha (a large alpha
+# switches early); bottom-up -> top-down when n_f < n / beta (a large beta stays).
+@pytest.mark.parametrize("alpha,beta,hub", [(1e-9, 18, 2048), (1e9, 1e9, 2048), (15, 18, 4), (2, 4, 64)])
+def test_bfs_direction_modes(arena, oracle, alpha, beta, hub):
+    e, w, ref = oracle_graph(14)
+    g = check_csr(arena, oracle, 1 << 14, e, w)
+    arena.bfs_state = arena.bfs_create(g)
+    old = {k: arena.get_param(k) for k in ("bfs_alpha", "bfs_beta", "bfs_hub_degree")}
+    try:
+        arena.set_param("bfs_alpha", alpha)
+        arena.set_param("bfs_beta", beta)
+        arena.set_param("bfs_hub_degree", hub)
+        for r in oracle.roots(ref)[:4]:
+            say("bfs modes", alpha, beta, hub, int(r))
+            bfs_check(arena, oracle, g, ref, int(r))
+            modes, sizes = arena.level_log(arena.bfs_state)
+            if alpha < 1e-3:
+                assert not any(modes)      # top-down only
+            if alpha > 1e8 and beta > 1e8:
+                assert all(modes)          # bottom-up only
+    finally:
+        for k, v in old.items():
+            arena.set_param(k, v)
+
+
+def test_validation_parity_on_corrupted_trees(arena, oracle):
+    e, w, ref = oracle_graph(12)
+    g = check_csr(arena, oracle, 1 << 12, e, w)
+    b = arena.bfs_create(g)
+    root = int(oracle.roots(ref)[0])
+    arena.bfs(g, b, root)
+    pred = arena.bfs_pred(b)
+    rng = np.random.default_rng(1)
+    reached = np.flatnonzero(pred >= 0)
+    cases = []
+    for kind in range(7):
+        p = pred.copy()
+        v = int(rng.choice(reached[reached != root]))
+        if kind == 0: p[v] = g.nlocalverts + 9
+        if kind == 1: p[root] = -1
+        if kind == 2: p[v] = -1
+        if kind == 3: p[v] = v
+        if kind == 4: p[v] = int(rng.choice(reached))
+        if kind == 5: p[v] = -7
+        if kind == 6: p[int(rng.choice(np.flatnonzero(pred < 0)))] = root if (pred < 0).any() else root
+        cases.append(p)
+    for p in cases:
+        arena.to_device(b.pred, p)
+        ok, nerr = arena.bfs_validate(g, b, root)
+        ref_err = oracle.bfs_validate(ref, p, root)
+        assert nerr == ref_err and ok == (ref_err == 0)
+        depth = arena.bfs_depth(g, b, root)
+        assert np.array_equal(depth, oracle.dist_from_pred(ref, p, root))
+
+
+def test_bfs_errors_return(arena):
+    import g500
+    e, w, ref = oracle_graph(10)
+    tg = arena.upload_tuples(e, w)
+    g = arena.csr(tg, 1 << 10)
+    b = arena.bfs_create(g)
+    with pytest.raises(g500.LibraryError):
+        arena.bfs(g, b, 1 << 10)
+    with pytest.raises(g500.LibraryError):
+        s = arena.sssp_create(g)
+        arena.sssp(g, s, (1 << 10) + 5)
+
+
+# ----------------------------------------------------------------------- SSSP
+@pytest.mark.parametrize("scale", [10, 12, 16])
+def test_sssp_bitexact(arena, oracle, scale):
+    e, w, ref = oracle_graph(scale)
+    g = check_csr(arena, oracle, 1 << scale, e, w)
+    s = arena.sssp_create(g)
+    for r in oracle.roots(ref)[:4]:
+        r = int(r)
+        d_ref, _ = oracle.dijkstra_f32(ref, r)
+        for delta in (0.1, 0.05, 0.25, 0.003, 4.0):
+            say("sssp", scale, r, delta)
+            s.delta = delta
+            arena.sssp(g, s, r)
+            d, p = arena.sssp_result(s)
+            assert np.array_equal(d.view(np.uint32), d_ref.view(np.uint32))
+            assert oracle.sssp_check(ref, r, d, p) == 0
+            assert arena.edge_count(g, s.pred) == oracle.edge_count(ref, p)
+
+
+def test_sssp_max_error_vs_f64(arena, oracle):
+    e, w, ref = oracle_graph(14)
+    g = check_csr(arena, oracle, 1 << 14, e, w)
+    s = arena.sssp_create(g)
+    r = int(oracle.roots(ref)[0])
+    arena.sssp(g, s, r)
+    d, _ = arena.sssp_result(s)
+    d64 = oracle.dijkstra_f64(ref, r)
+    err = float(np.max(np.abs(d.astype(np.float64) - d64)))
+    say("sssp max |f32 - f64| =", err)
+    assert err < 1e-5
+
+
+# --------------------------------------------------------------------- runner
+def test_runner_small():
+    import g500
+    env = dict(os.environ, G500_ARENA_BYTES=str(8 << 30))
+    r = subprocess.run([g500.RUNNER_PATH, "10", "16"], capture_output=True, timeout=300, env=env)
+    out = r.stdout.decode()
+    say(out[-2000:], r.stderr.decode()[-2000:])
+    assert r.returncode == 0
+    kv = {}
+    for line in out.splitlines():
+        if ":" in line:
+            k, v = line.split(":", 1)
+            kv[k.strip()] = v.replace("!", "").strip()
+    assert kv["SCALE"] == "10" and kv["NBFS"] == "64"
+    assert float(kv["bfs  harmonic_mean_TEPS"]) > 0
+    assert float(kv["sssp median_TEPS"]) > 0
