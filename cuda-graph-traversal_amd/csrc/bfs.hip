@@ -5,7 +5,8 @@
 // The reference runs a top-down level loop in one persistent kernel with one
 // CAS per examined edge. Here each level is either
 //   top-down: frontier queue, one wave per vertex (lanes over its row), and
-//             high-degree "hub" vertices split over the whole grid; a plain
+//             high-degree "hub" vertices in a flattened edge work space whose
+//             fixed-size chunks are spread over the whole grid; a plain
 //             load of the visited bitmap filters most edges before the atomic
 //             OR that claims a vertex (exactly one winner writes pred);
 //   bottom-up: every unvisited vertex scans its row until it meets a frontier
@@ -25,13 +26,14 @@ constexpr int kMaxLevelLog = 256;
 struct BfsCtl {
     GridBarrier bar;
     alignas(256) u32 qlen[3];
-    u32 hlen[3];
+    u64 hcnt[3];   // packed hub counters (count << 38 | padded edge total)
     u64 fcount[3];
     u64 fedges[3];
     u32 levels;
     u32 depth_changed;
     alignas(256) i32 mode_log[kMaxLevelLog];
     u64 size_log[kMaxLevelLog];
+    u64 tick_log[kMaxLevelLog];   // constant-clock ticks of each level seen by block 0
     alignas(256) BlockTrace trace[kMaxTraceBlocks];
 };
 
@@ -54,6 +56,8 @@ struct BfsArgs {
     u64 *fb[2];
     u32 *q[2];
     u32 *hubs[2];
+    u64 *hoff[2];
+    u32 *hlen[2];
     BfsCtl *ctl;
     u64 root;
     f32 alpha, beta;
@@ -66,7 +70,7 @@ __device__ __forceinline__ u64 ld64(const u64 *p) {
 
 // Claims v for parent u when this lane is active; appends winners to the next
 // queue (or the hub queue) with one atomic per wave. All lanes must call it.
-__device__ __forceinline__ void td_visit(const BfsArgs &a, u32 u, u32 v, bool active, u32 *qn, u32 *hn,
+__device__ __forceinline__ void td_visit(const BfsArgs &a, u32 u, u32 v, bool active, u32 *qn, int nb,
                                          BfsCtl *ctl, int nxt, u32 &cnt, u64 &edges) {
     bool won = false;
     if (active) {
@@ -86,8 +90,7 @@ __device__ __forceinline__ void td_visit(const BfsArgs &a, u32 u, u32 v, bool ac
     const bool hub = won && deg > a.hub_degree;
     const u32 s1 = wave_append(&ctl->qlen[nxt], won && !hub);
     if (won && !hub) qn[s1] = v;
-    const u32 s2 = wave_append(&ctl->hlen[nxt], hub);
-    if (hub) hn[s2] = v;
+    hub_append(&ctl->hcnt[nxt], a.hubs[nb], a.hoff[nb], a.hlen[nb], hub, v, deg, a.n, &ctl->bar.error);
 }
 
 __global__ __launch_bounds__(512) void bfs_kernel(BfsArgs a) {
@@ -104,7 +107,9 @@ __global__ __launch_bounds__(512) void bfs_kernel(BfsArgs a) {
         const u32 deg = (u32)(a.rs[r + 1] - a.rs[r]);
         if (deg > a.hub_degree) {
             a.hubs[0][0] = (u32)r;
-            st_relaxed(&ctl->hlen[0], 1u);
+            a.hoff[0][0] = 0;
+            a.hlen[0][0] = deg;
+            st_relaxed(&ctl->hcnt[0], (1ull << kHubCountShift) | ((deg + kHubChunk - 1) / kHubChunk * kHubChunk));
         } else {
             a.q[0][0] = (u32)r;
             st_relaxed(&ctl->qlen[0], 1u);
@@ -116,6 +121,7 @@ __global__ __launch_bounds__(512) void bfs_kernel(BfsArgs a) {
 
     bool bottom_up = false;
     u64 m_explored = 0, prev_nf = 0;
+    u64 t_prev = gtid == 0 ? wall_clock64() : 0;
     for (u32 L = 0;; ++L) {
         const int cur = L % 3, nxt = (L + 1) % 3, clr = (L + 2) % 3;
         const u64 nf = ld64(&ctl->fcount[cur]);
@@ -129,7 +135,7 @@ __global__ __launch_bounds__(512) void bfs_kernel(BfsArgs a) {
         prev_nf = nf;
         if (gtid == 0) {
             st_relaxed(&ctl->qlen[clr], 0u);
-            st_relaxed(&ctl->hlen[clr], 0u);
+            st_relaxed(&ctl->hcnt[clr], 0ull);
             st_relaxed(&ctl->fcount[clr], 0ull);
             st_relaxed(&ctl->fedges[clr], 0ull);
             if (L < kMaxLevelLog) {
@@ -139,13 +145,14 @@ __global__ __launch_bounds__(512) void bfs_kernel(BfsArgs a) {
             st_relaxed(&ctl->levels, L + 1);
         }
         u32 *qc = a.q[L & 1], *qn = a.q[(L + 1) & 1];
-        u32 *hc = a.hubs[L & 1], *hn = a.hubs[(L + 1) & 1];
         u64 *fbc = a.fb[L & 1], *fbn = a.fb[(L + 1) & 1];
 
         if (next_bu && !bottom_up) {  // frontier queue -> bitmap
             for (u64 w = gtid; w < a.nwords; w += gthreads) fbc[w] = 0;
             bfs_sync(ctl, 2, L);
-            const u32 nq = ld_relaxed(&ctl->qlen[cur]), nh = ld_relaxed(&ctl->hlen[cur]);
+            const u32 nq = ld_relaxed(&ctl->qlen[cur]);
+            const u64 nh = hub_count(ld_relaxed(&ctl->hcnt[cur]));
+            const u32 *hc = a.hubs[L & 1];
             for (u64 i = gtid; i < (u64)nq + nh; i += gthreads) {
                 const u32 v = i < nq ? qc[i] : hc[i - nq];
                 atomicOr((unsigned long long *)&fbc[v >> 6], 1ull << (v & 63));
@@ -169,8 +176,8 @@ __global__ __launch_bounds__(512) void bfs_kernel(BfsArgs a) {
                     const bool hub = has && deg > a.hub_degree;
                     const u32 s1 = wave_append(&ctl->qlen[cur], has && !hub);
                     if (has && !hub) qc[s1] = v;
-                    const u32 s2 = wave_append(&ctl->hlen[cur], hub);
-                    if (hub) hc[s2] = v;
+                    hub_append(&ctl->hcnt[cur], a.hubs[L & 1], a.hoff[L & 1], a.hlen[L & 1], hub, v, deg, a.n,
+                               &ctl->bar.error);
                 }
             }
             bfs_sync(ctl, 4, L);
@@ -180,14 +187,25 @@ __global__ __launch_bounds__(512) void bfs_kernel(BfsArgs a) {
         u32 cnt = 0;
         u64 edges = 0;
         if (!bottom_up) {
-            const u32 nq = ld_relaxed(&ctl->qlen[cur]), nh = ld_relaxed(&ctl->hlen[cur]);
-            for (u32 h = 0; h < nh; ++h) {  // hubs: the whole grid walks each row
-                const u32 u = hc[h];
-                const u64 b = a.rs[u], e = a.rs[u + 1];
-                for (u64 k0 = b + wave_gid * kWave; k0 < e; k0 += nwaves * kWave) {
-                    const u64 k = k0 + lane;
+            const u32 nq = ld_relaxed(&ctl->qlen[cur]);
+            const u64 hp = ld_relaxed(&ctl->hcnt[cur]);
+            const u64 nh = hub_count(hp), ht = hub_total(hp);
+            const u32 *hc = a.hubs[L & 1];
+            const u64 *ho = a.hoff[L & 1];
+            const u32 *hl = a.hlen[L & 1];
+            u64 hi = 0;
+            // hubs: chunks of the flattened hub edge space over the whole grid
+            for (u64 c = wave_gid; c * kHubChunk < ht; c += nwaves) {
+                const u64 p = c * kHubChunk;
+                hi = hub_locate(ho, hi, nh, p);
+                const u32 u = hc[hi];
+                const u64 b = a.rs[u] + (p - ho[hi]);
+                const u64 e = a.rs[u] + hl[hi];
+#pragma unroll
+                for (u64 j = 0; j < kHubChunk; j += kWave) {
+                    const u64 k = b + j + lane;
                     const bool act = k < e;
-                    td_visit(a, u, act ? a.col[k] : 0, act, qn, hn, ctl, nxt, cnt, edges);
+                    td_visit(a, u, act ? a.col[k] : 0, act, qn, (int)((L + 1) & 1), ctl, nxt, cnt, edges);
                 }
             }
             for (u64 i = wave_gid; i < nq; i += nwaves) {  // one wave per frontier vertex
@@ -196,7 +214,7 @@ __global__ __launch_bounds__(512) void bfs_kernel(BfsArgs a) {
                 for (u64 k0 = b; k0 < e; k0 += kWave) {
                     const u64 k = k0 + lane;
                     const bool act = k < e;
-                    td_visit(a, u, act ? a.col[k] : 0, act, qn, hn, ctl, nxt, cnt, edges);
+                    td_visit(a, u, act ? a.col[k] : 0, act, qn, (int)((L + 1) & 1), ctl, nxt, cnt, edges);
                 }
             }
         } else {
@@ -231,6 +249,11 @@ __global__ __launch_bounds__(512) void bfs_kernel(BfsArgs a) {
             atomicAdd((unsigned long long *)&ctl->fedges[nxt], (unsigned long long)we);
         }
         bfs_sync(ctl, bottom_up ? 6 : 5, L);
+        if (gtid == 0 && L < kMaxLevelLog) {
+            const u64 t = wall_clock64();
+            st_relaxed(&ctl->tick_log[L], t - t_prev);
+            t_prev = t;
+        }
     }
 }
 
@@ -303,6 +326,8 @@ Bfs_State oned_graph_bfs_create(Worker_State *ws, const Oned_Graph *g) {
         b.frontier_bits[i] = (u64 *)arena_alloc(ws, nw * sizeof(u64), "BFS frontier bitmap");
     }
     b.hubs = (u32 *)arena_alloc(ws, 2 * n * sizeof(u32), "BFS hub queues");
+    b.hub_offsets = (u64 *)arena_alloc(ws, 2 * n * sizeof(u64), "BFS hub offsets");
+    b.hub_lengths = (u32 *)arena_alloc(ws, 2 * n * sizeof(u32), "BFS hub lengths");
     b.ctl = arena_alloc(ws, sizeof(BfsCtl), "BFS control");
     if (failed()) return b;
     oned_graph_bfs_clear(ws, &b);
@@ -340,8 +365,11 @@ void oned_graph_bfs_run(Worker_State *ws, const Oned_Graph *g, Bfs_State *bfs, u
     a.fb[1] = bfs->frontier_bits[1];
     a.q[0] = bfs->frontier[0];
     a.q[1] = bfs->frontier[1];
-    a.hubs[0] = bfs->hubs;
-    a.hubs[1] = bfs->hubs + a.n;
+    for (int i = 0; i < 2; ++i) {
+        a.hubs[i] = bfs->hubs + i * a.n;
+        a.hoff[i] = bfs->hub_offsets + i * a.n;
+        a.hlen[i] = bfs->hub_lengths + i * a.n;
+    }
     a.ctl = (BfsCtl *)bfs->ctl;
     a.root = root;
     a.alpha = (f32)c.bfs_alpha;
@@ -353,7 +381,7 @@ void oned_graph_bfs_run(Worker_State *ws, const Oned_Graph *g, Bfs_State *bfs, u
     void *args[] = {&a};
     G500_HIP(hipMemsetAsync(bfs->ctl, 0, sizeof(BfsCtl), c.stream));
     timer_start();
-    G500_HIP(hipLaunchCooperativeKernel((const void *)bfs_kernel, dim3(grid), dim3(block), args, 0, c.stream));
+    launch_persistent((const void *)bfs_kernel, grid, block, args);
     timer_stop();
     static BfsCtl h;
     g500_copy_to_host(&h, bfs->ctl, sizeof(h));
@@ -364,9 +392,9 @@ void oned_graph_bfs_run(Worker_State *ws, const Oned_Graph *g, Bfs_State *bfs, u
             if (h.trace[b].barriers < lo) { lo = h.trace[b].barriers; at_lo = b; }
             if (h.trace[b].barriers > hi) { hi = h.trace[b].barriers; at_hi = b; }
         }
-        fail("BFS kernel grid barrier timed out; %d blocks; barriers passed min %u (block %d phase %u level %u) "
-             "max %u (block %d phase %u level %u)",
-             grid, lo, at_lo, h.trace[at_lo].phase, h.trace[at_lo].a, hi, at_hi, h.trace[at_hi].phase,
+        fail("BFS kernel error %u (1 barrier timeout, 2 hub queue overflow); %d blocks; barriers passed min %u "
+             "(block %d phase %u level %u) max %u (block %d phase %u level %u)",
+             h.bar.error, grid, lo, at_lo, h.trace[at_lo].phase, h.trace[at_lo].a, hi, at_hi, h.trace[at_hi].phase,
              h.trace[at_hi].a);
     }
 }
@@ -403,13 +431,14 @@ void oned_graph_bfs_destroy(Bfs_State *bfs) {
     if (bfs) memset(bfs, 0, sizeof(*bfs));
 }
 
-i32 g500_bfs_level_log(const Bfs_State *bfs, i32 *modes, u64 *frontier, i32 max_levels) {
+i32 g500_bfs_level_log(const Bfs_State *bfs, i32 *modes, u64 *frontier, f64 *ms, i32 max_levels) {
     static BfsCtl h;
     g500_copy_to_host(&h, bfs->ctl, sizeof(h));
     i32 L = (i32)MIN((u32)h.levels, (u32)kMaxLevelLog);
     for (i32 i = 0; i < L && i < max_levels; ++i) {
         if (modes) modes[i] = h.mode_log[i];
         if (frontier) frontier[i] = h.size_log[i];
+        if (ms) ms[i] = (f64)h.tick_log[i] / ctx().wall_clock_khz;
     }
     return L;
 }

@@ -23,6 +23,7 @@ struct Context {
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     int num_cus = 0;
+    double wall_clock_khz = 100000.0;  // rate of the device constant clock (wall_clock64)
     int error_mode = G500_ERRORS_ABORT;
     int status = 0;
     char msg[1024] = {0};
@@ -34,6 +35,7 @@ struct Context {
     int bfs_block = 512;          // threads per block of the persistent BFS kernel
     int bfs_hub_degree = 2048;    // frontier vertices above this degree are split grid-wide
     int sssp_block = 512;
+    int coop_launch = 1;          // 1: cooperative launch first; 0: plain launch of the co-resident grid
     int gen_blocks = 0;           // 0: one block per 256-edge tile (capped)
 };
 
@@ -63,6 +65,13 @@ inline void sync() { G500_HIP(hipStreamSynchronize(ctx().stream)); }
 
 // Grid size for a cooperative (persistent) kernel: every block co-resident.
 int cooperative_blocks(const void *kernel, int block, size_t dyn_lds);
+
+// Launches a persistent kernel whose grid fits on the device at once. Uses a
+// cooperative launch; if the runtime refuses it (the cooperative queue is held
+// by another process on the same device), launches the same co-resident grid
+// normally. The kernels' grid barriers are bounded, so a grid that is not
+// fully resident ends with an error instead of waiting forever.
+void launch_persistent(const void *kernel, int grid, int block, void **args);
 
 inline unsigned int grid_for(u64 work, int block, u64 cap = 1u << 20) {
     u64 b = (work + block - 1) / block;
@@ -99,9 +108,83 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
+__device__ __forceinline__ u32 wave_min(u32 v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const u32 y = __shfl_xor(v, o);
+        v = y < v ? y : v;
+    }
+    return v;
+}
+
 __device__ __forceinline__ u64 shfl_xor64(u64 v, int m) {
     u32 lo = __shfl_xor((u32)v, m), hi = __shfl_xor((u32)(v >> 32), m);
     return ((u64)hi << 32) | lo;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_scan(T v) {
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        T y = __shfl_up(v, o);
+        if (lane_id() >= o) v += y;
+    }
+    return v;
+}
+
+// ------------------------------------------------------------ hub work lists
+// A hub list holds high-degree vertices together with the offset of each
+// one's edge range in a flattened work space where every range is padded to a
+// multiple of kHubChunk. One packed u64 counter (count << 38 | padded edge
+// total) reserves the slot and the offset with a single atomic, so offsets
+// increase with the slot index and a chunk of the work space maps to exactly
+// one hub by a monotone search.
+constexpr u64 kHubChunk = 256;
+constexpr int kHubCountShift = 38;
+constexpr u64 kHubTotalMask = (1ull << kHubCountShift) - 1;
+
+__device__ __forceinline__ u64 hub_count(u64 packed) { return packed >> kHubCountShift; }
+__device__ __forceinline__ u64 hub_total(u64 packed) { return packed & kHubTotalMask; }
+
+// Wave-converged append of (v, len) for lanes with `want`.
+__device__ __forceinline__ void hub_append(u64 *counter, u32 *hv, u64 *hoff, u32 *hlen, bool want, u32 v,
+                                           u32 len, u64 cap, u32 *error) {
+    const u64 m = ballot(want);
+    if (!m) return;
+    const u64 padded = want ? (u64)(len + kHubChunk - 1) / kHubChunk * kHubChunk : 0;
+    const u64 incl = wave_inclusive_scan(padded);
+    const u64 total = __shfl(incl, kWave - 1);
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    u64 old = 0;
+    if (lane_id() == leader)
+        old = atomicAdd((unsigned long long *)counter, ((u64)__popcll(m) << kHubCountShift) | total);
+    old = __shfl(old, leader);
+    if (want) {
+        const u64 slot = hub_count(old) + mask_rank(m);
+        if (slot >= cap) {
+            atomicOr(error, 2u);
+            return;
+        }
+        hv[slot] = v;
+        hoff[slot] = hub_total(old) + incl - padded;
+        hlen[slot] = len;
+    }
+}
+
+// Largest i in [lo, n) with hoff[i] <= p, searching forward from lo
+// (galloping, then bisection). Requires hoff[lo] <= p.
+__device__ __forceinline__ u64 hub_locate(const u64 *hoff, u64 lo, u64 n, u64 p) {
+    u64 step = 1;
+    while (lo + step < n && hoff[lo + step] <= p) {
+        lo += step;
+        step <<= 1;
+    }
+    u64 hi = lo + step < n ? lo + step : n;
+    while (hi - lo > 1) {
+        const u64 mid = lo + (hi - lo) / 2;
+        if (hoff[mid] <= p) lo = mid;
+        else hi = mid;
+    }
+    return lo;
 }
 
 __device__ __forceinline__ u64 ld_acquire(const u64 *p) {
@@ -152,9 +235,11 @@ __device__ __forceinline__ void grid_sync(GridBarrier *b, u32 *error) {
             __hip_atomic_store(&b->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&b->gen, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
+            // poll with relaxed loads (an acquire load would invalidate this XCD's L2
+            // on every poll); one acquire fence follows the wait
             const u64 t0 = wall_clock64();
-            while (__hip_atomic_load(&b->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
-                __builtin_amdgcn_s_sleep(2);
+            while (__hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                __builtin_amdgcn_s_sleep(1);
                 // ~10 s at the 100 MHz constant clock: only reached if blocks are not
                 // co-resident, which the cooperative launch rules out.
                 if (wall_clock64() - t0 > 1000000000ull) {

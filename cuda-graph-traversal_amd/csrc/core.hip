@@ -54,7 +54,31 @@ int cooperative_blocks(const void *kernel, int block, size_t dyn_lds) {
     return per_cu * ctx().num_cus;
 }
 
+void launch_persistent(const void *kernel, int grid, int block, void **args) {
+    Context &c = ctx();
+    if (c.coop_launch) {
+        hipError_t e = hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(block), args, 0, c.stream);
+        if (e == hipSuccess) return;
+        (void)hipGetLastError();
+        static bool warned = false;
+        if (!warned) {
+            fprintf(stderr, "libg500: cooperative launch unavailable (%s); using plain launches of the "
+                            "co-resident grid\n", hipGetErrorString(e));
+            warned = true;
+        }
+    }
+    G500_HIP(hipLaunchKernel(kernel, dim3(grid), dim3(block), args, 0, c.stream));
+}
+
 void init_mrg_table();  // generator.hip
+
+__global__ __launch_bounds__(256) void copy16_kernel(uint4 *__restrict__ dst, const uint4 *__restrict__ src, u64 n) {
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
+__global__ __launch_bounds__(256) void copy1_kernel(u8 *__restrict__ dst, const u8 *__restrict__ src, u64 n) {
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) dst[i] = src[i];
+}
 
 }  // namespace g500
 
@@ -148,6 +172,9 @@ Worker_State worker_init(i32 argc, char **argv, u64 gpu_memory_size) {
     hipDeviceProp_t prop;
     G500_HIP(hipGetDeviceProperties(&prop, c.device));
     c.num_cus = prop.multiProcessorCount;
+    int wc = 0;
+    if (hipDeviceGetAttribute(&wc, hipDeviceAttributeWallClockRate, c.device) == hipSuccess && wc > 0)
+        c.wall_clock_khz = wc;
     if (!c.stream) {
         G500_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
         G500_HIP(hipEventCreate(&c.ev_start));
@@ -207,6 +234,20 @@ void g500_copy_to_device(void *dst_device, const void *src, u64 bytes) {
     sync();
 }
 
+void g500_copy_device(void *dst_device, const void *src_device, u64 bytes) {
+    Context &c = ctx();
+    if (!bytes) return;
+    timer_start();
+    if (((uintptr_t)dst_device | (uintptr_t)src_device | bytes) % 16 == 0)
+        copy16_kernel<<<grid_for(bytes / 16, 256, (u64)c.num_cus * 16), 256, 0, c.stream>>>(
+            (uint4 *)dst_device, (const uint4 *)src_device, bytes / 16);
+    else
+        copy1_kernel<<<grid_for(bytes, 256, (u64)c.num_cus * 16), 256, 0, c.stream>>>((u8 *)dst_device,
+                                                                                     (const u8 *)src_device, bytes);
+    G500_HIP(hipGetLastError());
+    timer_stop();
+}
+
 i32 g500_set_param(const char *name, f64 value) {
     Context &c = ctx();
     std::string n(name);
@@ -216,6 +257,7 @@ i32 g500_set_param(const char *name, f64 value) {
     else if (n == "bfs_hub_degree") c.bfs_hub_degree = (int)value;
     else if (n == "sssp_block") c.sssp_block = (int)value;
     else if (n == "gen_blocks") c.gen_blocks = (int)value;
+    else if (n == "coop_launch") c.coop_launch = (int)value;
     else return -1;
     return 0;
 }
@@ -229,6 +271,7 @@ i32 g500_get_param(const char *name, f64 *value) {
     else if (n == "bfs_hub_degree") *value = c.bfs_hub_degree;
     else if (n == "sssp_block") *value = c.sssp_block;
     else if (n == "gen_blocks") *value = c.gen_blocks;
+    else if (n == "coop_launch") *value = c.coop_launch;
     else if (n == "num_cus") *value = c.num_cus;
     else return -1;
     return 0;

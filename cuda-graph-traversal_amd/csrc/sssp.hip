@@ -7,42 +7,56 @@
 // lowers a label with a compare-and-swap loop only while the candidate is
 // strictly smaller (the reference's CAS plus per-vertex spin lock,
 // traversal.cu:211-240, becomes one lock-free 64-bit CAS). Every terminating
-// label-correcting schedule reaches the same FP32 fixed point, so the bucket
-// schedule below does not change any distance.
+// label-correcting schedule reaches the same FP32 fixed point, so the schedule
+// below does not change any distance.
 //
-// Buckets are [k*delta, (k+1)*delta) by the index floor(d / delta). Rows are
-// sorted by weight, so the light edges (w < delta) of a row are its first
-// split[u] entries and the heavy edges the rest. Per bucket:
-//   light iterations: frontier vertices relax their light edges; vertices
-//     lowered into the bucket are queued for the next iteration (de-duplicated
-//     per iteration) and logged as bucket members;
-//   heavy pass: every member not yet processed relaxes its heavy edges once;
-//     a heavy candidate that rounds back into the bucket re-opens the light
-//     phase for a new member round;
-//   scan: one pass over all labels collects bucket k+1 and the smallest later
-//     bucket (empty buckets are skipped).
-// Work is mapped by row length: a lane per vertex (<= 32 entries), a wave per
-// vertex (<= 8192), or the whole grid per vertex (longer).
+// Buckets follow the reference's bounds: [lo, hi) with hi = fl(lo + delta)
+// (traversal.cu:693-694). Rows are sorted by weight, so the light edges
+// (w < delta) of a row are its first split[u] entries. Because rounding is
+// monotone, a heavy edge from u (d_u >= lo, w >= delta) yields
+// fl(d_u + w) >= fl(lo + delta) = hi: heavy relaxations never land back in
+// the current bucket. Per bucket:
+//   light steps: queued vertices relax their light edges; vertices lowered
+//     below hi are queued for the next step (de-duplicated per queue);
+//   one fused dense step: every vertex of the bucket relaxes its heavy edges
+//     (short rows inline, longer rows through a wave list and a hub list in a
+//     second phase), and every vertex already in the next bucket
+//     [hi, fl(hi + delta)) is queued, together with those that the heavy
+//     relaxations lower into it;
+//   when the next bucket is empty: a min step finds the smallest remaining
+//     distance and a collect step queues the bucket that holds it.
+// Work is mapped by row-segment length: a lane per vertex (<= 32 entries), a
+// wave per vertex (<= 1024), or chunks of a flattened hub work space spread
+// over the whole grid (longer).
+//
+// Steps and shared counters: step x reads the queue written by step x - 1
+// (slot x % 3), writes only slot (x + 1) % 3 and clears slot (x + 2) % 3, so
+// a block acting on a decision never changes a counter that a slower block
+// still has to read for the same decision.
 #include "common.hpp"
 
 namespace g500 {
 
 constexpr u32 kSmall = 32;
-constexpr u32 kHub = 8192;
-constexpr u64 kInfLabel = (0x7F800000ull << 32) | 0xFFFFFFFFull;
+constexpr u32 kHub = 1024;
+constexpr u32 kInfBits = 0x7F800000u;
+constexpr u64 kInfLabel = ((u64)kInfBits << 32) | 0xFFFFFFFFull;
 
-// Error bits: 1 barrier timeout, 2 member log overflow, 4 queue entry out of
-// range, 8 queue length above n.
+// Error bits: 1 barrier timeout, 2 hub list overflow, 4 queue entry out of
+// range, 8 queue length above n, 16 bucket invariant violated.
 struct SsspCtl {
     GridBarrier bar;
-    alignas(256) u32 flen[3][3];   // [slot][tier] light queue lengths
-    u32 mlen[3];                   // member log lengths per tier
-    u32 nmin[3];                   // smallest bucket index beyond the next, per scan slot
-    alignas(64) u64 buckets, light_iters, improvements;
+    alignas(256) u32 flen[3][2];   // light queue lengths [slot][tier 0-1]
+    u64 fhub[3];                   // light queue hub counters [slot]
+    u32 hlen[3];                   // heavy wave-tier list length [slot]
+    u64 hhub[3];                   // heavy hub-tier counters [slot]
+    u32 nmin[3];                   // smallest distance bits >= hi, per slot
+    alignas(64) u64 buckets, light_iters, improvements, scans;
+    u64 ticks[4];                  // constant-clock ticks of block 0: light, fused, min+collect, kernel
     alignas(256) BlockTrace trace[kMaxTraceBlocks];
 };
 
-enum : u32 { kPhaseInit = 1, kPhaseLight = 2, kPhaseHeavy = 3, kPhaseScan = 4 };
+enum : u32 { kPhaseInit = 1, kPhaseLight = 2, kPhaseFused = 3, kPhaseHeavy = 4, kPhaseMin = 5, kPhaseCollect = 6 };
 
 __device__ __forceinline__ void sync_traced(SsspCtl *ctl, u32 phase, u32 a, u32 b) {
     grid_sync(&ctl->bar, &ctl->bar.error);
@@ -62,22 +76,27 @@ struct SsspArgs {
     const f32 *w;
     const u32 *split;
     u64 *lab;
-    u32 *fq[2][3];   // light queues [buffer][tier]
-    u32 *mq[3];      // member logs [tier]
-    u64 mcap;        // capacity of each member log
-    u32 *stamp, *mstamp;
+    u32 *fq[2][3];   // light queues [buffer][tier]; tier 2 holds hub vertices
+    u64 *fq_off[2];  // hub-tier work offsets per buffer
+    u32 *fq_len[2];  // hub-tier segment lengths per buffer
+    u32 *hq1;        // heavy wave-tier list
+    u32 *hqh;        // heavy hub-tier list
+    u64 *hq_off;
+    u32 *hq_len;
+    u32 *stamp;
     SsspCtl *ctl;
     i64 *pred;
     f32 *dist;
     u64 root;
-    f32 inv_delta;
+    f32 delta;
 };
 
 __device__ __forceinline__ f32 label_dist(u64 l) { return __uint_as_float((u32)(l >> 32)); }
 
-__device__ __forceinline__ u32 bucket_of(f32 d, f32 inv_delta) {
-    const f32 x = d * inv_delta;
-    return x >= 4.0e9f ? 0xFFFFFFF0u : (u32)x;
+// The reference's next bucket bound, fl(hi + delta), forced to progress.
+__device__ __forceinline__ f32 next_bound(f32 hi, f32 delta) {
+    const f32 x = hi + delta;
+    return x > hi ? x : __uint_as_float(__float_as_uint(hi) + 1);
 }
 
 __device__ __forceinline__ int tier_of(u32 len) { return len <= kSmall ? 0 : (len <= kHub ? 1 : 2); }
@@ -95,46 +114,38 @@ __device__ __forceinline__ bool relax(u64 *lab, u32 v, f32 cand, u32 parent) {
     return false;
 }
 
-// Wave-converged: queues v (lanes with `want`) into the light queue `slot` of
-// buffer `buf` under tag `ltag`, and logs it as a member under `mtag`.
-__device__ __forceinline__ void push_vertex(const SsspArgs &a, bool want, u32 v, int buf, int slot, u32 ltag,
-                                            bool log_member, u32 mtag) {
+// Where the pushes of a step go: queue buffer, counter slot and tag of the
+// queue, and the bound below which a lowered vertex is queued.
+struct Target {
+    int buf, slot;
+    u32 tag;
+    f32 below;   // queue v when its new distance is < below
+    f32 floor;   // new distances below this would break the bucket invariant
+};
+
+// Wave-converged: queues v (lanes with `want`) into the light queue of `t`,
+// once per queue (stamp tag), in the work tier of its light segment.
+__device__ __forceinline__ void push_light(const SsspArgs &a, bool want, u32 v, const Target &t) {
     if (!ballot(want)) return;
-    bool q = false, m = false;
-    u32 lw = 0, hw = 0;
+    bool q = false;
+    u32 lw = 0;
     if (want) {
-        q = atomicExch(&a.stamp[v], ltag) != ltag;
-        if (log_member) m = atomicExch(&a.mstamp[v], mtag) != mtag;
-        if (q || m) {
-            const u32 deg = (u32)(a.rs[v + 1] - a.rs[v]);
-            lw = a.split[v];
-            hw = deg - lw;
-        }
+        q = atomicExch(&a.stamp[v], t.tag) != t.tag;
+        if (q) lw = a.split[v];
     }
     SsspCtl *ctl = a.ctl;
 #pragma unroll
-    for (int t = 0; t < 3; ++t) {
-        const bool qt = q && tier_of(lw) == t;
-        const u32 s = wave_append(&ctl->flen[slot][t], qt);
-        if (qt) a.fq[buf][t][s] = v;
-        const bool mt = m && tier_of(hw) == t;
-        const u32 s2 = wave_append(&ctl->mlen[t], mt);
-        if (mt) {
-            if (s2 < a.mcap) a.mq[t][s2] = v;
-            else atomicOr(&ctl->bar.error, 2u);
-        }
+    for (int tier = 0; tier < 2; ++tier) {
+        const bool qt = q && tier_of(lw) == tier;
+        const u32 s = wave_append(&ctl->flen[t.slot][tier], qt);
+        if (qt) a.fq[t.buf][tier][s] = v;
     }
+    hub_append(&ctl->fhub[t.slot], a.fq[t.buf][2], a.fq_off[t.buf], a.fq_len[t.buf], q && tier_of(lw) == 2, v, lw,
+               a.n, &ctl->bar.error);
 }
 
-struct Visit {
-    int buf, slot;     // where pushes go
-    u32 ltag, mtag;
-    bool log_member;
-    u32 bucket;        // current bucket: pushes only for candidates landing in it or below
-};
-
 // One relaxation per active lane, then the wave-converged pushes.
-__device__ __forceinline__ void visit_edge(const SsspArgs &a, const Visit &vs, bool act, u32 u, f32 du, u64 k,
+__device__ __forceinline__ void visit_edge(const SsspArgs &a, const Target &t, bool act, u32 u, f32 du, u64 k,
                                            u64 &improved) {
     bool push = false;
     u32 v = 0;
@@ -143,190 +154,284 @@ __device__ __forceinline__ void visit_edge(const SsspArgs &a, const Visit &vs, b
         const f32 cand = du + a.w[k];
         if (relax(a.lab, v, cand, u)) {
             ++improved;
-            push = bucket_of(cand, a.inv_delta) <= vs.bucket;
+            push = cand < t.below;
+            if (cand < t.floor) atomicOr(&a.ctl->bar.error, 16u);
         }
     }
-    push_vertex(a, push, v, vs.buf, vs.slot, vs.ltag, vs.log_member, vs.mtag);
+    push_light(a, push, v, t);
 }
 
-// Processes the vertices list[begin, end) of one work tier; light selects the
-// row prefix [0, split) or the suffix [split, deg).
+// Segment of u's row: the light prefix or the heavy suffix.
 template <bool LIGHT>
-__device__ void process_tier(const SsspArgs &a, const u32 *list, u32 begin, u32 end, int tier, const Visit &vs,
+__device__ __forceinline__ void segment(const SsspArgs &a, u32 u, u64 &b, u64 &e) {
+    const u64 r0 = a.rs[u], s = a.split[u];
+    b = LIGHT ? r0 : r0 + s;
+    e = LIGHT ? r0 + s : a.rs[u + 1];
+}
+
+// Vertices list[0, count) of work tier 0 (a lane each) or 1 (a wave each).
+template <bool LIGHT>
+__device__ void process_tier(const SsspArgs &a, const u32 *list, u32 count, int tier, const Target &t,
                              u64 &improved) {
     const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
     const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
     if (tier == 0) {
-        for (u64 i0 = begin + wave_gid * kWave; i0 < end; i0 += nwaves * kWave) {
+        for (u64 i0 = wave_gid * kWave; i0 < count; i0 += nwaves * kWave) {
             const u64 i = i0 + lane;
             u64 b = 0, e = 0;
             u32 u = 0;
             f32 du = 0.f;
-            if (i < end && list[i] < a.n) {
+            if (i < count) {
                 u = list[i];
-                const u64 r0 = a.rs[u], s = a.split[u];
-                b = LIGHT ? r0 : r0 + s;
-                e = LIGHT ? r0 + s : a.rs[u + 1];
-                du = label_dist(a.lab[u]);
+                if (u < a.n) {
+                    segment<LIGHT>(a, u, b, e);
+                    du = label_dist(a.lab[u]);
+                } else {
+                    atomicOr(&a.ctl->bar.error, 4u);
+                }
             }
-            u32 len = (u32)(e - b), mx = len;
+            const u32 len = (u32)(e - b);
+            u32 mx = len;
             for (int o = 32; o > 0; o >>= 1) mx = max(mx, (u32)__shfl_xor((int)mx, o));
-            for (u32 j = 0; j < mx; ++j) visit_edge(a, vs, j < len, u, du, b + j, improved);
-        }
-    } else if (tier == 1) {
-        for (u64 i = begin + wave_gid; i < end; i += nwaves) {
-            const u32 u = list[i];
-            if (u >= a.n) {
-                atomicOr(&a.ctl->bar.error, 4u);
-                continue;
-            }
-            const u64 r0 = a.rs[u], s = a.split[u];
-            const u64 b = LIGHT ? r0 : r0 + s, e = LIGHT ? r0 + s : a.rs[u + 1];
-            const f32 du = label_dist(a.lab[u]);
-            for (u64 k0 = b; k0 < e; k0 += kWave) visit_edge(a, vs, k0 + lane < e, u, du, k0 + lane, improved);
+            for (u32 j = 0; j < mx; ++j) visit_edge(a, t, j < len, u, du, b + j, improved);
         }
     } else {
-        for (u64 i = begin; i < end; ++i) {
+        for (u64 i = wave_gid; i < count; i += nwaves) {
             const u32 u = list[i];
             if (u >= a.n) {
                 atomicOr(&a.ctl->bar.error, 4u);
                 continue;
             }
-            const u64 r0 = a.rs[u], s = a.split[u];
-            const u64 b = LIGHT ? r0 : r0 + s, e = LIGHT ? r0 + s : a.rs[u + 1];
+            u64 b, e;
+            segment<LIGHT>(a, u, b, e);
             const f32 du = label_dist(a.lab[u]);
-            for (u64 k0 = b + wave_gid * kWave; k0 < e; k0 += nwaves * kWave)
-                visit_edge(a, vs, k0 + lane < e, u, du, k0 + lane, improved);
+            for (u64 k0 = b; k0 < e; k0 += kWave) visit_edge(a, t, k0 + lane < e, u, du, k0 + lane, improved);
         }
     }
 }
 
-// Control flow is a sequence of steps (light iterations, heavy passes, scans).
-// Step `st` reads its queue and its decision counters from slot st % 3,
-// appends only into slot (st + 1) % 3 (queue buffer (st + 1) & 1, tag
-// st + 2) and clears slot (st + 2) % 3, so a block that already acts on the
-// decision of step st never changes a counter that a slower block has still to
-// read for the same decision.
+// Hub tier: the flattened work space [0, total) of hubs [0, count); each wave
+// takes kHubChunk-edge chunks, so a long row is spread over the whole grid.
+template <bool LIGHT>
+__device__ void process_hubs(const SsspArgs &a, const u32 *hv, const u64 *hoff, const u32 *hlen, u64 packed,
+                             const Target &t, u64 &improved) {
+    const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
+    const u32 lane = lane_id();
+    const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
+    const u64 count = hub_count(packed), total = hub_total(packed);
+    u64 i = 0;
+    for (u64 c = wave_gid; c * kHubChunk < total; c += nwaves) {
+        const u64 p = c * kHubChunk;
+        i = hub_locate(hoff, i, count, p);
+        const u32 u = hv[i];
+        if (u >= a.n) {
+            atomicOr(&a.ctl->bar.error, 4u);
+            continue;
+        }
+        const u64 local = p - hoff[i];
+        const u32 len = hlen[i];
+        u64 b, e;
+        segment<LIGHT>(a, u, b, e);
+        const f32 du = label_dist(a.lab[u]);
+#pragma unroll
+        for (u64 j = 0; j < kHubChunk; j += kWave) {
+            const u64 off = local + j + lane;
+            visit_edge(a, t, off < len, u, du, b + off, improved);
+        }
+    }
+}
+
 __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
     SsspCtl *ctl = a.ctl;
     const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
+    const u64 nround = (a.n + gthreads - 1) / gthreads * gthreads;  // uniform trip count of dense passes
     u64 improved = 0;
+    u32 x = 0;                       // step
+    f32 lo = 0.0f, hi = next_bound(0.0f, a.delta);
 
-    u32 st = 0;      // step counter
-    u32 mt = 1;      // member round tag
-    u32 mproc[3] = {0, 0, 0};
-    u32 k = 0;       // current bucket
-
-    if (gtid == 0) {
-        const u32 r = (u32)a.root;
-        st_relaxed(&a.lab[r], (u64)r);  // distance 0.0f, parent = root
+    const bool timer = gtid == 0;
+    u64 t_prev = timer ? wall_clock64() : 0, t_start = t_prev, t_acc[3] = {0, 0, 0};
+    auto tick = [&](int phase) {
+        if (timer) {
+            const u64 t = wall_clock64();
+            t_acc[phase] += t - t_prev;
+            t_prev = t;
+        }
+    };
+    auto clear_ahead = [&](u32 step) {
+        if (gtid == 0) {
+            const int s = (int)((step + 2) % 3);
+            st_relaxed(&ctl->flen[s][0], 0u);
+            st_relaxed(&ctl->flen[s][1], 0u);
+            st_relaxed(&ctl->fhub[s], 0ull);
+            st_relaxed(&ctl->hlen[s], 0u);
+            st_relaxed(&ctl->hhub[s], 0ull);
+            st_relaxed(&ctl->nmin[s], 0xFFFFFFFFu);
+        }
+    };
+    // Light queue of step `step`: counts per tier and the packed hub counter.
+    auto queue_size = [&](u32 step, u32 *cnt, u64 *hub) {
+        u64 total = 0;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) st_relaxed(&ctl->nmin[i], 0xFFFFFFFFu);
-    }
-    sync_traced(ctl, kPhaseInit, 0, 0);
-    // step 0 queues the root (lane 0 of the first wave) into the list of step 1
-    if (gtid < kWave) push_vertex(a, lane == 0, (u32)a.root, 1, 1, 2, true, mt);
-    sync_traced(ctl, kPhaseInit, 1, 0);
-    st = 1;
-
-    auto slot_total = [&](u32 step, u32 *cnt) {
-        u32 total = 0;
-#pragma unroll
-        for (int t = 0; t < 3; ++t) {
+        for (int t = 0; t < 2; ++t) {
             u32 c = ld_relaxed(&ctl->flen[step % 3][t]);
             if (c > a.n) {
                 if (gtid == 0) atomicOr(&ctl->bar.error, 8u);
                 c = 0;
             }
-            if (cnt) cnt[t] = c;
+            cnt[t] = c;
             total += c;
         }
-        return total;
+        *hub = ld_relaxed(&ctl->fhub[step % 3]);
+        return total + hub_count(*hub);
     };
-    // Every step clears the queue counters of step + 2 and the scan minimum of
-    // step + 1: both were last read right after the barrier that closed step - 1.
-    auto clear_ahead = [&](u32 step) {
-        if (gtid == 0) {
-#pragma unroll
-            for (int t = 0; t < 3; ++t) st_relaxed(&ctl->flen[(step + 2) % 3][t], 0u);
-            st_relaxed(&ctl->nmin[(step + 1) % 3], 0xFFFFFFFFu);
-        }
+    auto target = [&](u32 step, f32 below, f32 floor) {
+        return Target{(int)((step + 1) & 1), (int)((step + 1) % 3), step + 2, below, floor};
     };
 
-    for (;;) {  // buckets
-        for (;;) {  // member rounds of bucket k
-            for (;;) {  // light iterations
-                u32 cnt[3];
-                const u32 total = slot_total(st, cnt);
-                clear_ahead(st);
-                if (total == 0 || ld_relaxed(&ctl->bar.error)) break;
-                if (gtid == 0) atomicAdd((unsigned long long *)&ctl->light_iters, 1ull);
-                const Visit vs{(int)((st + 1) & 1), (int)((st + 1) % 3), st + 2, mt, true, k};
+    if (gtid == 0) {
+        st_relaxed(&a.lab[(u32)a.root], (u64)(u32)a.root);  // distance 0.0f, parent = root
+        for (int i = 0; i < 3; ++i) st_relaxed(&ctl->nmin[i], 0xFFFFFFFFu);
+    }
+    sync_traced(ctl, kPhaseInit, 0, 0);
+    if (gtid < kWave) push_light(a, lane == 0, (u32)a.root, target(0, 0.f, 0.f));  // step 0 queues the root
+    sync_traced(ctl, kPhaseInit, 1, 0);
+    x = 1;
+
+    for (;;) {  // buckets [lo, hi)
+        for (;;) {  // light steps
+            u32 cnt[2];
+            u64 hub;
+            const u64 total = queue_size(x, cnt, &hub);
+            clear_ahead(x);
+            if (total == 0 || ld_relaxed(&ctl->bar.error)) break;
+            if (gtid == 0) atomicAdd((unsigned long long *)&ctl->light_iters, 1ull);
+            const Target t = target(x, hi, lo);
 #pragma unroll
-                for (int t = 0; t < 3; ++t) process_tier<true>(a, a.fq[st & 1][t], 0, cnt[t], t, vs, improved);
-                sync_traced(ctl, kPhaseLight, k, st);
-                ++st;
-            }
-            if (ld_relaxed(&ctl->bar.error)) break;
-            // heavy pass (same step: its input queue was empty) over the
-            // unprocessed member suffix; re-opened vertices go to the next step's
-            // queue under a new member round
-            u32 mend[3];
-#pragma unroll
-            for (int t = 0; t < 3; ++t) mend[t] = min(ld_relaxed(&ctl->mlen[t]), (u32)a.mcap);
-            const Visit vs{(int)((st + 1) & 1), (int)((st + 1) % 3), st + 2, mt + 1, true, k};
-#pragma unroll
-            for (int t = 0; t < 3; ++t) process_tier<false>(a, a.mq[t], mproc[t], mend[t], t, vs, improved);
-#pragma unroll
-            for (int t = 0; t < 3; ++t) mproc[t] = mend[t];
-            sync_traced(ctl, kPhaseHeavy, k, st);
-            ++st;
-            ++mt;
-            if (!slot_total(st, nullptr) || ld_relaxed(&ctl->bar.error)) break;
+            for (int tier = 0; tier < 2; ++tier) process_tier<true>(a, a.fq[x & 1][tier], cnt[tier], tier, t, improved);
+            process_hubs<true>(a, a.fq[x & 1][2], a.fq_off[x & 1], a.fq_len[x & 1], hub, t, improved);
+            sync_traced(ctl, kPhaseLight, __float_as_uint(lo), x);
+            tick(0);
+            ++x;
         }
         if (ld_relaxed(&ctl->bar.error)) break;
+        // fused step x (its light queue was empty): heavy edges of [lo, hi) and
+        // the next bucket [hi, hi2) queued
         if (gtid == 0) atomicAdd((unsigned long long *)&ctl->buckets, 1ull);
-        // scan steps until the next non-empty bucket is queued
-        bool done = false;
-        for (;;) {
-            clear_ahead(st);
-            const int ss = (int)(st % 3);
-            const u64 nround = (a.n + gthreads - 1) / gthreads * gthreads;
+        const f32 hi2 = next_bound(hi, a.delta);
+        {
+            const Target t = target(x, hi2, hi);
+            const int hs = (int)((x + 1) % 3);
+            for (u64 v = gtid; v < nround; v += gthreads) {
+                bool mine = false, next = false;
+                u64 b = 0, e = 0;
+                f32 dv = 0.f;
+                if (v < a.n) {
+                    const u32 bits = (u32)(a.lab[v] >> 32);
+                    if (bits < kInfBits) {
+                        const f32 d = __uint_as_float(bits);
+                        if (d >= lo && d < hi) {
+                            mine = true;
+                            dv = d;
+                            segment<false>(a, (u32)v, b, e);
+                        } else if (d >= hi && d < hi2) {
+                            next = true;
+                        }
+                    }
+                }
+                push_light(a, next, (u32)v, t);
+                const u32 len = (u32)(e - b);
+                const bool medium = mine && len > kSmall && len <= kHub;
+                const u32 s1 = wave_append(&ctl->hlen[hs], medium);
+                if (medium) a.hq1[s1] = (u32)v;
+                hub_append(&ctl->hhub[hs], a.hqh, a.hq_off, a.hq_len, mine && len > kHub, (u32)v, len, a.n,
+                           &ctl->bar.error);
+                const u32 slen = mine && len <= kSmall ? len : 0;
+                u32 mx = slen;
+                for (int o = 32; o > 0; o >>= 1) mx = max(mx, (u32)__shfl_xor((int)mx, o));
+                for (u32 j = 0; j < mx; ++j) visit_edge(a, t, j < slen, (u32)v, dv, b + j, improved);
+            }
+            sync_traced(ctl, kPhaseFused, __float_as_uint(lo), x);
+            const u32 nmed = ld_relaxed(&ctl->hlen[hs]);
+            const u64 hh = ld_relaxed(&ctl->hhub[hs]);
+            if (nmed || hub_count(hh)) {  // second phase of the same step: longer heavy rows
+                process_tier<false>(a, a.hq1, min(nmed, (u32)a.n), 1, t, improved);
+                process_hubs<false>(a, a.hqh, a.hq_off, a.hq_len, hh, t, improved);
+                sync_traced(ctl, kPhaseHeavy, __float_as_uint(lo), x);
+            }
+            tick(1);
+            if (gtid == 0) atomicAdd((unsigned long long *)&ctl->scans, 1ull);
+            ++x;
+        }
+        {
+            u32 cnt[2];
+            u64 hub;
+            if (queue_size(x, cnt, &hub)) {  // next bucket is not empty
+                lo = hi;
+                hi = hi2;
+                continue;
+            }
+        }
+        if (ld_relaxed(&ctl->bar.error)) break;
+        // min step: smallest distance >= hi among all labels (nothing changes now)
+        clear_ahead(x);
+        for (u64 v = gtid; v < nround; v += gthreads) {
+            u32 bits = 0xFFFFFFFFu;
+            if (v < a.n) {
+                const u32 lb = (u32)(a.lab[v] >> 32);
+                if (lb < kInfBits && __uint_as_float(lb) >= hi) bits = lb;
+            }
+            bits = wave_min(bits);
+            if (lane == 0 && bits != 0xFFFFFFFFu) atomicMin(&ctl->nmin[(x + 1) % 3], bits);
+        }
+        sync_traced(ctl, kPhaseMin, __float_as_uint(lo), x);
+        ++x;
+        const u32 nm = ld_relaxed(&ctl->nmin[x % 3]);
+        if (gtid == 0) atomicAdd((unsigned long long *)&ctl->scans, 1ull);
+        if (nm == 0xFFFFFFFFu) {
+            tick(2);
+            break;
+        }
+        const f32 dmin = __uint_as_float(nm);
+        lo = hi;
+        hi = next_bound(lo, a.delta);
+        while (dmin >= hi) {
+            lo = hi;
+            hi = next_bound(lo, a.delta);
+        }
+        // collect step: queue the bucket [lo, hi) that holds dmin
+        clear_ahead(x);
+        {
+            const Target t = target(x, hi, lo);
             for (u64 v = gtid; v < nround; v += gthreads) {
                 bool want = false;
                 if (v < a.n) {
-                    const u64 l = a.lab[v];
-                    if ((u32)(l >> 32) < 0x7F800000u) {
-                        const u32 bk = bucket_of(label_dist(l), a.inv_delta);
-                        if (bk == k + 1) want = true;
-                        else if (bk > k + 1) atomicMin(&ctl->nmin[ss], bk);
+                    const u32 lb = (u32)(a.lab[v] >> 32);
+                    if (lb < kInfBits) {
+                        const f32 d = __uint_as_float(lb);
+                        want = d >= lo && d < hi;
                     }
                 }
-                push_vertex(a, want, (u32)v, (int)((st + 1) & 1), (int)((st + 1) % 3), st + 2, true, mt);
+                push_light(a, want, (u32)v, t);
             }
-            sync_traced(ctl, kPhaseScan, k, st);
-            const u32 nm = ld_relaxed(&ctl->nmin[ss]);
-            ++st;
-            if (slot_total(st, nullptr)) {
-                ++k;
-                break;
-            }
-            if (nm == 0xFFFFFFFFu || ld_relaxed(&ctl->bar.error)) {
-                done = true;
-                break;
-            }
-            k = nm - 1;
         }
-        if (done) break;
+        sync_traced(ctl, kPhaseCollect, __float_as_uint(lo), x);
+        if (gtid == 0) atomicAdd((unsigned long long *)&ctl->scans, 1ull);
+        tick(2);
+        ++x;
     }
     improved = wave_sum(improved);
     if (lane == 0 && improved) atomicAdd((unsigned long long *)&ctl->improvements, (unsigned long long)improved);
+    if (timer) {
+        for (int i = 0; i < 3; ++i) st_relaxed(&ctl->ticks[i], t_acc[i]);
+        st_relaxed(&ctl->ticks[3], wall_clock64() - t_start);
+    }
     // export (traversal.cu:333-340): unreached -> -1.0f / -1
     for (u64 v = gtid; v < a.n; v += gthreads) {
         const u64 l = a.lab[v];
-        const bool reached = (u32)(l >> 32) < 0x7F800000u;
+        const bool reached = (u32)(l >> 32) < kInfBits;
         a.dist[v] = reached ? label_dist(l) : -1.0f;
         a.pred[v] = reached ? (i64)(u32)l : -1;
     }
@@ -346,12 +451,10 @@ __global__ __launch_bounds__(256) void sssp_split_kernel(const u64 *__restrict__
     }
 }
 
-__global__ __launch_bounds__(256) void sssp_clear_kernel(u64 *__restrict__ lab, u32 *__restrict__ stamp,
-                                                         u32 *__restrict__ mstamp, u64 n) {
+__global__ __launch_bounds__(256) void sssp_clear_kernel(u64 *__restrict__ lab, u32 *__restrict__ stamp, u64 n) {
     for (u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (u64)gridDim.x * blockDim.x) {
         lab[v] = kInfLabel;
         stamp[v] = 0;
-        mstamp[v] = 0;
     }
 }
 
@@ -371,10 +474,11 @@ Sssp_State oned_graph_sssp_create(Worker_State *ws, const Oned_Graph *g) {
     s.dist = (f32 *)arena_alloc(ws, n * sizeof(f32), "SSSP dist");
     s.dist_pred = (u64 *)arena_alloc(ws, n * sizeof(u64), "SSSP labels");
     s.frontier = (u32 *)arena_alloc(ws, 6 * n * sizeof(u32), "SSSP queues");
-    s.members = (u32 *)arena_alloc(ws, 6 * n * sizeof(u32), "SSSP member logs");
+    s.heavy = (u32 *)arena_alloc(ws, 2 * n * sizeof(u32), "SSSP heavy lists");
+    s.hub_offsets = (u64 *)arena_alloc(ws, 3 * n * sizeof(u64), "SSSP hub offsets");
+    s.hub_lengths = (u32 *)arena_alloc(ws, 3 * n * sizeof(u32), "SSSP hub lengths");
     s.split = (u32 *)arena_alloc(ws, n * sizeof(u32), "SSSP light split");
     s.stamp = (u32 *)arena_alloc(ws, n * sizeof(u32), "SSSP stamps");
-    s.mstamp = (u32 *)arena_alloc(ws, n * sizeof(u32), "SSSP member stamps");
     s.ctl = arena_alloc(ws, sizeof(SsspCtl), "SSSP control");
     if (failed()) return s;
     oned_graph_sssp_clear(ws, &s);
@@ -385,7 +489,7 @@ void oned_graph_sssp_clear(Worker_State *ws, Sssp_State *s) {
     (void)ws;
     Context &c = ctx();
     const u64 n = s->max_nlocalverts;
-    sssp_clear_kernel<<<grid_for(n, 256, (u64)c.num_cus * 32), 256, 0, c.stream>>>(s->dist_pred, s->stamp, s->mstamp, n);
+    sssp_clear_kernel<<<grid_for(n, 256, (u64)c.num_cus * 32), 256, 0, c.stream>>>(s->dist_pred, s->stamp, n);
     G500_HIP(hipGetLastError());
     G500_HIP(hipMemsetAsync(s->ctl, 0, sizeof(SsspCtl), c.stream));
     sync();
@@ -399,8 +503,8 @@ void oned_graph_sssp_run(Worker_State *ws, const Oned_Graph *g, Sssp_State *s, u
              (unsigned long long)g->nglobalverts);
         return;
     }
-    if (!(s->delta > 0.0f)) {
-        fail("SSSP delta must be positive (got %g)", (double)s->delta);
+    if (!(s->delta > 0.0f) || !(s->delta < 3.0e38f)) {
+        fail("SSSP delta must be positive and finite (got %g)", (double)s->delta);
         return;
     }
     const u64 n = g->nlocalverts;
@@ -417,24 +521,28 @@ void oned_graph_sssp_run(Worker_State *ws, const Oned_Graph *g, Sssp_State *s, u
     a.w = g->weights;
     a.split = s->split;
     a.lab = s->dist_pred;
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < 2; ++b) {
         for (int t = 0; t < 3; ++t) a.fq[b][t] = s->frontier + (u64)(b * 3 + t) * n;
-    a.mcap = 2 * n;
-    for (int t = 0; t < 3; ++t) a.mq[t] = s->members + (u64)t * 2 * n;
+        a.fq_off[b] = s->hub_offsets + (u64)b * n;
+        a.fq_len[b] = s->hub_lengths + (u64)b * n;
+    }
+    a.hq1 = s->heavy;
+    a.hqh = s->heavy + n;
+    a.hq_off = s->hub_offsets + 2 * n;
+    a.hq_len = s->hub_lengths + 2 * n;
     a.stamp = s->stamp;
-    a.mstamp = s->mstamp;
     a.ctl = (SsspCtl *)s->ctl;
     a.pred = s->pred;
     a.dist = s->dist;
     a.root = root;
-    a.inv_delta = 1.0f / s->delta;
+    a.delta = s->delta;
     const int block = 512;
     int grid = cooperative_blocks((const void *)sssp_kernel, block, 0);
     if (failed()) return;
     void *args[] = {&a};
     G500_HIP(hipMemsetAsync(s->ctl, 0, sizeof(SsspCtl), c.stream));
     timer_start();
-    G500_HIP(hipLaunchCooperativeKernel((const void *)sssp_kernel, dim3(grid), dim3(block), args, 0, c.stream));
+    launch_persistent((const void *)sssp_kernel, grid, block, args);
     timer_stop();
     static SsspCtl h;
     g500_copy_to_host(&h, s->ctl, sizeof(h));
@@ -446,9 +554,9 @@ void oned_graph_sssp_run(Worker_State *ws, const Oned_Graph *g, Sssp_State *s, u
             if (h.trace[b].barriers > hi) { hi = h.trace[b].barriers; at_hi = b; }
         }
         const BlockTrace &tl = h.trace[at_lo], &th = h.trace[at_hi];
-        fail("SSSP kernel error %u (1 barrier timeout, 2 member log overflow, 4 bad queue entry, 8 queue "
-             "overflow); %d blocks; barriers passed min %u (block %d phase %u tags %u/%u) max %u (block %d "
-             "phase %u tags %u/%u)",
+        fail("SSSP kernel error %u (1 barrier timeout, 2 hub list overflow, 4 bad queue entry, 8 queue overflow, "
+             "16 bucket invariant); %d blocks; barriers passed min %u (block %d phase %u tags %u/%u) max %u "
+             "(block %d phase %u tags %u/%u)",
              h.bar.error, grid, lo, at_lo, tl.phase, tl.a, tl.b, hi, at_hi, th.phase, th.a, th.b);
     }
 }
@@ -463,6 +571,14 @@ void g500_sssp_run_stats(const Sssp_State *s, u64 *buckets, u64 *light_iteration
     if (buckets) *buckets = h.buckets;
     if (light_iterations) *light_iterations = h.light_iters;
     if (relaxations) *relaxations = h.improvements;
+}
+
+void g500_sssp_phase_ms(const Sssp_State *s, f64 *ms) {
+    static SsspCtl h;
+    g500_copy_to_host(&h, s->ctl, sizeof(h));
+    const f64 khz = ctx().wall_clock_khz;
+    for (int i = 0; i < 4; ++i) ms[i] = (f64)h.ticks[i] / khz;
+    ms[4] = (f64)h.scans;
 }
 
 }  // extern "C"

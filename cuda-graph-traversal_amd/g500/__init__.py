@@ -44,13 +44,14 @@ class OnedGraph(C.Structure):
 
 class BfsState(C.Structure):
     _fields_ = [("pred", vp), ("dist", vp), ("frontier", vp * 2), ("frontier_bits", vp * 2),
-                ("visited", vp), ("hubs", vp), ("ctl", vp), ("max_nlocalverts", u64)]
+                ("visited", vp), ("hubs", vp), ("hub_offsets", vp), ("hub_lengths", vp), ("ctl", vp),
+                ("max_nlocalverts", u64)]
 
 
 class SsspState(C.Structure):
-    _fields_ = [("pred", vp), ("dist", vp), ("dist_pred", vp), ("frontier", vp), ("members", vp),
-                ("split", vp), ("stamp", vp), ("mstamp", vp), ("ctl", vp), ("delta", f32),
-                ("split_delta", f32), ("max_nlocalverts", u64)]
+    _fields_ = [("pred", vp), ("dist", vp), ("dist_pred", vp), ("frontier", vp), ("heavy", vp),
+                ("hub_offsets", vp), ("hub_lengths", vp), ("split", vp), ("stamp", vp), ("ctl", vp),
+                ("delta", f32), ("split_delta", f32), ("max_nlocalverts", u64)]
 
 
 P = C.POINTER
@@ -100,10 +101,12 @@ _SIGS = {
     "g500_last_run_ms": (f64, []),
     "g500_copy_to_host": (None, [vp, vp, u64]),
     "g500_copy_to_device": (None, [vp, vp, u64]),
+    "g500_copy_device": (None, [vp, vp, u64]),
     "g500_set_param": (i32, [C.c_char_p, f64]),
     "g500_get_param": (i32, [C.c_char_p, P(f64)]),
-    "g500_bfs_level_log": (i32, [P(BfsState), P(i32), P(u64), i32]),
+    "g500_bfs_level_log": (i32, [P(BfsState), P(i32), P(u64), P(f64), i32]),
     "g500_sssp_run_stats": (None, [P(SsspState), P(u64), P(u64), P(u64)]),
+    "g500_sssp_phase_ms": (None, [P(SsspState), P(f64)]),
 }
 
 _LIB = None
@@ -252,11 +255,19 @@ class Session:
         self.check()
         return bool(ok), err.value
 
-    def level_log(self, b, max_levels=256):
+    def level_log(self, b, max_levels=256, with_ms=False):
         modes = (i32 * max_levels)()
         sizes = (u64 * max_levels)()
-        n = self.lib.g500_bfs_level_log(C.byref(b), modes, sizes, max_levels)
+        ms = (f64 * max_levels)()
+        n = self.lib.g500_bfs_level_log(C.byref(b), modes, sizes, ms, max_levels)
+        if with_ms:
+            return list(modes[:n]), list(sizes[:n]), list(ms[:n])
         return list(modes[:n]), list(sizes[:n])
+
+    def sssp_phase_ms(self, s):
+        out = (f64 * 5)()
+        self.lib.g500_sssp_phase_ms(C.byref(s), out)
+        return list(out)
 
     def edge_count(self, g, pred_dev):
         c = self.lib.oned_graph_edge_count(C.byref(self.ws), C.byref(g), pred_dev)

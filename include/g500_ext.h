@@ -50,18 +50,27 @@ f64 g500_last_run_ms(void);
 /* Blocking copies between host memory and the library's device memory. */
 void g500_copy_to_host(void *dst, const void *src_device, u64 bytes);
 void g500_copy_to_device(void *dst_device, const void *src, u64 bytes);
+/* Device-to-device copy with the library's own streaming kernel (16-byte
+ * vectors when both pointers and the size allow it); also the reference
+ * workload for calibrating HBM byte counters. */
+void g500_copy_device(void *dst_device, const void *src_device, u64 bytes);
 
 /* Tuning knobs (names: "bfs_alpha", "bfs_beta", "bfs_block", "bfs_hub_degree",
- * "sssp_block", "gen_blocks"). Returns 0 on success, -1 for an unknown name. */
+ * "sssp_block", "gen_blocks", "coop_launch"; read-only "num_cus"). Returns 0
+ * on success, -1 for an unknown name. */
 i32 g500_set_param(const char *name, f64 value);
 i32 g500_get_param(const char *name, f64 *value);
 
-/* Per-level log of the last BFS run: mode (0 top-down, 1 bottom-up) and
- * frontier size of each level. Returns the number of levels. */
-i32 g500_bfs_level_log(const Bfs_State *bfs, i32 *modes, u64 *frontier, i32 max_levels);
-/* Bucket / light-iteration counts of the last SSSP run. */
+/* Per-level log of the last BFS run: mode (0 top-down, 1 bottom-up),
+ * frontier size and device milliseconds of each level (any pointer may be
+ * NULL). Returns the number of levels. */
+i32 g500_bfs_level_log(const Bfs_State *bfs, i32 *modes, u64 *frontier, f64 *ms, i32 max_levels);
+/* Bucket / light-iteration / label-improvement counts of the last SSSP run. */
 void g500_sssp_run_stats(const Sssp_State *sssp, u64 *buckets, u64 *light_iterations,
                          u64 *relaxations);
+/* Device milliseconds of the last SSSP run spent in light iterations,
+ * heavy passes, bucket scans and the whole kernel; ms[4] = number of scans. */
+void g500_sssp_phase_ms(const Sssp_State *sssp, f64 ms[5]);
 
 #ifdef __cplusplus
 }

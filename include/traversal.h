@@ -51,7 +51,9 @@ struct Bfs_State {
     u32 *frontier[2];      /* [n] vertex queues */
     u64 *frontier_bits[2]; /* [ceil(n/64)] frontier bitmaps */
     u64 *visited;          /* [ceil(n/64)] */
-    u32 *hubs;             /* [n] queue of high-degree frontier vertices */
+    u32 *hubs;             /* [2n] queues of high-degree frontier vertices */
+    u64 *hub_offsets;      /* [2n] their offsets in the flattened edge work space */
+    u32 *hub_lengths;      /* [2n] their degrees */
     void *ctl;             /* device control block */
     u64 max_nlocalverts;
 };
@@ -63,10 +65,11 @@ struct Sssp_State {
     f32 *dist;          /* [n] distance; -1.0f unreached */
     u64 *dist_pred;     /* [n] packed working labels */
     u32 *frontier;      /* [6n] light-phase queues: 2 buffers x 3 work tiers */
-    u32 *members;       /* [6n] bucket-member logs: 3 work tiers x 2n */
+    u32 *heavy;         /* [2n] heavy-pass lists: wave tier, hub tier */
+    u64 *hub_offsets;   /* [3n] hub-tier work offsets (2 light queues, 1 heavy list) */
+    u32 *hub_lengths;   /* [3n] hub-tier row-segment lengths */
     u32 *split;         /* [n] number of light entries (w < split_delta) per row */
     u32 *stamp;         /* [n] light-queue de-duplication */
-    u32 *mstamp;        /* [n] bucket-member de-duplication */
     void *ctl;          /* device control block */
     f32 delta;          /* bucket width (default 0.1f) */
     f32 split_delta;    /* delta that `split` was computed for (0: none yet) */

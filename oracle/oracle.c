@@ -24,11 +24,13 @@
  *     uniquely; delta-stepping emulation equals Dijkstra bit for bit.
  *   - Statistics: numpy percentiles / mean / std(ddof=1) on odd and even n.
  */
+#define _POSIX_C_SOURCE 199309L
 #include "oracle.h"
 #include <math.h>
 #include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -546,6 +548,45 @@ void o_dijkstra_f32(uint64_t n, const uint64_t *rs, const uint32_t *col, const f
     free(h.a);
     for (uint64_t v = 0; v < n; ++v)
         if (isinf(dist[v])) dist[v] = -1.0f;
+}
+
+static double now_seconds(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+void o_dijkstra_f32_timed(uint64_t n, const uint64_t *rs, const uint32_t *col, const float *w,
+                          uint64_t root, double max_seconds, uint64_t *scanned, uint64_t *settled,
+                          double *elapsed) {
+    float *dist = (float *)malloc((n ? n : 1) * sizeof(float));
+    for (uint64_t v = 0; v < n; ++v) dist[v] = INFINITY;
+    heap h = {0, 0, 0};
+    const double t0 = now_seconds();
+    uint64_t sc = 0, st = 0, pops = 0;
+    dist[root] = 0.0f;
+    hpush(&h, 0.0, (uint32_t)root);
+    while (h.n) {
+        hent e = hpop(&h);
+        float du = dist[e.v];
+        if ((float)e.d > du) continue;
+        ++st;
+        for (uint64_t k = rs[e.v]; k < rs[e.v + 1]; ++k) {
+            volatile float cand = du + w[k];
+            uint32_t v = col[k];
+            if (cand < dist[v]) {
+                dist[v] = cand;
+                hpush(&h, cand, v);
+            }
+        }
+        sc += rs[e.v + 1] - rs[e.v];
+        if ((++pops & 1023) == 0 && now_seconds() - t0 > max_seconds) break;
+    }
+    *elapsed = now_seconds() - t0;
+    *scanned = sc;
+    *settled = st;
+    free(h.a);
+    free(dist);
 }
 
 void o_dijkstra_f64(uint64_t n, const uint64_t *rs, const uint32_t *col, const float *w,

@@ -80,6 +80,12 @@ void o_dijkstra_f32(uint64_t n, const uint64_t *rs, const uint32_t *col, const f
                     uint64_t root, float *dist, int64_t *pred);
 void o_dijkstra_f64(uint64_t n, const uint64_t *rs, const uint32_t *col, const float *w,
                     uint64_t root, double *dist);
+/* CPU-baseline sample: binary-heap FP32 Dijkstra that stops after
+ * max_seconds; reports CSR entries scanned and vertices settled so far and
+ * the elapsed seconds (whole run when it finishes first). */
+void o_dijkstra_f32_timed(uint64_t n, const uint64_t *rs, const uint32_t *col, const float *w,
+                          uint64_t root, double max_seconds, uint64_t *scanned, uint64_t *settled,
+                          double *elapsed);
 /* The reference's delta-stepping bucket schedule run sequentially (same
  * light/heavy split and bucket bounds as traversal.cu:638-711). */
 void o_delta_stepping_f32(uint64_t n, const uint64_t *rs, const uint32_t *col, const float *w,

@@ -56,6 +56,9 @@ def lib():
         L.o_generate_roots.argtypes = [C.c_uint64, u64p, C.c_uint64, C.c_uint64, u64p, C.c_int]
         L.o_generate_roots.restype = C.c_int
         L.o_dijkstra_f32.argtypes = [C.c_uint64, u64p, u32p, f32p, C.c_uint64, f32p, i64p]
+        L.o_dijkstra_f32_timed.argtypes = [C.c_uint64, u64p, u32p, f32p, C.c_uint64, C.c_double,
+                                           C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                           C.POINTER(C.c_double)]
         L.o_dijkstra_f64.argtypes = [C.c_uint64, u64p, u32p, f32p, C.c_uint64, f64p]
         L.o_delta_stepping_f32.argtypes = [C.c_uint64, u64p, u32p, f32p, C.c_uint64, C.c_float,
                                            f32p, i64p]
@@ -182,6 +185,14 @@ def dijkstra_f32(g, root):
     lib().o_dijkstra_f32(g.n, g.rowstarts, _c(g.column, np.uint32), _c(g.weights, np.float32), root,
                          dist, pred)
     return dist, pred
+
+
+def dijkstra_f32_timed(g, root, max_seconds):
+    """(entries scanned, vertices settled, seconds) of a time-bounded Dijkstra."""
+    sc, st, el = C.c_uint64(), C.c_uint64(), C.c_double()
+    lib().o_dijkstra_f32_timed(g.n, g.rowstarts, _c(g.column, np.uint32), _c(g.weights, np.float32), root,
+                               max_seconds, C.byref(sc), C.byref(st), C.byref(el))
+    return sc.value, st.value, el.value
 
 
 def dijkstra_f64(g, root):

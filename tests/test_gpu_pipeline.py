@@ -258,3 +258,25 @@ def test_runner_small():
     assert kv["SCALE"] == "10" and kv["NBFS"] == "64"
     assert float(kv["bfs  harmonic_mean_TEPS"]) > 0
     assert float(kv["sssp median_TEPS"]) > 0
+
+
+def test_plain_launch_path(arena, oracle):
+    """Persistent kernels launched without the cooperative API (the fallback
+    used when another process holds the device's cooperative queue)."""
+    e, w, ref = oracle_graph(12)
+    g = check_csr(arena, oracle, 1 << 12, e, w)
+    b = arena.bfs_create(g)
+    s = arena.sssp_create(g)
+    arena.set_param("coop_launch", 0)
+    try:
+        for r in oracle.roots(ref)[:3]:
+            r = int(r)
+            arena.bfs(g, b, r)
+            pred = arena.bfs_pred(b)
+            assert oracle.bfs_validate(ref, pred, r) == 0
+            arena.sssp(g, s, r)
+            d, p = arena.sssp_result(s)
+            d_ref, _ = oracle.dijkstra_f32(ref, r)
+            assert np.array_equal(d.view(np.uint32), d_ref.view(np.uint32))
+    finally:
+        arena.set_param("coop_launch", 1)
