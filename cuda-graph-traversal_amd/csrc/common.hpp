@@ -34,6 +34,8 @@ struct Context {
     double bfs_beta = 18.0;       // bottom-up -> top-down when n_frontier < n / beta
     int bfs_block = 512;          // threads per block of the persistent BFS kernel
     int bfs_hub_degree = 2048;    // frontier vertices above this degree are split grid-wide
+    double batch_alpha = 15.0;    // the same two thresholds for the batched multi-root BFS
+    double batch_beta = 18.0;
     int sssp_block = 512;
     int coop_launch = 1;          // 1: cooperative launch first; 0: plain launch of the co-resident grid
     int gen_blocks = 0;           // 0: one block per 256-edge tile (capped)
@@ -114,6 +116,11 @@ __device__ __forceinline__ u32 wave_min(u32 v) {
         v = y < v ? y : v;
     }
     return v;
+}
+
+__device__ __forceinline__ u64 shfl64(u64 v, int src) {
+    u32 lo = __shfl((u32)v, src), hi = __shfl((u32)(v >> 32), src);
+    return ((u64)hi << 32) | lo;
 }
 
 __device__ __forceinline__ u64 shfl_xor64(u64 v, int m) {

@@ -253,6 +253,8 @@ i32 g500_set_param(const char *name, f64 value) {
     std::string n(name);
     if (n == "bfs_alpha") c.bfs_alpha = value;
     else if (n == "bfs_beta") c.bfs_beta = value;
+    else if (n == "batch_alpha") c.batch_alpha = value;
+    else if (n == "batch_beta") c.batch_beta = value;
     else if (n == "bfs_block") c.bfs_block = (int)value;
     else if (n == "bfs_hub_degree") c.bfs_hub_degree = (int)value;
     else if (n == "sssp_block") c.sssp_block = (int)value;
@@ -267,6 +269,8 @@ i32 g500_get_param(const char *name, f64 *value) {
     std::string n(name);
     if (n == "bfs_alpha") *value = c.bfs_alpha;
     else if (n == "bfs_beta") *value = c.bfs_beta;
+    else if (n == "batch_alpha") *value = c.batch_alpha;
+    else if (n == "batch_beta") *value = c.batch_beta;
     else if (n == "bfs_block") *value = c.bfs_block;
     else if (n == "bfs_hub_degree") *value = c.bfs_hub_degree;
     else if (n == "sssp_block") *value = c.sssp_block;

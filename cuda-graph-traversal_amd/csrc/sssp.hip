@@ -25,6 +25,10 @@
 //     relaxations lower into it;
 //   when the next bucket is empty: a min step finds the smallest remaining
 //     distance and a collect step queues the bucket that holds it.
+// The fused step also marks the bucket's vertices settled in a bitmap; every
+// later candidate is >= the current bucket floor > a settled distance, so
+// relaxations into settled vertices skip the label (the bitmap is 1/64 of the
+// label array and stays in cache).
 // Work is mapped by row-segment length: a lane per vertex (<= 32 entries), a
 // wave per vertex (<= 1024), or chunks of a flattened hub work space spread
 // over the whole grid (longer).
@@ -84,6 +88,7 @@ struct SsspArgs {
     u64 *hq_off;
     u32 *hq_len;
     u32 *stamp;
+    u64 *settled;    // bit per vertex: distance final (its bucket is done)
     SsspCtl *ctl;
     i64 *pred;
     f32 *dist;
@@ -152,7 +157,8 @@ __device__ __forceinline__ void visit_edge(const SsspArgs &a, const Target &t, b
     if (act) {
         v = a.col[k];
         const f32 cand = du + a.w[k];
-        if (relax(a.lab, v, cand, u)) {
+        // a settled vertex has d_v < floor <= cand: skip its label
+        if (!((a.settled[v >> 6] >> (v & 63)) & 1) && relax(a.lab, v, cand, u)) {
             ++improved;
             push = cand < t.below;
             if (cand < t.floor) atomicOr(&a.ctl->bar.error, 16u);
@@ -342,6 +348,9 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
                     }
                 }
                 push_light(a, next, (u32)v, t);
+                // the bucket's vertices are final now: one word per wave (64 consecutive v)
+                const u64 sm = ballot(mine);
+                if (sm && lane == 0) a.settled[v >> 6] |= sm;
                 const u32 len = (u32)(e - b);
                 const bool medium = mine && len > kSmall && len <= kHub;
                 const u32 s1 = wave_append(&ctl->hlen[hs], medium);
@@ -451,10 +460,12 @@ __global__ __launch_bounds__(256) void sssp_split_kernel(const u64 *__restrict__
     }
 }
 
-__global__ __launch_bounds__(256) void sssp_clear_kernel(u64 *__restrict__ lab, u32 *__restrict__ stamp, u64 n) {
+__global__ __launch_bounds__(256) void sssp_clear_kernel(u64 *__restrict__ lab, u32 *__restrict__ stamp,
+                                                         u64 *__restrict__ settled, u64 n) {
     for (u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (u64)gridDim.x * blockDim.x) {
         lab[v] = kInfLabel;
         stamp[v] = 0;
+        if ((v & 63) == 0) settled[v >> 6] = 0;
     }
 }
 
@@ -479,6 +490,7 @@ Sssp_State oned_graph_sssp_create(Worker_State *ws, const Oned_Graph *g) {
     s.hub_lengths = (u32 *)arena_alloc(ws, 3 * n * sizeof(u32), "SSSP hub lengths");
     s.split = (u32 *)arena_alloc(ws, n * sizeof(u32), "SSSP light split");
     s.stamp = (u32 *)arena_alloc(ws, n * sizeof(u32), "SSSP stamps");
+    s.settled = (u64 *)arena_alloc(ws, (n + 63) / 64 * sizeof(u64), "SSSP settled bitmap");
     s.ctl = arena_alloc(ws, sizeof(SsspCtl), "SSSP control");
     if (failed()) return s;
     oned_graph_sssp_clear(ws, &s);
@@ -489,7 +501,8 @@ void oned_graph_sssp_clear(Worker_State *ws, Sssp_State *s) {
     (void)ws;
     Context &c = ctx();
     const u64 n = s->max_nlocalverts;
-    sssp_clear_kernel<<<grid_for(n, 256, (u64)c.num_cus * 32), 256, 0, c.stream>>>(s->dist_pred, s->stamp, n);
+    sssp_clear_kernel<<<grid_for(n, 256, (u64)c.num_cus * 32), 256, 0, c.stream>>>(s->dist_pred, s->stamp,
+                                                                                  s->settled, n);
     G500_HIP(hipGetLastError());
     G500_HIP(hipMemsetAsync(s->ctl, 0, sizeof(SsspCtl), c.stream));
     sync();
@@ -531,6 +544,7 @@ void oned_graph_sssp_run(Worker_State *ws, const Oned_Graph *g, Sssp_State *s, u
     a.hq_off = s->hub_offsets + 2 * n;
     a.hq_len = s->hub_lengths + 2 * n;
     a.stamp = s->stamp;
+    a.settled = s->settled;
     a.ctl = (SsspCtl *)s->ctl;
     a.pred = s->pred;
     a.dist = s->dist;

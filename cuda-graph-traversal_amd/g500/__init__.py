@@ -50,8 +50,14 @@ class BfsState(C.Structure):
 
 class SsspState(C.Structure):
     _fields_ = [("pred", vp), ("dist", vp), ("dist_pred", vp), ("frontier", vp), ("heavy", vp),
-                ("hub_offsets", vp), ("hub_lengths", vp), ("split", vp), ("stamp", vp), ("ctl", vp),
+                ("hub_offsets", vp), ("hub_lengths", vp), ("split", vp), ("stamp", vp), ("settled", vp), ("ctl", vp),
                 ("delta", f32), ("split_delta", f32), ("max_nlocalverts", u64)]
+
+
+class BfsBatchState(C.Structure):
+    _fields_ = [("seen", vp), ("fmask", vp * 2), ("next", vp), ("touched", vp), ("log_vertex", vp),
+                ("log_mask", vp), ("level_start", vp), ("hubs", vp), ("hub_offsets", vp), ("hub_lengths", vp),
+                ("ctl", vp), ("log_capacity", u64), ("max_nlocalverts", u64), ("nroots", i32), ("levels", i32)]
 
 
 P = C.POINTER
@@ -94,6 +100,11 @@ _SIGS = {
     "oned_graph_edge_count": (u64, [P(WorkerState), P(OnedGraph), vp]),
     "oned_graph_bfs_validate": (i32, [P(WorkerState), P(OnedGraph), P(BfsState), u64, P(u64)]),
     "get_statistics": (None, [P(f64), i32, P(f64)]),
+    "oned_graph_bfs_batch_create": (BfsBatchState, [P(WorkerState), P(OnedGraph)]),
+    "oned_graph_bfs_batch_run": (None, [P(WorkerState), P(OnedGraph), P(BfsBatchState), P(u64), i32]),
+    "oned_graph_bfs_batch_depth": (None, [P(WorkerState), P(OnedGraph), P(BfsBatchState), i32, vp]),
+    "oned_graph_bfs_batch_destroy": (None, [P(BfsBatchState)]),
+    "g500_bfs_batch_level_log": (i32, [P(BfsBatchState), P(i32), P(u64), P(f64), i32]),
     "g500_set_error_mode": (None, [i32]),
     "g500_status": (i32, []),
     "g500_last_error": (C.c_char_p, []),
@@ -263,6 +274,39 @@ class Session:
         if with_ms:
             return list(modes[:n]), list(sizes[:n]), list(ms[:n])
         return list(modes[:n]), list(sizes[:n])
+
+    def batch_create(self, g):
+        b = self.lib.oned_graph_bfs_batch_create(C.byref(self.ws), C.byref(g))
+        self.check()
+        return b
+
+    def batch(self, g, b, roots):
+        r = np.ascontiguousarray(roots, np.uint64)
+        self.lib.oned_graph_bfs_batch_run(C.byref(self.ws), C.byref(g), C.byref(b),
+                                          r.ctypes.data_as(P(u64)), int(r.size))
+        self.check()
+        return self.lib.g500_last_run_ms()
+
+    def batch_depth(self, g, b, r):
+        """Depths from root index r of the last batched run (host array)."""
+        n = g.nlocalverts
+        if getattr(self, "_depth_buf", (None, 0))[1] != n:
+            cnt = u64()
+            p = self.lib._worker_arena_push_array(C.byref(self.ws), n, 8, 8, C.byref(cnt))
+            if not p:
+                raise LibraryError("device arena exhausted")
+            self._depth_buf = (p, n)
+        p = self._depth_buf[0]
+        self.lib.oned_graph_bfs_batch_depth(C.byref(self.ws), C.byref(g), C.byref(b), int(r), p)
+        self.check()
+        return self.to_host(p, n, np.int64)
+
+    def batch_level_log(self, b, max_levels=256):
+        modes = (i32 * max_levels)()
+        recs = (u64 * max_levels)()
+        ms = (f64 * max_levels)()
+        n = self.lib.g500_bfs_batch_level_log(C.byref(b), modes, recs, ms, max_levels)
+        return list(modes[:n]), list(recs[:n]), list(ms[:n])
 
     def sssp_phase_ms(self, s):
         out = (f64 * 5)()

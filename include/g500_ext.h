@@ -30,6 +30,45 @@ b32 oned_graph_bfs_validate(Worker_State *ws, const Oned_Graph *g, Bfs_State *bf
 /* graph500_runner.cu:22-41: min, q1, median, q3, max, mean, stddev. */
 void get_statistics(const f64 *x, i32 n, f64 r[7]);
 
+/* ---- batched multi-source BFS ------------------------------------------- */
+/* Up to 64 roots traversed together in one persistent kernel: per-vertex u64
+ * masks (bit r = reached from root r), top-down and bottom-up levels over all
+ * roots at once. No counterpart in the reference (its runner runs one root at
+ * a time, graph500_runner.cu:306-336); the depths per root equal the plain
+ * BFS's. The result is a discovery log: for every level, the vertices that
+ * some root reached first at that level and the mask of those roots. */
+typedef struct Bfs_Batch_State Bfs_Batch_State;
+struct Bfs_Batch_State {
+    u64 *seen;         /* [n] roots that reached each vertex */
+    u64 *fmask[2];     /* [n] per-level frontier masks (dense, bottom-up levels) */
+    u64 *next;         /* [n] top-down accumulation of the next level */
+    u32 *touched;      /* [n] vertices touched in a top-down level */
+    u32 *log_vertex;   /* [log_capacity] discovery log: vertex */
+    u64 *log_mask;     /* [log_capacity] discovery log: newly reaching roots */
+    u64 *level_start;  /* [n + 2] first log record of each level */
+    u32 *hubs;         /* [2n] hub-tier record indices (two consecutive levels) */
+    u64 *hub_offsets;  /* [2n] */
+    u32 *hub_lengths;  /* [2n] */
+    void *ctl;
+    u64 log_capacity;
+    u64 max_nlocalverts;
+    i32 nroots;        /* roots of the last run */
+    i32 levels;        /* levels of the last run (records exist for 0..levels-1) */
+};
+Bfs_Batch_State oned_graph_bfs_batch_create(Worker_State *ws, const Oned_Graph *g);
+/* roots: host array of nroots (1..64) vertex ids (repeats allowed). */
+void oned_graph_bfs_batch_run(Worker_State *ws, const Oned_Graph *g, Bfs_Batch_State *b, const u64 *roots,
+                              i32 nroots);
+/* Depth of every vertex from root index r of the last run into the device
+ * array depth[n] (-1: unreached), the convention of bfs_compute_dist_from_pred. */
+void oned_graph_bfs_batch_depth(Worker_State *ws, const Oned_Graph *g, const Bfs_Batch_State *b, i32 r,
+                                i64 *depth);
+void oned_graph_bfs_batch_destroy(Bfs_Batch_State *b);
+/* Per-level log of the last batched run: mode (0 top-down, 1 bottom-up),
+ * record count and device milliseconds (any pointer may be NULL). Returns
+ * the number of levels. */
+i32 g500_bfs_batch_level_log(const Bfs_Batch_State *b, i32 *modes, u64 *records, f64 *ms, i32 max_levels);
+
 /* ---- library control --------------------------------------------------- */
 /* G500_ERRORS_ABORT (default): fatal errors print and exit like the
  * reference. G500_ERRORS_RETURN: they set a status and the entry point

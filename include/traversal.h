@@ -70,6 +70,7 @@ struct Sssp_State {
     u32 *hub_lengths;   /* [3n] hub-tier row-segment lengths */
     u32 *split;         /* [n] number of light entries (w < split_delta) per row */
     u32 *stamp;         /* [n] light-queue de-duplication */
+    u64 *settled;       /* [ceil(n/64)] vertices whose distance is final */
     void *ctl;          /* device control block */
     f32 delta;          /* bucket width (default 0.1f) */
     f32 split_delta;    /* delta that `split` was computed for (0: none yet) */

@@ -39,14 +39,15 @@ def test_library_exports_every_declared_function():
 
 
 STRUCTS = ["Arena", "Worker_State", "Tuple_Graph", "Packed_Edge", "Mrg_State", "Mrg_Transition_Matrix",
-           "Oned_Graph", "Bfs_State", "Sssp_State"]
+           "Oned_Graph", "Bfs_State", "Sssp_State", "Bfs_Batch_State"]
 
 
 def test_struct_layout_matches_headers():
     """sizeof/offsetof from a C compiler against the ctypes mirror."""
     import g500
     mirror = {"Arena": g500.Arena, "Worker_State": g500.WorkerState, "Tuple_Graph": g500.TupleGraph,
-              "Oned_Graph": g500.OnedGraph, "Bfs_State": g500.BfsState, "Sssp_State": g500.SsspState}
+              "Oned_Graph": g500.OnedGraph, "Bfs_State": g500.BfsState, "Sssp_State": g500.SsspState,
+              "Bfs_Batch_State": g500.BfsBatchState}
     src = ["#include <stddef.h>", "#include <stdio.h>", '#include "g500_ext.h"', "int main(void){"]
     for s in STRUCTS:
         src.append(f'printf("{s} %zu\\n", sizeof({s}));')

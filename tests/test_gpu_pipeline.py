@@ -167,6 +167,67 @@ def test_bfs_direction_modes(arena, oracle, alpha, beta, hub):
             arena.set_param(k, v)
 
 
+# ----------------------------------------------------------- batched BFS (64 roots)
+def batch_check(sess, oracle, g, ref, b, roots):
+    sess.batch(g, b, roots)
+    for i, r in enumerate(roots):
+        _, depth_ref = oracle.bfs(ref, int(r))
+        depth = sess.batch_depth(g, b, i)
+        assert np.array_equal(depth, depth_ref), (i, int(r))
+
+
+@pytest.mark.parametrize("scale", [10, 12, 16])
+def test_bfs_batch_depths(arena, oracle, scale):
+    e, w, ref = oracle_graph(scale)
+    g = check_csr(arena, oracle, 1 << scale, e, w)
+    b = arena.batch_create(g)
+    roots = oracle.roots(ref)
+    say("batch", scale, len(roots))
+    batch_check(arena, oracle, g, ref, b, roots)
+    # fewer roots, repeats, an isolated vertex as a root
+    isolated = int(np.flatnonzero(np.diff(ref.rowstarts.astype(np.int64)) == 0)[0])
+    some = np.array([roots[3], roots[3], isolated, roots[0], roots[5]], np.uint64)
+    say("batch repeats", scale, some.tolist())
+    batch_check(arena, oracle, g, ref, b, some)
+
+
+# a huge alpha switches to bottom-up at once; a tiny one stays top-down; a
+# small hub degree sends most records through the grid-wide hub lists
+@pytest.mark.parametrize("alpha,beta,hub", [(1e-9, 18, 2048), (1e9, 1e9, 2048), (15, 18, 4), (2, 4, 64)])
+def test_bfs_batch_direction_modes(arena, oracle, alpha, beta, hub):
+    e, w, ref = oracle_graph(14)
+    g = check_csr(arena, oracle, 1 << 14, e, w)
+    b = arena.batch_create(g)
+    old = {k: arena.get_param(k) for k in ("batch_alpha", "batch_beta", "bfs_hub_degree")}
+    try:
+        arena.set_param("batch_alpha", alpha)
+        arena.set_param("batch_beta", beta)
+        arena.set_param("bfs_hub_degree", hub)
+        say("batch modes", alpha, beta, hub)
+        batch_check(arena, oracle, g, ref, b, oracle.roots(ref))
+        modes, recs, ms = arena.batch_level_log(b)
+        if alpha < 1e-3:
+            assert not any(modes)
+        if alpha > 1e8 and beta > 1e8:
+            assert all(modes[1:])
+    finally:
+        for k, v in old.items():
+            arena.set_param(k, v)
+
+
+def test_bfs_batch_errors_return(arena):
+    import g500
+    e, w, ref = oracle_graph(10)
+    g = arena.csr(arena.upload_tuples(e, w), 1 << 10)
+    b = arena.batch_create(g)
+    for roots in ([], list(range(65)), [1 << 10]):
+        with pytest.raises(g500.LibraryError):
+            arena.batch(g, b, np.array(roots, np.uint64))
+    arena.batch(g, b, np.array([1], np.uint64))
+    with pytest.raises(g500.LibraryError):
+        arena.batch_depth(g, b, 1)
+
+
 def test_validation_parity_on_corrupted_trees(arena, oracle):
     e, w, ref = oracle_graph(12)
     g = check_csr(arena, oracle, 1 << 12, e, w)
