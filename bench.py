@@ -143,16 +143,16 @@ def main():
         "sssp_kernel": {"bytes": float(bytes_s.mean()), "time": float(ts.mean()), "total": float(ts.sum())},
     }
     dom = max(kern, key=lambda k: kern[k]["total"])
-    traffic = None
+    traffic = {}
     tpath = os.path.join(ROOT, "profiles", f"traffic_s{args.scale}_ef{args.edgefactor}.json")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get(dom)
+        traffic = json.load(open(tpath))
 
     def roof(name):
         k = kern[name]
         ach = k["bytes"] / k["time"] / 1e9
         return {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": round(ach * 1e9 / HBM_PEAK, 4), "traffic": traffic if name == dom else None,
+                "frac": round(ach * 1e9 / HBM_PEAK, 4), "traffic": traffic.get(name),
                 "kernel": name, "bytes_per_launch": k["bytes"], "avg_launch_ms": k["time"] * 1e3}
 
     out = {
