@@ -2,15 +2,23 @@
 """Graph500 BFS + SSSP benchmark on one MI355X.
 
 Workload (default): Kronecker graph SCALE 26, edgefactor 16 (seed 2, 3),
-generated and CSR-built on the GPU, then one step per root: the plain
-single-root BFS and delta-stepping SSSP (delta 0.1) through the C-ABI. All
-inputs are resident in HBM before the timed region. Per-root times are the
-kernels' HIP-event durations on the library stream; TEPS follow the Graph500
-runner (m_c = undirected component edges, harmonic mean over roots);
-`value` is the combined harmonic mean 2 / (1/HM_BFS + 1/HM_SSSP) in GTEPS.
+generated and CSR-built on the GPU, 64 roots from the Graph500 root rule. All
+inputs are resident in HBM before any timed region. Times are the kernels'
+HIP-event durations on the library stream.
 
-Prints one JSON line (see the keys below). Options exist for smaller scales
-and for the CPU baseline sample; --gpus above 1 is rejected (one GPU).
+Batched mode (default): one step = one group of 16 roots, traversed by one
+batched BFS launch (per-vertex root masks) and one batched delta-stepping SSSP
+launch (delta 0.1, shared buckets); 4 steps cover the 64 roots.
+Plain mode: one step = one root, single-root BFS then single-root SSSP.
+
+`value` is whole-job throughput in GTEPS: the TEPS edges of every traversal
+(m_c = undirected edges of the root's component, as the Graph500 runner counts
+them; once for the BFS and once for the SSSP of each root) over the summed
+kernel time. A plain reference pass over the same roots is reported beside the
+batched value (`plain`), with the runner's harmonic means; `batched_vs_plain`
+is the ratio of the two whole-job throughputs.
+
+Prints one JSON line. --gpus above 1 is rejected (one GPU).
 """
 import argparse
 import json
@@ -25,6 +33,7 @@ sys.path.insert(0, os.path.join(ROOT, "cuda-graph-traversal_amd"))
 
 METRIC = "harmonic-mean GTEPS BFS+SSSP @SCALE26 ef16, batched vs plain; p99 req latency"
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+GROUP = 16         # roots per batched step (the batched SSSP width)
 
 
 def hm(times_s, edges):
@@ -60,14 +69,21 @@ def cpu_baseline(sess, g, roots, m_c, seconds, nthreads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=64, help="roots timed (BFS + SSSP each)")
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--mode", choices=("batched", "plain"), default="batched")
+    ap.add_argument("--steps", type=int, default=4,
+                    help="timed steps (batched: 16-root groups; plain: roots)")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--plain-roots", type=int, default=64,
+                    help="roots of the plain reference pass in batched mode (0: skip)")
     ap.add_argument("--scale", type=int, default=26)
     ap.add_argument("--edgefactor", type=int, default=16)
     ap.add_argument("--delta", type=float, default=0.1)
-    ap.add_argument("--arena-gib", type=int, default=128)
+    ap.add_argument("--arena-gib", type=int, default=160)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--serve-requests", type=int, default=512,
+                    help="requests of the serving leg (0: skip); resident graphs S16-S24")
+    ap.add_argument("--serve-rate", type=float, default=1000.0, help="Poisson arrivals per second")
     ap.add_argument("--calibrate-mib", type=int, default=0,
                     help="also run the device copy kernel on this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -100,48 +116,96 @@ def main():
     nroots = len(roots)
     bfs = sess.bfs_create(g)
     sssp = sess.sssp_create(g, args.delta)
-    need = min(nroots, args.steps + args.warmup)
 
     # per-root TEPS numerators and reached counts (untimed, as in the runner)
     m_c = np.zeros(nroots, np.float64)
     n_c = np.zeros(nroots, np.float64)
-    for i in range(need):
+    for i in range(nroots):
         sess.bfs(g, bfs, roots[i])
         m_c[i] = sess.edge_count(g, bfs.pred)
         n_c[i] = sum(sess.level_log(bfs)[1])
-    log(f"[bench] {nroots} roots; edges per root ~{np.median(m_c[:need]):.4g}, reached ~{np.median(n_c[:need]):.4g}; "
-        f"smallest components (edges): {sorted(m_c[:need])[:4]}")
+    log(f"[bench] {nroots} roots; edges per root ~{np.median(m_c):.4g}, reached ~{np.median(n_c):.4g}; "
+        f"smallest components (edges): {sorted(m_c)[:4]}")
 
-    for i in range(args.warmup):
-        r = roots[(args.steps + i) % nroots]
-        sess.bfs(g, bfs, r)
-        sess.sssp(g, sssp, r)
+    def plain_pass(idx, label):
+        tb = np.zeros(len(idx))
+        ts = np.zeros(len(idx))
+        for i, ri in enumerate(idx):
+            tb[i] = sess.bfs(g, bfs, roots[ri]) * 1e-3
+            ts[i] = sess.sssp(g, sssp, roots[ri]) * 1e-3
+        log(f"[bench] {label}: bfs ms median {np.median(tb)*1e3:.3f} max {tb.max()*1e3:.3f}; "
+            f"sssp ms median {np.median(ts)*1e3:.3f} max {ts.max()*1e3:.3f}")
+        return tb, ts
 
-    tb = np.zeros(args.steps)
-    ts = np.zeros(args.steps)
-    idx = [i % nroots for i in range(args.steps)]
-    # every entry point is blocking, so the device is idle when the clock starts and stops
-    t0 = time.perf_counter()
-    for i, ri in enumerate(idx):
-        tb[i] = sess.bfs(g, bfs, roots[ri]) * 1e-3
-        ts[i] = sess.sssp(g, sssp, roots[ri]) * 1e-3
-    t1 = time.perf_counter()
-    for i in np.argsort(tb)[:3]:
-        log(f"[bench] root {int(roots[idx[i]])}: m_c {m_c[idx[i]]:.0f} bfs {tb[i]*1e3:.3f} ms sssp {ts[i]*1e3:.3f} ms")
-    log(f"[bench] bfs ms median {np.median(tb)*1e3:.3f} max {tb.max()*1e3:.3f}; "
-        f"sssp ms median {np.median(ts)*1e3:.3f} max {ts.max()*1e3:.3f}")
-    me = m_c[idx]
-    nv = n_c[idx]
-    hm_b = hm(tb, me)
-    hm_s = hm(ts, me)
-    value = 2.0 / (1.0 / hm_b + 1.0 / hm_s) / 1e9
+    kern = {}
+    if args.mode == "plain":
+        for i in range(args.warmup):
+            r = roots[(args.steps + i) % nroots]
+            sess.bfs(g, bfs, r)
+            sess.sssp(g, sssp, r)
+        idx = [i % nroots for i in range(args.steps)]
+        t0 = time.perf_counter()
+        tb, ts = plain_pass(idx, "plain")
+        t1 = time.perf_counter()
+        me, nv = m_c[idx], n_c[idx]
+        edges_total = 2.0 * me.sum()
+        kern_time = tb.sum() + ts.sum()
+        kern["bfs_kernel"] = {"bytes": float((16.0 * me + 16.0 * nv).mean()), "time": float(tb.mean()),
+                              "total": float(tb.sum()), "units": "one root"}
+        kern["sssp_kernel"] = {"bytes": float((24.0 * me + 16.0 * nv).mean()), "time": float(ts.mean()),
+                               "total": float(ts.sum()), "units": "one root"}
+        plain = None
+        workload = (f"plain single-root BFS + delta-stepping SSSP per step (one root), Kronecker SCALE "
+                    f"{args.scale} edgefactor {args.edgefactor}, roots from the Graph500 root rule")
+    else:
+        bb = sess.batch_create(g)
+        sb = sess.sssp_batch_create(g, args.delta)
+        ngroups = max(1, nroots // GROUP)
+        groups = [list(range(k * GROUP, min((k + 1) * GROUP, nroots))) for k in range(ngroups)]
 
-    bytes_b = 16.0 * me + 16.0 * nv
-    bytes_s = 24.0 * me + 16.0 * nv
-    kern = {
-        "bfs_kernel": {"bytes": float(bytes_b.mean()), "time": float(tb.mean()), "total": float(tb.sum())},
-        "sssp_kernel": {"bytes": float(bytes_s.mean()), "time": float(ts.mean()), "total": float(ts.sum())},
-    }
+        def batched_step(grp):
+            rr = roots[grp]
+            b = sess.batch(g, bb, rr) * 1e-3
+            s = sess.sssp_batch(g, sb, rr) * 1e-3
+            return b, s
+
+        for i in range(args.warmup):
+            batched_step(groups[(args.steps + i) % ngroups])
+        tb = np.zeros(args.steps)
+        ts = np.zeros(args.steps)
+        me = np.zeros(args.steps)
+        nv = np.zeros(args.steps)
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            grp = groups[i % ngroups]
+            tb[i], ts[i] = batched_step(grp)
+            me[i] = m_c[grp].sum()
+            nv[i] = n_c[grp].sum()
+        t1 = time.perf_counter()
+        log(f"[bench] batched: bfs ms per {GROUP}-root launch median {np.median(tb)*1e3:.2f}; "
+            f"sssp ms per launch median {np.median(ts)*1e3:.1f}")
+        edges_total = 2.0 * me.sum()
+        kern_time = tb.sum() + ts.sum()
+        kern["batch_bfs_kernel"] = {"bytes": float((16.0 * me + 16.0 * nv).mean()), "time": float(tb.mean()),
+                                    "total": float(tb.sum()), "units": f"{GROUP} roots"}
+        kern["sssp_batch_kernel"] = {"bytes": float((24.0 * me + 16.0 * nv).mean()), "time": float(ts.mean()),
+                                     "total": float(ts.sum()), "units": f"{GROUP} roots"}
+        plain = None
+        if args.plain_roots:
+            idx = list(range(min(args.plain_roots, nroots)))
+            ptb, pts = plain_pass(idx, "plain reference")
+            pm = m_c[idx]
+            pw = 2.0 * pm.sum() / (ptb.sum() + pts.sum()) / 1e9
+            plain = {"roots": len(idx), "gteps": round(pw, 4),
+                     "bfs_gteps": round(pm.sum() / ptb.sum() / 1e9, 4),
+                     "sssp_gteps": round(pm.sum() / pts.sum() / 1e9, 4),
+                     "bfs_hm_gteps": round(hm(ptb, pm) / 1e9, 6), "sssp_hm_gteps": round(hm(pts, pm) / 1e9, 6),
+                     "bfs_ms_median": round(float(np.median(ptb)) * 1e3, 3),
+                     "sssp_ms_median": round(float(np.median(pts)) * 1e3, 3)}
+        workload = (f"batched: per step {GROUP} roots, one 64-bit-mask BFS launch and one 16-root delta-stepping "
+                    f"SSSP launch, Kronecker SCALE {args.scale} edgefactor {args.edgefactor}, roots from the "
+                    f"Graph500 root rule")
+    value = edges_total / kern_time / 1e9
     dom = max(kern, key=lambda k: kern[k]["total"])
     traffic = {}
     tpath = os.path.join(ROOT, "profiles", f"traffic_s{args.scale}_ef{args.edgefactor}.json")
@@ -153,8 +217,10 @@ def main():
         ach = k["bytes"] / k["time"] / 1e9
         return {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": round(ach * 1e9 / HBM_PEAK, 4), "traffic": traffic.get(name),
-                "kernel": name, "bytes_per_launch": k["bytes"], "avg_launch_ms": k["time"] * 1e3}
+                "kernel": name, "bytes_per_launch": k["bytes"], "avg_launch_ms": k["time"] * 1e3,
+                "launch_covers": k["units"]}
 
+    other = [k for k in kern if k != dom][0]
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -168,17 +234,28 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: Graph500 Kronecker generator on the GPU, user seed (2, 3), U[0,1] f32 weights",
-        "config": {"workload": (f"plain single-root BFS + delta-stepping SSSP per step, Kronecker SCALE "
-                                f"{args.scale} edgefactor {args.edgefactor}, roots from the Graph500 root rule"),
-                   "scale": args.scale, "edgefactor": args.edgefactor, "delta": args.delta,
-                   "roots": nroots, "mode": "plain"},
-        "bfs_hm_gteps": round(hm_b / 1e9, 4),
-        "sssp_hm_gteps": round(hm_s / 1e9, 4),
+        "config": {"workload": workload, "scale": args.scale, "edgefactor": args.edgefactor,
+                   "delta": args.delta, "roots": nroots, "mode": args.mode},
+        "bfs_gteps": round(me.sum() / tb.sum() / 1e9, 4),
+        "sssp_gteps": round(me.sum() / ts.sum() / 1e9, 4),
+        "plain": plain,
+        "batched_vs_plain": round(value / plain["gteps"], 3) if plain else None,
         "generator_ms": round(gen_ms, 3),
         "csr_build_ms": round(csr_ms, 3),
         "roofline": roof(dom),
-        "roofline_other": roof("sssp_kernel" if dom == "bfs_kernel" else "bfs_kernel"),
+        "roofline_other": roof(other),
     }
+    if args.mode == "plain":
+        out["bfs_hm_gteps"] = round(hm(tb, me) / 1e9, 6)
+        out["sssp_hm_gteps"] = round(hm(ts, me) / 1e9, 6)
+    if args.serve_requests:
+        from g500 import serving
+        log("[bench] serving leg: loading 8 resident graphs")
+        res = serving.load_graphs(sess)
+        out["serving"] = serving.poisson_run(sess, res, args.serve_rate, args.serve_requests)
+        log(f"[bench] serving: {out['serving']}")
+    else:
+        out["serving"] = None
     if not args.no_cpu:
         nthreads = min(16, len(os.sched_getaffinity(0)))
         log(f"[bench] CPU baseline sample ({nthreads} threads)")

@@ -26,6 +26,7 @@
 namespace g500 {
 
 constexpr int kBatchLevelLog = 256;
+constexpr int kSteps = 4;  // bottom-up chunks in flight per wave
 
 struct BatchCtl {
     GridBarrier bar;
@@ -35,11 +36,12 @@ struct BatchCtl {
     u64 hub[3];                    // packed hub counters (record indices), per slot
     u64 fedges[3];                 // sum of degrees of a level's records, per slot
     u64 alive[3];                  // OR of a level's masks, per slot
+    u64 bhub;                      // packed counter of the heavy-vertex list (bottom-up)
     u32 levels;
     u32 pad;
-    u64 roots[64];
     alignas(256) i32 mode_log[kBatchLevelLog];
     u64 size_log[kBatchLevelLog];
+    u64 edge_log[kBatchLevelLog];  // sum of the degrees of a level's records
     u64 tick_log[kBatchLevelLog];
     alignas(256) BlockTrace trace[kMaxTraceBlocks];
 };
@@ -50,6 +52,7 @@ struct BatchArgs {
     const u32 *col;
     u64 *seen;
     u64 *fm[2];
+    u64 *fb[2];       // frontier bitmaps matching fm
     u64 *next;
     u32 *touched;
     u32 *lv;          // log vertex
@@ -60,9 +63,11 @@ struct BatchArgs {
     u64 *hoff;
     u32 *hlen;
     BatchCtl *ctl;
+    u64 roots[64];
     i32 nroots;
     f32 alpha, beta;
-    u32 hub_degree;
+    u32 hub_degree;    // top-down: records above this degree go to the grid-wide hub list
+    u32 heavy_degree;  // bottom-up: vertices above this degree are split over the grid
 };
 
 __device__ __forceinline__ void batch_sync(BatchCtl *ctl, u32 phase, u32 level) {
@@ -151,18 +156,217 @@ __device__ void compact_touched(const BatchArgs &a, int slot, int buf) {
     flush_level(a, slot, edges, alive);
 }
 
+__device__ __forceinline__ u64 shfl_up64(u64 v, int o) {
+    u32 lo = __shfl_up((u32)v, o), hi = __shfl_up((u32)(v >> 32), o);
+    return ((u64)hi << 32) | lo;
+}
+
+__device__ __forceinline__ u64 uniform64(u64 v) {
+    return ((u64)__builtin_amdgcn_readfirstlane((u32)(v >> 32)) << 32) | __builtin_amdgcn_readfirstlane((u32)v);
+}
+
+// Segmented walk over the concatenated edge lists of one wave's 64 items:
+// lane j's item owns positions [off, off + deg) of the concatenation. For the
+// 64-position chunk starting at c, returns the item whose range holds
+// position c + lane (valid when c + lane < total), and sets `inter`/`last`
+// for the lane's own item: whether its range meets the chunk and the lane
+// that holds its last position in the chunk. `slot` is the wave's 64 LDS
+// words; LDS operations of one wave complete in order.
+__device__ __forceinline__ int chunk_owner(volatile int *slot, u32 off, u32 deg, u32 c, bool &inter, int &last) {
+    const u32 lane = lane_id();
+    const u32 lo = off > c ? off : c;
+    const u32 hi = off + deg < c + kWave ? off + deg : c + kWave;
+    inter = lo < hi;
+    last = inter ? (int)(hi - 1 - c) : (int)lane;
+    slot[lane] = -1;
+    if (inter) slot[lo - c] = (int)lane;
+    int j = slot[lane];
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int y = __shfl_up(j, o);
+        if ((int)lane >= o && y > j) j = y;
+    }
+    return j;
+}
+
+__device__ __forceinline__ bool frontier_bit(const u64 *fb, u32 v) { return (fb[v >> 6] >> (v & 63)) & 1; }
+
+// Bottom-up pass over the vertices that still miss live roots. A wave takes
+// 64 consecutive vertices; the edge lists of those with missing roots and
+// degree <= heavy_degree are walked as one concatenated range, 64 edges per
+// step (coalesced column reads). The frontier bitmap is tested before the
+// 8-byte mask gather; a segmented OR scan hands each vertex's partial result
+// to its own lane, and vertices whose missing roots are all found stop.
+__device__ void bu_groups(const BatchArgs &a, u32 L, u64 alive_cur, int nxt, u64 &edges, u64 &alive,
+                          volatile int *slot) {
+    const u64 *fmc = a.fm[L & 1], *fbc = a.fb[L & 1];
+    u64 *fmn = a.fm[(L + 1) & 1], *fbn = a.fb[(L + 1) & 1];
+    const u32 lane = lane_id();
+    const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
+    const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
+    const u64 ngroups = (a.n + kWave - 1) / kWave;
+    for (u64 grp = wave_gid; grp < ngroups; grp += nwaves) {
+        const u64 u = grp * kWave + lane;
+        u64 need = 0, base = 0;
+        u32 deg = 0;
+        if (u < a.n) {
+            need = alive_cur & ~a.seen[u];
+            if (need) {
+                const u64 r0 = a.rs[u], r1 = a.rs[u + 1];
+                if (r1 - r0 > a.heavy_degree) need = 0;  // handled by bu_heavy_chunks
+                deg = (u32)(r1 - r0);
+                base = r0;
+            }
+        }
+        if (!need) deg = 0;
+        const u32 incl = wave_inclusive_scan(deg);
+        const u32 off = incl - deg, total = __shfl(incl, kWave - 1);
+        const u64 shift = base - (u64)off;  // CSR position of concatenated position e: shift + e
+        u64 acc = 0;
+        u64 done = ballot(deg == 0);
+        // kSteps chunks per round: their column reads, bitmap probes and mask
+        // gathers are issued together, so each wave keeps several independent
+        // load chains in flight
+        for (u32 c = 0; c < total && done != ~0ull; c += kSteps * kWave) {
+            int jq[kSteps], lastq[kSteps];
+            bool interq[kSteps], okq[kSteps];
+            u32 vq[kSteps];
+            u64 mq[kSteps];
+#pragma unroll
+            for (int q = 0; q < kSteps; ++q) {
+                jq[q] = chunk_owner(slot, off, deg, c + q * kWave, interq[q], lastq[q]);
+                if (jq[q] < 0) jq[q] = 0;  // chunk past the end: no valid lane
+            }
+#pragma unroll
+            for (int q = 0; q < kSteps; ++q) {
+                const u32 e = c + q * kWave + lane;
+                const u64 sj = shfl64(shift, jq[q]);
+                okq[q] = e < total && !((done >> jq[q]) & 1);
+                vq[q] = okq[q] ? a.col[sj + e] : 0;
+            }
+#pragma unroll
+            for (int q = 0; q < kSteps; ++q) okq[q] = okq[q] && frontier_bit(fbc, vq[q]);
+#pragma unroll
+            for (int q = 0; q < kSteps; ++q) {
+                const u64 nj = shfl64(need, jq[q]);
+                mq[q] = okq[q] ? fmc[vq[q]] & nj : 0;
+            }
+#pragma unroll
+            for (int q = 0; q < kSteps; ++q) {
+                if (!ballot(mq[q] != 0)) continue;
+                u64 x = mq[q];
+                const int j = jq[q];
+#pragma unroll
+                for (int o = 1; o < kWave; o <<= 1) {
+                    const u64 y = shfl_up64(x, o);
+                    const int yj = __shfl_up(j, o);
+                    if ((int)lane >= o && yj == j) x |= y;
+                }
+                const u64 got = shfl64(x, lastq[q]);
+                if (interq[q]) acc |= got;
+            }
+            done |= ballot(deg > 0 && (acc & need) == need);
+        }
+        const u64 nb = acc & need;
+        if (nb) {
+            a.seen[u] |= nb;
+            fmn[u] = nb;
+        }
+        const u64 word = ballot(nb != 0);
+        if (lane == 0) fbn[grp] = word;
+        append_record(a, nb != 0, (u32)u, nb, nxt, (int)((L + 1) & 1), edges, alive);
+    }
+}
+
+// Heavy vertices with missing roots: their edge lists in 256-edge chunks over
+// the grid; each chunk ORs its finds into next[v].
+__device__ void bu_heavy_chunks(const BatchArgs &a, u32 L, u64 alive_cur) {
+    const u64 *fmc = a.fm[L & 1], *fbc = a.fb[L & 1];
+    const u32 lane = lane_id();
+    const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
+    const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
+    const u64 hp = ld_relaxed(&a.ctl->bhub);
+    const u64 nh = hub_count(hp), ht = hub_total(hp);
+    const u64 hb = 2 * a.n;
+    const u32 *hv = a.hubs + hb, *hlen = a.hlen + hb;
+    const u64 *hoff = a.hoff + hb;
+    u64 hi_idx = 0;
+    for (u64 c = wave_gid; c * kHubChunk < ht; c += nwaves) {
+        const u64 p = c * kHubChunk;
+        hi_idx = hub_locate(hoff, hi_idx, nh, p);
+        const u32 v = hv[hi_idx];
+        const u64 need = uniform64(alive_cur & ~a.seen[v]);
+        if (!need) continue;
+        if ((uniform64(ld_relaxed(&a.next[v])) & need) == need) continue;
+        const u64 b = a.rs[v] + (p - hoff[hi_idx]), e = a.rs[v] + hlen[hi_idx];
+        u64 acc = 0;
+#pragma unroll
+        for (u64 j = 0; j < kHubChunk; j += kWave) {
+            const u64 k = b + j + lane;
+            if (k < e) {
+                const u32 w = a.col[k];
+                if (frontier_bit(fbc, w)) acc |= fmc[w] & need;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) acc |= shfl_xor64(acc, o);
+        if (lane == 0 && acc) atomicOr((unsigned long long *)&a.next[v], (unsigned long long)acc);
+    }
+}
+
+// Heavy vertices: fold the chunk results (next[v], cleared here) into seen,
+// the next frontier and the log.
+__device__ void bu_heavy_fixup(const BatchArgs &a, u32 L, u64 alive_cur, int nxt, u64 &edges, u64 &alive) {
+    u64 *fmn = a.fm[(L + 1) & 1], *fbn = a.fb[(L + 1) & 1];
+    const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
+    const u64 nh = MIN(hub_count(ld_relaxed(&a.ctl->bhub)), a.n);
+    const u64 nround = (nh + gthreads - 1) / gthreads * gthreads;
+    const u32 *hv = a.hubs + 2 * a.n;
+    for (u64 i = gtid; i < nround; i += gthreads) {
+        u32 v = 0;
+        u64 nb = 0;
+        if (i < nh) {
+            v = hv[i];
+            const u64 nx = a.next[v];
+            if (nx) {
+                a.next[v] = 0;
+                const u64 s = a.seen[v];
+                nb = nx & alive_cur & ~s;
+                if (nb) {
+                    a.seen[v] = s | nb;
+                    fmn[v] = nb;
+                    atomicOr((unsigned long long *)&fbn[v >> 6], 1ull << (v & 63));
+                }
+            }
+        }
+        append_record(a, nb != 0, v, nb, nxt, (int)((L + 1) & 1), edges, alive);
+    }
+}
+
 __global__ __launch_bounds__(512) void batch_bfs_kernel(BatchArgs a) {
     BatchCtl *ctl = a.ctl;
     const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
     const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
 
+    __shared__ int slots[512 / kWave][kWave];
+    volatile int *slot = slots[threadIdx.x / kWave];
+
+    // heavy-vertex list for bottom-up levels
+    {
+        const u64 nround = (a.n + gthreads - 1) / gthreads * gthreads;
+        const u64 hb = 2 * a.n;
+        for (u64 v = gtid; v < nround; v += gthreads) {
+            const u64 deg = v < a.n ? a.rs[v + 1] - a.rs[v] : 0;
+            hub_append(&ctl->bhub, a.hubs + hb, a.hoff + hb, a.hlen + hb, deg > a.heavy_degree, (u32)v, (u32)deg,
+                       a.n, &ctl->bar.error);
+        }
+    }
     // level 0: the roots (repeats share one record)
     if (gtid < kWave) {
         bool first = false;
         u32 v = 0;
         if ((i32)gtid < a.nroots) {
-            v = (u32)ctl->roots[gtid];
+            v = (u32)a.roots[gtid];
             first = atomicOr((unsigned long long *)&a.next[v], 1ull << gtid) == 0;
         }
         const u32 s = wave_append(&ctl->ntouched[0], first);
@@ -175,7 +379,7 @@ __global__ __launch_bounds__(512) void batch_bfs_kernel(BatchArgs a) {
     u64 lo = 0, hi = ld_relaxed(&ctl->rcount[0]);
     bool bu_prev = false;  // whether the current level was produced by a bottom-up pass
     bool bottom_up = false;
-    u64 explored = 0, prev_count = 0;
+    u64 prev_count = 0;
     u64 t_prev = gtid == 0 ? wall_clock64() : 0;
     for (u32 L = 0;; ++L) {
         const int cur = (int)(L % 3), nxt = (int)((L + 1) % 3), clr = (int)((L + 2) % 3);
@@ -189,10 +393,10 @@ __global__ __launch_bounds__(512) void batch_bfs_kernel(BatchArgs a) {
         const u64 count = hi - lo;
         const u64 fedges = ld_relaxed(&ctl->fedges[cur]);
         const u64 alive_cur = ld_relaxed(&ctl->alive[cur]);
-        const u64 unexplored = a.m_total > explored + fedges ? a.m_total - explored - fedges : 0;
-        if (!bottom_up) bottom_up = unexplored > 0 && (f32)fedges > (f32)unexplored / a.alpha;
+        // Beamer's rule adapted to root masks: a level's top-down work is the
+        // degree sum of its records; bottom-up work is bounded by the edge total
+        if (!bottom_up) bottom_up = (f32)fedges > (f32)a.m_total / a.alpha;
         else bottom_up = !((f32)count < (f32)a.n / a.beta && count < prev_count);
-        explored += fedges;
         prev_count = count;
         if (gtid == 0) {
             st_relaxed(&ctl->ntouched[clr], 0u);
@@ -204,35 +408,29 @@ __global__ __launch_bounds__(512) void batch_bfs_kernel(BatchArgs a) {
             if (L < kBatchLevelLog) {
                 st_relaxed(&ctl->mode_log[L], (i32)bottom_up);
                 st_relaxed(&ctl->size_log[L], count);
+                st_relaxed(&ctl->edge_log[L], fedges);
             }
         }
         u64 edges = 0, alive = 0;
         if (bottom_up) {
-            u64 *fmc = a.fm[L & 1], *fmn = a.fm[(L + 1) & 1];
-            if (!bu_prev) {  // the level's records -> dense frontier masks
-                for (u64 v = gtid; v < a.n; v += gthreads) fmc[v] = 0;
+            if (!bu_prev) {  // the level's records -> frontier bitmap and masks
+                u64 *fmc = a.fm[L & 1], *fbc = a.fb[L & 1];
+                const u64 nwords = (a.n + 63) / 64;
+                for (u64 w = gtid; w < nwords; w += gthreads) fbc[w] = 0;
                 batch_sync(ctl, 3, L);
-                for (u64 i = lo + gtid; i < hi; i += gthreads) fmc[a.lv[i]] = a.lm[i];
+                for (u64 i = lo + gtid; i < hi; i += gthreads) {
+                    const u32 v = a.lv[i];
+                    fmc[v] = a.lm[i];
+                    atomicOr((unsigned long long *)&fbc[v >> 6], 1ull << (v & 63));
+                }
                 batch_sync(ctl, 4, L);
             }
-            const u64 nround = (a.n + gthreads - 1) / gthreads * gthreads;
-            for (u64 u = gtid; u < nround; u += gthreads) {
-                u64 nb = 0;
-                if (u < a.n) {
-                    const u64 s = a.seen[u];
-                    const u64 need = alive_cur & ~s;
-                    if (need) {
-                        u64 acc = 0;
-                        for (u64 k = a.rs[u], e = a.rs[u + 1]; k < e; ++k) {
-                            acc |= fmc[a.col[k]];
-                            if ((acc & need) == need) break;
-                        }
-                        nb = acc & need;
-                        if (nb) a.seen[u] = s | nb;
-                    }
-                    fmn[u] = nb;
-                }
-                append_record(a, nb != 0, (u32)u, nb, nxt, (int)((L + 1) & 1), edges, alive);
+            const bool heavy = hub_count(ld_relaxed(&ctl->bhub)) > 0;
+            if (heavy) bu_heavy_chunks(a, L, alive_cur);
+            bu_groups(a, L, alive_cur, nxt, edges, alive, slot);
+            if (heavy) {
+                batch_sync(ctl, 8, L);
+                bu_heavy_fixup(a, L, alive_cur, nxt, edges, alive);
             }
             flush_level(a, nxt, edges, alive);
             batch_sync(ctl, 5, L);
@@ -289,8 +487,10 @@ __global__ __launch_bounds__(256) void batch_depth_fill(i64 *__restrict__ depth,
 }
 
 __global__ __launch_bounds__(256) void batch_depth_scatter(const u32 *__restrict__ lv, const u64 *__restrict__ lm,
-                                                           const u64 *__restrict__ level_start, u32 levels, int r,
+                                                           const u64 *__restrict__ level_start,
+                                                           const BatchCtl *__restrict__ ctl, int r,
                                                            i64 *__restrict__ depth) {
+    const u32 levels = ctl->levels;
     const u64 nrec = level_start[levels];
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < nrec; i += (u64)gridDim.x * blockDim.x) {
         if (!((lm[i] >> r) & 1)) continue;
@@ -302,6 +502,133 @@ __global__ __launch_bounds__(256) void batch_depth_scatter(const u32 *__restrict
         }
         depth[lv[i]] = lo;
     }
+}
+
+bool bfs_batch_enqueue(const Oned_Graph *g, Bfs_Batch_State *b, const u64 *roots, i32 nroots, hipStream_t stream,
+                       int grid) {
+    Context &c = ctx();
+    if (nroots < 1 || nroots > 64) {
+        fail("batched BFS takes 1..64 roots (got %d)", nroots);
+        return false;
+    }
+    for (i32 i = 0; i < nroots; ++i)
+        if (roots[i] >= g->nglobalverts) {
+            fail("BFS root %llu is outside graph vertex range [0, %llu)", (unsigned long long)roots[i],
+                 (unsigned long long)g->nglobalverts);
+            return false;
+        }
+    const u64 n = g->nlocalverts;
+    if (n > b->max_nlocalverts) {
+        fail("batched BFS state holds %llu vertices, graph has %llu", (unsigned long long)b->max_nlocalverts,
+             (unsigned long long)n);
+        return false;
+    }
+    const bool library = stream == nullptr;
+    const hipStream_t s = library ? c.stream : stream;
+    G500_HIP(hipMemsetAsync(b->ctl, 0, sizeof(BatchCtl), s));
+    G500_HIP(hipMemsetAsync(b->seen, 0, n * sizeof(u64), s));
+    G500_HIP(hipMemsetAsync(b->next, 0, n * sizeof(u64), s));
+    BatchArgs a;
+    a.n = n;
+    a.m_total = g->nlocaledges;
+    a.rs = g->rowstarts;
+    a.col = g->column;
+    a.seen = b->seen;
+    a.fm[0] = b->fmask[0];
+    a.fm[1] = b->fmask[1];
+    a.fb[0] = b->fbits[0];
+    a.fb[1] = b->fbits[1];
+    a.next = b->next;
+    a.touched = b->touched;
+    a.lv = b->log_vertex;
+    a.lm = b->log_mask;
+    a.level_start = b->level_start;
+    a.cap = b->log_capacity;
+    a.hubs = b->hubs;
+    a.hoff = b->hub_offsets;
+    a.hlen = b->hub_lengths;
+    a.ctl = (BatchCtl *)b->ctl;
+    memset(a.roots, 0, sizeof(a.roots));
+    for (i32 i = 0; i < nroots; ++i) a.roots[i] = roots[i];
+    a.nroots = nroots;
+    a.alpha = (f32)c.batch_alpha;
+    a.beta = (f32)c.batch_beta;
+    a.hub_degree = (u32)c.bfs_hub_degree;
+    a.heavy_degree = (u32)c.batch_heavy_degree;
+    const int block = 512;
+    if (grid <= 0) grid = cooperative_blocks((const void *)batch_bfs_kernel, block, 0);
+    if (failed()) return false;
+    void *args[] = {&a};
+    b->nroots = nroots;
+    if (library) {
+        timer_start();
+        launch_persistent((const void *)batch_bfs_kernel, grid, block, args);
+        timer_stop();
+    } else {
+        launch_persistent((const void *)batch_bfs_kernel, grid, block, args, s, false);
+    }
+    return !failed();
+}
+
+void bfs_batch_depth_enqueue(const Oned_Graph *g, const Bfs_Batch_State *b, i32 r, i64 *depth, hipStream_t stream) {
+    Context &c = ctx();
+    const hipStream_t s = stream ? stream : c.stream;
+    const u64 n = g->nlocalverts;
+    batch_depth_fill<<<grid_for(n, 256, (u64)c.num_cus * 32), 256, 0, s>>>(depth, n);
+    batch_depth_scatter<<<(unsigned)c.num_cus * 32, 256, 0, s>>>(b->log_vertex, b->log_mask, b->level_start,
+                                                                  (const BatchCtl *)b->ctl, r, depth);
+    G500_HIP(hipGetLastError());
+}
+
+u32 bfs_batch_error(const Bfs_Batch_State *b, hipStream_t stream) {
+    u32 e = 0;
+    G500_HIP(hipMemcpyAsync(&e, &((const BatchCtl *)b->ctl)->bar.error, sizeof(u32), hipMemcpyDeviceToHost,
+                            stream ? stream : ctx().stream));
+    G500_HIP(hipStreamSynchronize(stream ? stream : ctx().stream));
+    return e;
+}
+
+int bfs_batch_grid() { return cooperative_blocks((const void *)batch_bfs_kernel, 512, 0); }
+
+// Per-root summary of a finished batch: reached vertices and depth sum.
+__global__ __launch_bounds__(256) void batch_summary_kernel(const u64 *__restrict__ lm,
+                                                            const u64 *__restrict__ level_start,
+                                                            const BatchCtl *__restrict__ ctl, u64 *__restrict__ out) {
+    __shared__ unsigned long long cnt[64], sum[64];
+    if (threadIdx.x < 64) {
+        cnt[threadIdx.x] = 0;
+        sum[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    const u32 levels = ctl->levels;
+    const u64 nrec = level_start[levels];
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < nrec; i += (u64)gridDim.x * blockDim.x) {
+        u32 lo = 0, hi = levels;
+        while (hi - lo > 1) {
+            const u32 mid = (lo + hi) / 2;
+            if (level_start[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        for (u64 m = lm[i]; m; m &= m - 1) {
+            const int r = __ffsll((unsigned long long)m) - 1;
+            atomicAdd(&cnt[r], 1ull);
+            atomicAdd(&sum[r], (unsigned long long)lo);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 64 && cnt[threadIdx.x]) {
+        atomicAdd((unsigned long long *)&out[threadIdx.x], cnt[threadIdx.x]);
+        atomicAdd((unsigned long long *)&out[64 + threadIdx.x], sum[threadIdx.x]);
+    }
+}
+
+void bfs_batch_summary_enqueue(const Bfs_Batch_State *b, u64 *out, hipStream_t stream) {
+    Context &c = ctx();
+    const hipStream_t s = stream ? stream : c.stream;
+    G500_HIP(hipMemsetAsync(out, 0, 128 * sizeof(u64), s));
+    batch_summary_kernel<<<(unsigned)c.num_cus * 4, 256, 0, s>>>(b->log_mask, b->level_start,
+                                                                 (const BatchCtl *)b->ctl, out);
+    G500_HIP(hipGetLastError());
 }
 
 }  // namespace g500
@@ -319,14 +646,17 @@ Bfs_Batch_State oned_graph_bfs_batch_create(Worker_State *ws, const Oned_Graph *
     b.seen = (u64 *)arena_alloc(ws, n * sizeof(u64), "batch seen");
     b.fmask[0] = (u64 *)arena_alloc(ws, n * sizeof(u64), "batch frontier masks");
     b.fmask[1] = (u64 *)arena_alloc(ws, n * sizeof(u64), "batch frontier masks");
+    const u64 nwords = (n + 63) / 64 + 1;
+    b.fbits[0] = (u64 *)arena_alloc(ws, nwords * sizeof(u64), "batch frontier bitmaps");
+    b.fbits[1] = (u64 *)arena_alloc(ws, nwords * sizeof(u64), "batch frontier bitmaps");
     b.next = (u64 *)arena_alloc(ws, n * sizeof(u64), "batch next masks");
     b.touched = (u32 *)arena_alloc(ws, n * sizeof(u32), "batch touched");
     b.log_vertex = (u32 *)arena_alloc(ws, b.log_capacity * sizeof(u32), "batch log vertices");
     b.log_mask = (u64 *)arena_alloc(ws, b.log_capacity * sizeof(u64), "batch log masks");
     b.level_start = (u64 *)arena_alloc(ws, (n + 2) * sizeof(u64), "batch level starts");
-    b.hubs = (u32 *)arena_alloc(ws, 2 * n * sizeof(u32), "batch hubs");
-    b.hub_offsets = (u64 *)arena_alloc(ws, 2 * n * sizeof(u64), "batch hub offsets");
-    b.hub_lengths = (u32 *)arena_alloc(ws, 2 * n * sizeof(u32), "batch hub lengths");
+    b.hubs = (u32 *)arena_alloc(ws, 3 * n * sizeof(u32), "batch hubs");
+    b.hub_offsets = (u64 *)arena_alloc(ws, 3 * n * sizeof(u64), "batch hub offsets");
+    b.hub_lengths = (u32 *)arena_alloc(ws, 3 * n * sizeof(u32), "batch hub lengths");
     b.ctl = arena_alloc(ws, sizeof(BatchCtl), "batch control");
     return b;
 }
@@ -334,57 +664,9 @@ Bfs_Batch_State oned_graph_bfs_batch_create(Worker_State *ws, const Oned_Graph *
 void oned_graph_bfs_batch_run(Worker_State *ws, const Oned_Graph *g, Bfs_Batch_State *b, const u64 *roots,
                               i32 nroots) {
     (void)ws;
-    Context &c = ctx();
-    if (nroots < 1 || nroots > 64) {
-        fail("batched BFS takes 1..64 roots (got %d)", nroots);
-        return;
-    }
-    for (i32 i = 0; i < nroots; ++i)
-        if (roots[i] >= g->nglobalverts) {
-            fail("BFS root %llu is outside graph vertex range [0, %llu)", (unsigned long long)roots[i],
-                 (unsigned long long)g->nglobalverts);
-            return;
-        }
-    const u64 n = g->nlocalverts;
+    if (!bfs_batch_enqueue(g, b, roots, nroots, nullptr, 0)) return;
     static BatchCtl h;
-    memset(&h, 0, sizeof(h));
-    for (i32 i = 0; i < nroots; ++i) h.roots[i] = roots[i];
-    const hipStream_t s = c.stream;
-    G500_HIP(hipMemcpyAsync(b->ctl, &h, sizeof(h), hipMemcpyHostToDevice, s));
-    G500_HIP(hipMemsetAsync(b->seen, 0, n * sizeof(u64), s));
-    G500_HIP(hipMemsetAsync(b->next, 0, n * sizeof(u64), s));
-    sync();
-    BatchArgs a;
-    a.n = n;
-    a.m_total = g->nlocaledges;
-    a.rs = g->rowstarts;
-    a.col = g->column;
-    a.seen = b->seen;
-    a.fm[0] = b->fmask[0];
-    a.fm[1] = b->fmask[1];
-    a.next = b->next;
-    a.touched = b->touched;
-    a.lv = b->log_vertex;
-    a.lm = b->log_mask;
-    a.level_start = b->level_start;
-    a.cap = b->log_capacity;
-    a.hubs = b->hubs;
-    a.hoff = b->hub_offsets;
-    a.hlen = b->hub_lengths;
-    a.ctl = (BatchCtl *)b->ctl;
-    a.nroots = nroots;
-    a.alpha = (f32)c.batch_alpha;
-    a.beta = (f32)c.batch_beta;
-    a.hub_degree = (u32)c.bfs_hub_degree;
-    const int block = 512;
-    int grid = cooperative_blocks((const void *)batch_bfs_kernel, block, 0);
-    if (failed()) return;
-    void *args[] = {&a};
-    timer_start();
-    launch_persistent((const void *)batch_bfs_kernel, grid, block, args);
-    timer_stop();
     g500_copy_to_host(&h, b->ctl, sizeof(h));
-    b->nroots = nroots;
     b->levels = (i32)h.levels;
     if (h.bar.error)
         fail("batched BFS kernel error %u (1 barrier timeout, 2 log or hub overflow)", h.bar.error);
@@ -393,29 +675,23 @@ void oned_graph_bfs_batch_run(Worker_State *ws, const Oned_Graph *g, Bfs_Batch_S
 void oned_graph_bfs_batch_depth(Worker_State *ws, const Oned_Graph *g, const Bfs_Batch_State *b, i32 r,
                                 i64 *depth) {
     (void)ws;
-    Context &c = ctx();
     if (r < 0 || r >= b->nroots) {
         fail("root index %d outside the last batch of %d roots", r, b->nroots);
         return;
     }
-    const u64 n = g->nlocalverts;
-    batch_depth_fill<<<grid_for(n, 256, (u64)c.num_cus * 32), 256, 0, c.stream>>>(depth, n);
-    u64 nrec = 0;
-    G500_HIP(hipMemcpyAsync(&nrec, b->level_start + b->levels, sizeof(u64), hipMemcpyDeviceToHost, c.stream));
-    sync();
-    batch_depth_scatter<<<grid_for(nrec, 256, (u64)c.num_cus * 32), 256, 0, c.stream>>>(
-        b->log_vertex, b->log_mask, b->level_start, (u32)b->levels, r, depth);
-    G500_HIP(hipGetLastError());
+    bfs_batch_depth_enqueue(g, b, r, depth, nullptr);
     sync();
 }
 
-i32 g500_bfs_batch_level_log(const Bfs_Batch_State *b, i32 *modes, u64 *records, f64 *ms, i32 max_levels) {
+i32 g500_bfs_batch_level_log(const Bfs_Batch_State *b, i32 *modes, u64 *records, u64 *edges, f64 *ms,
+                             i32 max_levels) {
     static BatchCtl h;
     g500_copy_to_host(&h, b->ctl, sizeof(h));
     const i32 L = (i32)MIN((u32)h.levels, (u32)kBatchLevelLog);
     for (i32 i = 0; i < L && i < max_levels; ++i) {
         if (modes) modes[i] = h.mode_log[i];
         if (records) records[i] = h.size_log[i];
+        if (edges) edges[i] = h.edge_log[i];
         if (ms) ms[i] = (f64)h.tick_log[i] / ctx().wall_clock_khz;
     }
     return L;

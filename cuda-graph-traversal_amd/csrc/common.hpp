@@ -36,6 +36,7 @@ struct Context {
     int bfs_hub_degree = 2048;    // frontier vertices above this degree are split grid-wide
     double batch_alpha = 15.0;    // the same two thresholds for the batched multi-root BFS
     double batch_beta = 18.0;
+    int batch_heavy_degree = 1024; // batched bottom-up: vertices above this degree are split grid-wide
     int sssp_block = 512;
     int coop_launch = 1;          // 1: cooperative launch first; 0: plain launch of the co-resident grid
     int gen_blocks = 0;           // 0: one block per 256-edge tile (capped)
@@ -68,12 +69,32 @@ inline void sync() { G500_HIP(hipStreamSynchronize(ctx().stream)); }
 // Grid size for a cooperative (persistent) kernel: every block co-resident.
 int cooperative_blocks(const void *kernel, int block, size_t dyn_lds);
 
-// Launches a persistent kernel whose grid fits on the device at once. Uses a
-// cooperative launch; if the runtime refuses it (the cooperative queue is held
-// by another process on the same device), launches the same co-resident grid
-// normally. The kernels' grid barriers are bounded, so a grid that is not
-// fully resident ends with an error instead of waiting forever.
-void launch_persistent(const void *kernel, int grid, int block, void **args);
+// Launches a persistent kernel whose grid fits on the device at once. With
+// `cooperative`, uses a cooperative launch; if the runtime refuses it (the
+// cooperative queue is held by another process on the same device), or
+// without `cooperative` (several persistent kernels sharing the device, each
+// with its share of the co-resident grid), launches the grid normally. The
+// kernels' grid barriers are bounded, so a grid that is not fully resident
+// ends with an error instead of waiting forever. stream 0: the library stream.
+void launch_persistent(const void *kernel, int grid, int block, void **args, hipStream_t stream = nullptr,
+                       bool cooperative = true);
+
+// Asynchronous batched launches on a caller's stream with a caller's grid (0:
+// the full co-resident grid, cooperative, on the library stream with timing).
+// They validate like the blocking entry points and return false on failure.
+bool bfs_batch_enqueue(const Oned_Graph *g, Bfs_Batch_State *b, const u64 *roots, i32 nroots, hipStream_t stream,
+                       int grid);
+void bfs_batch_depth_enqueue(const Oned_Graph *g, const Bfs_Batch_State *b, i32 r, i64 *depth, hipStream_t stream);
+u32 bfs_batch_error(const Bfs_Batch_State *b, hipStream_t stream);  // synchronizes the stream
+int bfs_batch_grid();
+bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roots, i32 nroots, hipStream_t stream,
+                        int grid);
+u32 sssp_batch_error(const Sssp_Batch_State *s, hipStream_t stream);  // synchronizes the stream
+int sssp_batch_grid();
+// Per-root summaries of the last batch: out[r] = reached vertices, out[R + r] =
+// depth sum (BFS, R = 64) or distance sum in 2^-24 fixed point (SSSP, R = 16).
+void bfs_batch_summary_enqueue(const Bfs_Batch_State *b, u64 *out, hipStream_t stream);
+void sssp_batch_summary_enqueue(const Oned_Graph *g, const Sssp_Batch_State *s, u64 *out, hipStream_t stream);
 
 inline unsigned int grid_for(u64 work, int block, u64 cap = 1u << 20) {
     u64 b = (work + block - 1) / block;
@@ -152,11 +173,12 @@ constexpr u64 kHubTotalMask = (1ull << kHubCountShift) - 1;
 __device__ __forceinline__ u64 hub_count(u64 packed) { return packed >> kHubCountShift; }
 __device__ __forceinline__ u64 hub_total(u64 packed) { return packed & kHubTotalMask; }
 
-// Wave-converged append of (v, len) for lanes with `want`.
-__device__ __forceinline__ void hub_append(u64 *counter, u32 *hv, u64 *hoff, u32 *hlen, bool want, u32 v,
-                                           u32 len, u64 cap, u32 *error) {
+// Wave-converged append of (v, len) for lanes with `want`; returns the lane's
+// slot (~0 when it appended nothing).
+__device__ __forceinline__ u64 hub_append(u64 *counter, u32 *hv, u64 *hoff, u32 *hlen, bool want, u32 v,
+                                          u32 len, u64 cap, u32 *error) {
     const u64 m = ballot(want);
-    if (!m) return;
+    if (!m) return ~0ull;
     const u64 padded = want ? (u64)(len + kHubChunk - 1) / kHubChunk * kHubChunk : 0;
     const u64 incl = wave_inclusive_scan(padded);
     const u64 total = __shfl(incl, kWave - 1);
@@ -169,12 +191,14 @@ __device__ __forceinline__ void hub_append(u64 *counter, u32 *hv, u64 *hoff, u32
         const u64 slot = hub_count(old) + mask_rank(m);
         if (slot >= cap) {
             atomicOr(error, 2u);
-            return;
+            return ~0ull;
         }
         hv[slot] = v;
         hoff[slot] = hub_total(old) + incl - padded;
         hlen[slot] = len;
+        return slot;
     }
+    return ~0ull;
 }
 
 // Largest i in [lo, n) with hoff[i] <= p, searching forward from lo

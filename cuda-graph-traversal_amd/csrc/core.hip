@@ -54,10 +54,12 @@ int cooperative_blocks(const void *kernel, int block, size_t dyn_lds) {
     return per_cu * ctx().num_cus;
 }
 
-void launch_persistent(const void *kernel, int grid, int block, void **args) {
+void launch_persistent(const void *kernel, int grid, int block, void **args, hipStream_t stream,
+                       bool cooperative) {
     Context &c = ctx();
-    if (c.coop_launch) {
-        hipError_t e = hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(block), args, 0, c.stream);
+    if (!stream) stream = c.stream;
+    if (cooperative && c.coop_launch) {
+        hipError_t e = hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(block), args, 0, stream);
         if (e == hipSuccess) return;
         (void)hipGetLastError();
         static bool warned = false;
@@ -67,7 +69,7 @@ void launch_persistent(const void *kernel, int grid, int block, void **args) {
             warned = true;
         }
     }
-    G500_HIP(hipLaunchKernel(kernel, dim3(grid), dim3(block), args, 0, c.stream));
+    G500_HIP(hipLaunchKernel(kernel, dim3(grid), dim3(block), args, 0, stream));
 }
 
 void init_mrg_table();  // generator.hip
@@ -255,6 +257,7 @@ i32 g500_set_param(const char *name, f64 value) {
     else if (n == "bfs_beta") c.bfs_beta = value;
     else if (n == "batch_alpha") c.batch_alpha = value;
     else if (n == "batch_beta") c.batch_beta = value;
+    else if (n == "batch_heavy_degree") c.batch_heavy_degree = (int)value;
     else if (n == "bfs_block") c.bfs_block = (int)value;
     else if (n == "bfs_hub_degree") c.bfs_hub_degree = (int)value;
     else if (n == "sssp_block") c.sssp_block = (int)value;
@@ -271,6 +274,7 @@ i32 g500_get_param(const char *name, f64 *value) {
     else if (n == "bfs_beta") *value = c.bfs_beta;
     else if (n == "batch_alpha") *value = c.batch_alpha;
     else if (n == "batch_beta") *value = c.batch_beta;
+    else if (n == "batch_heavy_degree") *value = c.batch_heavy_degree;
     else if (n == "bfs_block") *value = c.bfs_block;
     else if (n == "bfs_hub_degree") *value = c.bfs_hub_degree;
     else if (n == "sssp_block") *value = c.sssp_block;

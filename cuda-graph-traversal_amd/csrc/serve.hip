@@ -1,0 +1,437 @@
+// Serving loop: resident graphs, (graph, root, BFS | SSSP) requests and
+// continuous batching. No counterpart in the reference (its runner is a batch
+// program over 64 roots, examples/graph500_runner.cu:257-441).
+//
+// Requests enter a pending queue. Each execution slot (a host thread, a HIP
+// stream, and one batched BFS and one batched SSSP state sized for the largest
+// graph) repeatedly takes the oldest pending request's (graph, kind) and as
+// many further pending requests of that pair as a launch holds (64 BFS roots,
+// 16 SSSP roots) and as the result pool admits: each request reserves its
+// result footprint (BFS: i64 depths; SSSP: f32 distances + i64 parents) in a
+// device pool, and a request that does not fit waits until completions free
+// memory. Requests that arrive while launches run join the next launch of a
+// free slot. The slot launches the batch on its stream with its share of the
+// co-resident grid, so several batches run on the device at once; when the
+// batch ends it stages every request's results in the pool (device copies),
+// computes per-request summaries, and hands the requests to a copy thread,
+// which copies the results to the callers' host buffers, frees the pool
+// memory and completes the requests.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "common.hpp"
+
+namespace g500 {
+
+namespace {
+
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+    u64 id;
+    i32 graph, kind;
+    u64 root;
+    void *out0, *out1;
+    u64 off0, off1;  // pool offsets of the staged results (~0: none)
+    u64 bytes0, bytes1;
+    double t_submit, t_launch, t_staged;
+    i32 batch;
+    i32 status;
+    u64 reached;
+    f64 sum;
+};
+
+// First-fit allocator over the device result pool (256-byte granules).
+class Pool {
+public:
+    void init(u64 bytes) {
+        size_ = bytes;
+        free_.clear();
+        if (bytes) free_[0] = bytes;
+    }
+    u64 alloc(u64 bytes) {
+        bytes = (bytes + 255) & ~255ull;
+        if (!bytes) return 0;
+        for (auto it = free_.begin(); it != free_.end(); ++it)
+            if (it->second >= bytes) {
+                const u64 off = it->first, rest = it->second - bytes;
+                free_.erase(it);
+                if (rest) free_[off + bytes] = rest;
+                used_ += bytes;
+                return off;
+            }
+        return ~0ull;
+    }
+    void release(u64 off, u64 bytes) {
+        bytes = (bytes + 255) & ~255ull;
+        if (!bytes || off == ~0ull) return;
+        used_ -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto next = std::next(it);
+        if (next != free_.end() && it->first + it->second == next->first) {
+            it->second += next->second;
+            free_.erase(next);
+        }
+        if (it != free_.begin()) {
+            auto prev = std::prev(it);
+            if (prev->first + prev->second == it->first) {
+                prev->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    u64 size() const { return size_; }
+    u64 used() const { return used_; }
+
+private:
+    std::map<u64, u64> free_;
+    u64 size_ = 0, used_ = 0;
+};
+
+}  // namespace
+
+}  // namespace g500
+
+using namespace g500;
+
+struct G500_Server {
+    Worker_State *ws = nullptr;
+    int nslots = 0;
+    std::vector<Oned_Graph> graphs;
+    u8 *pool_base = nullptr;
+    Pool pool;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_copy, cv_done;
+    std::deque<Request *> pending, copying;
+    std::deque<G500_Completion> done;
+    u64 next_id = 1, submitted = 0, completed = 0, in_system = 0;
+    bool started = false, stopping = false;
+    struct Slot {
+        hipStream_t stream = nullptr;
+        Bfs_Batch_State bfs;
+        Sssp_Batch_State sssp;
+        u64 *summary = nullptr;  // device [128]
+        std::thread th;
+    };
+    std::vector<Slot> slots;
+    std::thread copier;
+    hipStream_t copy_stream = nullptr;
+    int grid_bfs = 0, grid_sssp = 0;
+    f32 delta = 0.1f;
+    u64 launches = 0, launched_requests = 0;
+};
+
+namespace {
+
+u64 footprint0(const G500_Server *s, const Request *r) {
+    const u64 n = s->graphs[r->graph].nlocalverts;
+    return r->kind == G500_SSSP ? n * sizeof(f32) : n * sizeof(i64);
+}
+u64 footprint1(const G500_Server *s, const Request *r) {
+    return r->kind == G500_SSSP && r->out1 ? s->graphs[r->graph].nlocalverts * sizeof(i64) : 0;
+}
+
+// Takes the next batch under the lock: the oldest request fixes (graph, kind);
+// later requests of the pair join while the launch has room and the pool
+// admits their footprint.
+std::vector<Request *> take_batch(G500_Server *s) {
+    std::vector<Request *> batch;
+    if (s->pending.empty()) return batch;
+    const i32 graph = s->pending.front()->graph, kind = s->pending.front()->kind;
+    const size_t cap = kind == G500_BFS ? 64 : 16;
+    for (auto it = s->pending.begin(); it != s->pending.end() && batch.size() < cap;) {
+        Request *r = *it;
+        if (r->graph != graph || r->kind != kind) {
+            ++it;
+            continue;
+        }
+        r->bytes0 = r->out0 ? footprint0(s, r) : 0;
+        r->bytes1 = footprint1(s, r);
+        r->off0 = r->bytes0 ? s->pool.alloc(r->bytes0) : ~0ull;
+        if (r->bytes0 && r->off0 == ~0ull) break;  // pool full: admit later
+        r->off1 = r->bytes1 ? s->pool.alloc(r->bytes1) : ~0ull;
+        if (r->bytes1 && r->off1 == ~0ull) {
+            s->pool.release(r->off0, r->bytes0);
+            break;
+        }
+        batch.push_back(r);
+        it = s->pending.erase(it);
+    }
+    return batch;
+}
+
+void slot_loop(G500_Server *s, int k) {
+    G500_HIP(hipSetDevice(ctx().device));
+    G500_Server::Slot &slot = s->slots[k];
+    std::vector<u64> roots;
+    u64 summary[128];
+    for (;;) {
+        std::vector<Request *> batch;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            for (;;) {
+                if (s->stopping) return;
+                batch = take_batch(s);
+                if (!batch.empty()) break;
+                s->cv_work.wait(lk);
+            }
+            ++s->launches;
+            s->launched_requests += batch.size();
+        }
+        const double t_launch = now_ms();
+        const Oned_Graph *g = &s->graphs[batch[0]->graph];
+        const u64 n = g->nlocalverts;
+        const bool bfs = batch[0]->kind == G500_BFS;
+        roots.clear();
+        for (Request *r : batch) roots.push_back(r->root);
+        u32 err;
+        if (bfs) {
+            bfs_batch_enqueue(g, &slot.bfs, roots.data(), (i32)roots.size(), slot.stream, s->grid_bfs);
+            err = bfs_batch_error(&slot.bfs, slot.stream);
+            for (size_t i = 0; i < batch.size(); ++i)
+                if (batch[i]->bytes0)
+                    bfs_batch_depth_enqueue(g, &slot.bfs, (i32)i, (i64 *)(s->pool_base + batch[i]->off0), slot.stream);
+            bfs_batch_summary_enqueue(&slot.bfs, slot.summary, slot.stream);
+        } else {
+            slot.sssp.delta = s->delta;
+            sssp_batch_enqueue(g, &slot.sssp, roots.data(), (i32)roots.size(), slot.stream, s->grid_sssp);
+            err = sssp_batch_error(&slot.sssp, slot.stream);
+            for (size_t i = 0; i < batch.size(); ++i) {
+                Request *r = batch[i];
+                if (r->bytes0)
+                    G500_HIP(hipMemcpyAsync(s->pool_base + r->off0, slot.sssp.dist + i * n, n * sizeof(f32),
+                                            hipMemcpyDeviceToDevice, slot.stream));
+                if (r->bytes1)
+                    G500_HIP(hipMemcpyAsync(s->pool_base + r->off1, slot.sssp.pred + i * n, n * sizeof(i64),
+                                            hipMemcpyDeviceToDevice, slot.stream));
+            }
+            sssp_batch_summary_enqueue(g, &slot.sssp, slot.summary, slot.stream);
+        }
+        G500_HIP(hipMemcpyAsync(summary, slot.summary, sizeof(summary), hipMemcpyDeviceToHost, slot.stream));
+        G500_HIP(hipStreamSynchronize(slot.stream));
+        const double t_staged = now_ms();
+        const int R = bfs ? 64 : 16;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            for (size_t i = 0; i < batch.size(); ++i) {
+                Request *r = batch[i];
+                r->t_launch = t_launch;
+                r->t_staged = t_staged;
+                r->batch = (i32)batch.size();
+                r->status = (i32)err;
+                r->reached = summary[i];
+                r->sum = bfs ? (f64)summary[R + i] : (f64)summary[R + i] / 16777216.0;
+                s->copying.push_back(r);
+            }
+        }
+        s->cv_copy.notify_one();
+    }
+}
+
+void copy_loop(G500_Server *s) {
+    G500_HIP(hipSetDevice(ctx().device));
+    for (;;) {
+        Request *r;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_copy.wait(lk, [&] { return s->stopping || !s->copying.empty(); });
+            if (s->copying.empty()) return;  // stopping
+            r = s->copying.front();
+            s->copying.pop_front();
+        }
+        if (r->bytes0)
+            G500_HIP(hipMemcpyAsync(r->out0, s->pool_base + r->off0, r->bytes0, hipMemcpyDeviceToHost, s->copy_stream));
+        if (r->bytes1)
+            G500_HIP(hipMemcpyAsync(r->out1, s->pool_base + r->off1, r->bytes1, hipMemcpyDeviceToHost, s->copy_stream));
+        G500_HIP(hipStreamSynchronize(s->copy_stream));
+        const double t_done = now_ms();
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->pool.release(r->off0, r->bytes0);
+            s->pool.release(r->off1, r->bytes1);
+            G500_Completion c;
+            c.id = r->id;
+            c.graph = r->graph;
+            c.kind = r->kind;
+            c.root = r->root;
+            c.status = r->status;
+            c.batch_size = r->batch;
+            c.reached = r->reached;
+            c.sum = r->sum;
+            c.latency_ms = t_done - r->t_submit;
+            c.queue_ms = r->t_launch - r->t_submit;
+            c.run_ms = r->t_staged - r->t_launch;
+            s->done.push_back(c);
+            ++s->completed;
+            --s->in_system;
+            delete r;
+        }
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();  // pool memory freed: waiting requests may be admitted
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+G500_Server *g500_server_create(Worker_State *ws, i32 slots, u64 result_pool_bytes) {
+    if (slots < 1 || slots > 8) {
+        fail("server slots must be 1..8 (got %d)", slots);
+        return nullptr;
+    }
+    G500_Server *s = new G500_Server;
+    s->ws = ws;
+    s->nslots = slots;
+    s->pool_base = (u8 *)arena_alloc(ws, result_pool_bytes ? result_pool_bytes : 256, "server result pool");
+    if (failed()) {
+        delete s;
+        return nullptr;
+    }
+    s->pool.init(result_pool_bytes);
+    return s;
+}
+
+i32 g500_server_add_graph(G500_Server *s, const Oned_Graph *g) {
+    if (s->started) {
+        fail("graphs are added before g500_server_start");
+        return -1;
+    }
+    s->graphs.push_back(*g);
+    return (i32)s->graphs.size() - 1;
+}
+
+i32 g500_server_start(G500_Server *s, f32 delta) {
+    if (s->started || s->graphs.empty()) {
+        fail("g500_server_start needs at least one graph and runs once");
+        return -1;
+    }
+    if (!(delta > 0.0f) || !(delta < 3.0e38f)) {
+        fail("SSSP delta must be positive and finite (got %g)", (double)delta);
+        return -1;
+    }
+    s->delta = delta;
+    const Oned_Graph *big = &s->graphs[0];
+    for (const Oned_Graph &g : s->graphs)
+        if (g.nlocalverts > big->nlocalverts) big = &g;
+    // each slot's persistent kernels get an equal share of the co-resident grid,
+    // so the launches of all slots are resident together
+    s->grid_bfs = std::max(1, bfs_batch_grid() / s->nslots);
+    s->grid_sssp = std::max(1, sssp_batch_grid() / s->nslots);
+    s->slots.resize(s->nslots);
+    for (auto &slot : s->slots) {
+        G500_HIP(hipStreamCreateWithFlags(&slot.stream, hipStreamNonBlocking));
+        slot.bfs = oned_graph_bfs_batch_create(s->ws, big);
+        slot.sssp = oned_graph_sssp_batch_create(s->ws, big);
+        slot.summary = (u64 *)arena_alloc(s->ws, 128 * sizeof(u64), "server summaries");
+    }
+    G500_HIP(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
+    if (failed()) return -1;
+    s->started = true;
+    for (int k = 0; k < s->nslots; ++k) s->slots[k].th = std::thread(slot_loop, s, k);
+    s->copier = std::thread(copy_loop, s);
+    return 0;
+}
+
+u64 g500_server_submit(G500_Server *s, i32 graph, i32 kind, u64 root, void *out0, void *out1) {
+    if (!s->started || graph < 0 || graph >= (i32)s->graphs.size() || (kind != G500_BFS && kind != G500_SSSP) ||
+        root >= s->graphs[graph].nglobalverts) {
+        fail("bad server request (graph %d kind %d root %llu)", graph, kind, (unsigned long long)root);
+        return 0;
+    }
+    const u64 n = s->graphs[graph].nlocalverts;
+    const u64 need = (out0 ? (kind == G500_SSSP ? n * sizeof(f32) : n * sizeof(i64)) : 0) +
+                     (kind == G500_SSSP && out1 ? n * sizeof(i64) : 0);
+    if (need + 512 > s->pool.size() && need) {
+        fail("request results (%llu bytes) exceed the server result pool (%llu bytes)", (unsigned long long)need,
+             (unsigned long long)s->pool.size());
+        return 0;
+    }
+    Request *r = new Request();
+    r->graph = graph;
+    r->kind = kind;
+    r->root = root;
+    r->out0 = out0;
+    r->out1 = kind == G500_SSSP ? out1 : nullptr;
+    r->off0 = r->off1 = ~0ull;
+    r->t_submit = now_ms();
+    u64 id;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        id = r->id = s->next_id++;
+        s->pending.push_back(r);
+        ++s->submitted;
+        ++s->in_system;
+    }
+    s->cv_work.notify_one();
+    return id;
+}
+
+void g500_server_replay(G500_Server *s, u64 count, const f64 *at_ms, const i32 *graph, const i32 *kind,
+                        const u64 *root) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (u64 i = 0; i < count; ++i) {
+        const auto due = t0 + std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                                  std::chrono::duration<double, std::milli>(at_ms[i]));
+        std::this_thread::sleep_until(due);
+        if (!g500_server_submit(s, graph[i], kind[i], root[i], nullptr, nullptr)) return;
+    }
+}
+
+i32 g500_server_poll(G500_Server *s, G500_Completion *out, i32 max, f64 wait_ms) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->done.empty() && wait_ms > 0)
+        s->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(wait_ms),
+                            [&] { return !s->done.empty(); });
+    i32 k = 0;
+    while (k < max && !s->done.empty()) {
+        out[k++] = s->done.front();
+        s->done.pop_front();
+    }
+    return k;
+}
+
+void g500_server_drain(G500_Server *s) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->in_system == 0; });
+}
+
+void g500_server_stats(const G500_Server *s, u64 *submitted, u64 *completed, u64 *launches, u64 *pool_used) {
+    G500_Server *m = const_cast<G500_Server *>(s);
+    std::lock_guard<std::mutex> lk(m->mu);
+    if (submitted) *submitted = s->submitted;
+    if (completed) *completed = s->completed;
+    if (launches) *launches = s->launches;
+    if (pool_used) *pool_used = s->pool.used();
+}
+
+void g500_server_destroy(G500_Server *s) {
+    if (!s) return;
+    if (s->started) {
+        g500_server_drain(s);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stopping = true;
+        }
+        s->cv_work.notify_all();
+        s->cv_copy.notify_all();
+        for (auto &slot : s->slots)
+            if (slot.th.joinable()) slot.th.join();
+        if (s->copier.joinable()) s->copier.join();
+        for (auto &slot : s->slots) (void)hipStreamDestroy(slot.stream);
+        (void)hipStreamDestroy(s->copy_stream);
+    }
+    for (Request *r : s->pending) delete r;
+    delete s;
+}
+
+}  // extern "C"

@@ -37,15 +37,12 @@
 // (slot x % 3), writes only slot (x + 1) % 3 and clears slot (x + 2) % 3, so
 // a block acting on a decision never changes a counter that a slower block
 // still has to read for the same decision.
-#include "common.hpp"
+#include "sssp_common.hpp"
 
 namespace g500 {
 
 constexpr u32 kSmall = 32;
 constexpr u32 kHub = 1024;
-constexpr u32 kInfBits = 0x7F800000u;
-constexpr u64 kInfLabel = ((u64)kInfBits << 32) | 0xFFFFFFFFull;
-
 // Error bits: 1 barrier timeout, 2 hub list overflow, 4 queue entry out of
 // range, 8 queue length above n, 16 bucket invariant violated.
 struct SsspCtl {
@@ -96,28 +93,7 @@ struct SsspArgs {
     f32 delta;
 };
 
-__device__ __forceinline__ f32 label_dist(u64 l) { return __uint_as_float((u32)(l >> 32)); }
-
-// The reference's next bucket bound, fl(hi + delta), forced to progress.
-__device__ __forceinline__ f32 next_bound(f32 hi, f32 delta) {
-    const f32 x = hi + delta;
-    return x > hi ? x : __uint_as_float(__float_as_uint(hi) + 1);
-}
-
 __device__ __forceinline__ int tier_of(u32 len) { return len <= kSmall ? 0 : (len <= kHub ? 1 : 2); }
-
-// Strictly lowers the label of v to (cand, parent); true if this call did it.
-__device__ __forceinline__ bool relax(u64 *lab, u32 v, f32 cand, u32 parent) {
-    const u32 cb = __float_as_uint(cand);
-    const u64 want = ((u64)cb << 32) | parent;
-    u64 old = lab[v];
-    while ((u32)(old >> 32) > cb) {
-        const u64 prev = atomicCAS((unsigned long long *)&lab[v], (unsigned long long)old, (unsigned long long)want);
-        if (prev == old) return true;
-        old = prev;
-    }
-    return false;
-}
 
 // Where the pushes of a step go: queue buffer, counter slot and tag of the
 // queue, and the bound below which a lowered vertex is queued.
@@ -158,7 +134,7 @@ __device__ __forceinline__ void visit_edge(const SsspArgs &a, const Target &t, b
         v = a.col[k];
         const f32 cand = du + a.w[k];
         // a settled vertex has d_v < floor <= cand: skip its label
-        if (!((a.settled[v >> 6] >> (v & 63)) & 1) && relax(a.lab, v, cand, u)) {
+        if (!((a.settled[v >> 6] >> (v & 63)) & 1) && relax(&a.lab[v], cand, u)) {
             ++improved;
             push = cand < t.below;
             if (cand < t.floor) atomicOr(&a.ctl->bar.error, 16u);
@@ -469,6 +445,13 @@ __global__ __launch_bounds__(256) void sssp_clear_kernel(u64 *__restrict__ lab, 
     }
 }
 
+void compute_split(const u64 *rs, const f32 *w, u64 n, f32 delta, u32 *split, hipStream_t stream) {
+    Context &c = ctx();
+    sssp_split_kernel<<<grid_for(n, 256, (u64)c.num_cus * 32), 256, 0, stream ? stream : c.stream>>>(rs, w, n,
+                                                                                                  delta, split);
+    G500_HIP(hipGetLastError());
+}
+
 }  // namespace g500
 
 using namespace g500;
@@ -522,9 +505,7 @@ void oned_graph_sssp_run(Worker_State *ws, const Oned_Graph *g, Sssp_State *s, u
     }
     const u64 n = g->nlocalverts;
     if (s->split_delta != s->delta) {  // light/heavy split of every row for this delta
-        sssp_split_kernel<<<grid_for(n, 256, (u64)c.num_cus * 32), 256, 0, c.stream>>>(g->rowstarts, g->weights, n,
-                                                                                      s->delta, s->split);
-        G500_HIP(hipGetLastError());
+        compute_split(g->rowstarts, g->weights, n, s->delta, s->split);
         s->split_delta = s->delta;
     }
     SsspArgs a;

@@ -1,0 +1,682 @@
+// Batched delta-stepping SSSP: up to 16 roots in one persistent cooperative
+// kernel with one bucket sequence shared by all roots. No counterpart in the
+// reference, whose runner solves one root at a time
+// (examples/graph500_runner.cu:393-416); each root's distances equal the
+// single-root kernel's, because every terminating label-correcting schedule
+// reaches the same FP32 fixed point.
+//
+// Labels are vertex-major: lab[v * 16 + r] = (distance bits << 32) | parent of
+// vertex v from root r, so the 16 labels of a vertex share one 128-byte line.
+// Buckets are [lo, hi) with hi = fl(lo + delta) (traversal.cu:693-694), the
+// same sequence for every root. Work items are (vertex, root mask) pairs: one
+// pass over a row serves every root of the mask, and relaxing an edge for
+// several roots touches one line of the target. Per bucket:
+//   light steps: items (u, M) with M = pend[u], the roots whose label of u went
+//     below hi in the previous step; their light edges (w < delta, a prefix of
+//     the weight-sorted row) are relaxed for every root of M; a label lowered
+//     below hi ORs its root into the target's pending mask for the next step,
+//     and the first OR queues the target;
+//   dense step: 16 lanes per vertex read its labels; roots with d in [lo, hi)
+//     form the vertex's heavy item, roots with d in [hi, fl(hi + delta)) are
+//     queued for the next bucket; after a barrier the heavy items relax their
+//     heavy edges (never into [lo, hi), as for one root);
+//   when the next bucket is empty: a min step over all labels and a collect
+//     step queueing the bucket that holds the minimum.
+// Work is mapped by segment length: a lane per item (<= 32 entries), a wave
+// per item (<= 1024) or 256-entry chunks over the grid. Every chunk of a hub
+// item reads its mask; the last chunk to finish clears it.
+// Counters rotate over three slots per step (written by the step's producer,
+// read by its consumer, cleared two steps ahead), as in the single-root kernel.
+#include "sssp_common.hpp"
+
+namespace g500 {
+
+constexpr int kBatchRoots = 16;  // label stride and largest batch
+constexpr u32 kBSmall = 32;
+constexpr u32 kBHub = 1024;
+
+// Error bits: 1 barrier timeout, 2 hub list overflow, 4 queue entry out of
+// range, 8 queue length above n, 16 bucket invariant violated.
+struct SsspBatchCtl {
+    GridBarrier bar;
+    alignas(256) u32 flen[3][2];  // light queue lengths [slot][tier 0-1]
+    u64 fhub[3];                  // light queue hub counters [slot]
+    u32 hlen[3][2];               // heavy item list lengths [slot][tier 0-1]
+    u64 hhub[3];                  // heavy hub counters [slot]
+    u32 nmin[3];                  // smallest distance bits >= hi, per slot
+    alignas(64) u64 buckets, light_iters, improvements, scans;
+    u64 ticks[4];                 // constant-clock ticks of block 0: light, dense+heavy, min+collect, kernel
+    alignas(256) BlockTrace trace[kMaxTraceBlocks];
+};
+
+struct SsspBatchArgs {
+    u64 n;
+    const u64 *rs;
+    const u32 *col;
+    const f32 *w;
+    const u32 *split;
+    u64 *lab;          // [n * 16]
+    u32 *pend[2];      // [n] pending root masks, by step parity
+    u32 *fq[2][3];     // light queues [buffer][tier]; tier 2 holds hub vertices
+    u64 *fq_off[2];    // light hub tier: work offsets
+    u32 *fq_len[2];    //   segment lengths
+    u32 *fq_done[2];   //   finished chunks
+    u64 *hq[2];        // heavy items (mask << 32 | v), tiers 0 and 1
+    u32 *hqh;          // heavy hub tier: vertices,
+    u64 *hq_off;       //   work offsets,
+    u32 *hq_len;       //   segment lengths,
+    u32 *hq_mask;      //   root masks
+    SsspBatchCtl *ctl;
+    f32 *dist;         // [nroots * n], root-major
+    i64 *pred;         // [nroots * n]
+    u64 roots[kBatchRoots];
+    i32 nroots;
+    f32 delta;
+};
+
+__device__ __forceinline__ void bsync(SsspBatchCtl *ctl, u32 phase, u32 a, u32 b) {
+    grid_sync(&ctl->bar, &ctl->bar.error);
+    if (threadIdx.x == 0 && blockIdx.x < kMaxTraceBlocks) {
+        BlockTrace *t = &ctl->trace[blockIdx.x];
+        st_relaxed(&t->barriers, ld_relaxed(&t->barriers) + 1);
+        st_relaxed(&t->phase, phase);
+        st_relaxed(&t->a, a);
+        st_relaxed(&t->b, b);
+    }
+}
+
+// Where the pushes of a step go: pending-mask and queue buffer, counter slot,
+// the bound below which a lowered label is queued, and the bucket floor.
+struct BTarget {
+    int buf, slot;
+    f32 below, floor;
+};
+
+__device__ __forceinline__ int btier(u32 len) { return len <= kBSmall ? 0 : (len <= kBHub ? 1 : 2); }
+
+// Wave-converged: ORs `mask` into v's pending mask; the first OR of the step
+// queues v in the tier of its light segment.
+__device__ __forceinline__ void push_item(const SsspBatchArgs &a, u32 v, u32 mask, const BTarget &t) {
+    bool q = false;
+    u32 lw = 0;
+    if (mask) {
+        q = atomicOr(&a.pend[t.buf][v], mask) == 0;
+        if (q) lw = a.split[v];
+    }
+    if (!ballot(q)) return;
+    SsspBatchCtl *ctl = a.ctl;
+#pragma unroll
+    for (int tier = 0; tier < 2; ++tier) {
+        const bool qt = q && btier(lw) == tier;
+        const u32 s = wave_append(&ctl->flen[t.slot][tier], qt);
+        if (qt) a.fq[t.buf][tier][s] = v;
+    }
+    hub_append(&ctl->fhub[t.slot], a.fq[t.buf][2], a.fq_off[t.buf], a.fq_len[t.buf], q && btier(lw) == 2, v, lw,
+               a.n, &ctl->bar.error);
+}
+
+// Relaxes entry k of u's row for the roots of M; returns the roots whose label
+// of the target went below t.below, and the target in v.
+__device__ __forceinline__ u32 relax_entry(const SsspBatchArgs &a, u32 u, u32 M, u64 k, const BTarget &t, u32 &v,
+                                           u64 &improved) {
+    v = a.col[k];
+    const f32 wk = a.w[k];
+    const u64 *lu = a.lab + (u64)u * kBatchRoots;
+    u64 *lv = a.lab + (u64)v * kBatchRoots;
+    u32 out = 0;
+    for (u32 m = M; m; m &= m - 1) {
+        const int r = __ffs(m) - 1;
+        const f32 cand = label_dist(lu[r]) + wk;
+        if (relax(&lv[r], cand, u)) {
+            ++improved;
+            if (cand < t.below) out |= 1u << r;
+            if (cand < t.floor) atomicOr(&a.ctl->bar.error, 16u);
+        }
+    }
+    return out;
+}
+
+template <bool LIGHT>
+__device__ __forceinline__ void bsegment(const SsspBatchArgs &a, u32 u, u64 &b, u64 &e) {
+    const u64 r0 = a.rs[u], s = a.split[u];
+    b = LIGHT ? r0 : r0 + s;
+    e = LIGHT ? r0 + s : a.rs[u + 1];
+}
+
+// Light queue tiers 0 (a lane per item) and 1 (a wave per item); the item's
+// mask is read from pend and cleared.
+__device__ void light_tier(const SsspBatchArgs &a, int buf, const u32 *list, u32 count, int tier, const BTarget &t,
+                           u64 &improved) {
+    const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
+    const u32 lane = lane_id();
+    const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
+    u32 *pend = a.pend[buf];
+    if (tier == 0) {
+        for (u64 i0 = wave_gid * kWave; i0 < count; i0 += nwaves * kWave) {
+            const u64 i = i0 + lane;
+            u64 b = 0, e = 0;
+            u32 u = 0, M = 0;
+            if (i < count) {
+                u = list[i];
+                if (u < a.n) {
+                    M = pend[u];
+                    pend[u] = 0;
+                    bsegment<true>(a, u, b, e);
+                } else {
+                    atomicOr(&a.ctl->bar.error, 4u);
+                }
+            }
+            const u32 len = M ? (u32)(e - b) : 0;
+            u32 mx = len;
+            for (int o = 32; o > 0; o >>= 1) mx = max(mx, (u32)__shfl_xor((int)mx, o));
+            for (u32 j = 0; j < mx; ++j) {
+                u32 v = 0, out = 0;
+                if (j < len) out = relax_entry(a, u, M, b + j, t, v, improved);
+                push_item(a, v, out, t);
+            }
+        }
+    } else {
+        for (u64 i = wave_gid; i < count; i += nwaves) {
+            const u32 u = list[i];
+            if (u >= a.n) {
+                if (lane == 0) atomicOr(&a.ctl->bar.error, 4u);
+                continue;
+            }
+            const u32 M = __builtin_amdgcn_readfirstlane(pend[u]);
+            if (lane == 0) pend[u] = 0;
+            u64 b, e;
+            bsegment<true>(a, u, b, e);
+            for (u64 k0 = b; k0 < e; k0 += kWave) {
+                const u64 k = k0 + lane;
+                u32 v = 0, out = 0;
+                if (k < e) out = relax_entry(a, u, M, k, t, v, improved);
+                push_item(a, v, out, t);
+            }
+        }
+    }
+}
+
+// Heavy item tiers 0 and 1: entries (mask << 32 | v).
+__device__ void heavy_tier(const SsspBatchArgs &a, const u64 *list, u32 count, int tier, const BTarget &t,
+                           u64 &improved) {
+    const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
+    const u32 lane = lane_id();
+    const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
+    if (tier == 0) {
+        for (u64 i0 = wave_gid * kWave; i0 < count; i0 += nwaves * kWave) {
+            const u64 i = i0 + lane;
+            u64 b = 0, e = 0;
+            u32 u = 0, M = 0;
+            if (i < count) {
+                const u64 it = list[i];
+                u = (u32)it;
+                M = (u32)(it >> 32);
+                if (u < a.n) {
+                    bsegment<false>(a, u, b, e);
+                } else {
+                    atomicOr(&a.ctl->bar.error, 4u);
+                    M = 0;
+                }
+            }
+            const u32 len = M ? (u32)(e - b) : 0;
+            u32 mx = len;
+            for (int o = 32; o > 0; o >>= 1) mx = max(mx, (u32)__shfl_xor((int)mx, o));
+            for (u32 j = 0; j < mx; ++j) {
+                u32 v = 0, out = 0;
+                if (j < len) out = relax_entry(a, u, M, b + j, t, v, improved);
+                push_item(a, v, out, t);
+            }
+        }
+    } else {
+        for (u64 i = wave_gid; i < count; i += nwaves) {
+            const u64 it = list[i];
+            const u32 u = (u32)it, M = (u32)(it >> 32);
+            if (u >= a.n) {
+                if (lane == 0) atomicOr(&a.ctl->bar.error, 4u);
+                continue;
+            }
+            u64 b, e;
+            bsegment<false>(a, u, b, e);
+            for (u64 k0 = b; k0 < e; k0 += kWave) {
+                const u64 k = k0 + lane;
+                u32 v = 0, out = 0;
+                if (k < e) out = relax_entry(a, u, M, k, t, v, improved);
+                push_item(a, v, out, t);
+            }
+        }
+    }
+}
+
+// Hub tier: chunks of the flattened work space. Light hubs read their mask
+// from pend (cleared by the item's last chunk); heavy hubs from `masks`.
+template <bool LIGHT>
+__device__ void hub_tier(const SsspBatchArgs &a, const u32 *hv, const u64 *hoff, const u32 *hlen, u32 *aux,
+                         u32 *pend, u64 packed, const BTarget &t, u64 &improved) {
+    const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
+    const u32 lane = lane_id();
+    const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
+    const u64 count = hub_count(packed), total = hub_total(packed);
+    u64 i = 0;
+    for (u64 c = wave_gid; c * kHubChunk < total; c += nwaves) {
+        const u64 p = c * kHubChunk;
+        i = hub_locate(hoff, i, count, p);
+        const u32 u = hv[i];
+        if (u >= a.n) {
+            if (lane == 0) atomicOr(&a.ctl->bar.error, 4u);
+            continue;
+        }
+        const u32 M = __builtin_amdgcn_readfirstlane(LIGHT ? pend[u] : aux[i]);
+        const u64 local = p - hoff[i];
+        const u32 len = hlen[i];
+        u64 b, e;
+        bsegment<LIGHT>(a, u, b, e);
+#pragma unroll
+        for (u64 j = 0; j < kHubChunk; j += kWave) {
+            const u64 off = local + j + lane;
+            u32 v = 0, out = 0;
+            if (off < len) out = relax_entry(a, u, M, b + off, t, v, improved);
+            push_item(a, v, out, t);
+        }
+        if (LIGHT && lane == 0) {
+            const u32 chunks = (len + (u32)kHubChunk - 1) / (u32)kHubChunk;
+            if (atomicAdd(&aux[i], 1u) + 1 == chunks) {  // every chunk has read the mask
+                pend[u] = 0;
+                aux[i] = 0;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(512) void sssp_batch_kernel(SsspBatchArgs a) {
+    SsspBatchCtl *ctl = a.ctl;
+    const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
+    const u32 lane = lane_id();
+    // dense passes: 16 lanes per vertex, uniform trip count
+    const u64 nlab = a.n * kBatchRoots;
+    const u64 nround = (nlab + gthreads - 1) / gthreads * gthreads;
+    u64 improved = 0;
+    u32 x = 0;
+    f32 lo = 0.0f, hi = next_bound(0.0f, a.delta);
+
+    const bool timer = gtid == 0;
+    u64 t_prev = timer ? wall_clock64() : 0, t_start = t_prev, t_acc[3] = {0, 0, 0};
+    auto tick = [&](int phase) {
+        if (timer) {
+            const u64 t = wall_clock64();
+            t_acc[phase] += t - t_prev;
+            t_prev = t;
+        }
+    };
+    auto clear_ahead = [&](u32 step) {
+        if (gtid == 0) {
+            const int s = (int)((step + 2) % 3);
+            st_relaxed(&ctl->flen[s][0], 0u);
+            st_relaxed(&ctl->flen[s][1], 0u);
+            st_relaxed(&ctl->fhub[s], 0ull);
+            st_relaxed(&ctl->hlen[s][0], 0u);
+            st_relaxed(&ctl->hlen[s][1], 0u);
+            st_relaxed(&ctl->hhub[s], 0ull);
+            st_relaxed(&ctl->nmin[s], 0xFFFFFFFFu);
+        }
+    };
+    auto queue_size = [&](u32 step, u32 *cnt, u64 *hub) {
+        u64 total = 0;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            u32 c = ld_relaxed(&ctl->flen[step % 3][t]);
+            if (c > a.n) {
+                if (gtid == 0) atomicOr(&ctl->bar.error, 8u);
+                c = 0;
+            }
+            cnt[t] = c;
+            total += c;
+        }
+        *hub = ld_relaxed(&ctl->fhub[step % 3]);
+        return total + hub_count(*hub);
+    };
+    auto target = [&](u32 step, f32 below, f32 floor) {
+        return BTarget{(int)((step + 1) & 1), (int)((step + 1) % 3), below, floor};
+    };
+    // this lane's label in a dense pass: vertex t / 16, root t % 16
+    auto dense_label = [&](u64 t, u32 &bits) {
+        bits = kInfBits;
+        if (t < nlab && (int)(t & (kBatchRoots - 1)) < a.nroots) bits = (u32)(a.lab[t] >> 32);
+    };
+
+    if (gtid < (u64)a.nroots) {
+        const u64 root = a.roots[gtid];
+        st_relaxed(&a.lab[root * kBatchRoots + gtid], root);  // distance 0.0f, parent = root
+    }
+    if (gtid == 0)
+        for (int i = 0; i < 3; ++i) st_relaxed(&ctl->nmin[i], 0xFFFFFFFFu);
+    bsync(ctl, 1, 0, 0);
+    if (gtid < kWave) {  // step 0 queues the roots (repeated roots share one item)
+        const bool r = (i32)gtid < a.nroots;
+        push_item(a, r ? (u32)a.roots[gtid] : 0u, r ? 1u << gtid : 0u, target(0, 0.f, 0.f));
+    }
+    bsync(ctl, 1, 1, 0);
+    x = 1;
+
+    for (;;) {  // buckets [lo, hi)
+        for (;;) {  // light steps
+            u32 cnt[2];
+            u64 hub;
+            const u64 total = queue_size(x, cnt, &hub);
+            clear_ahead(x);
+            if (total == 0 || ld_relaxed(&ctl->bar.error)) break;
+            if (gtid == 0) atomicAdd((unsigned long long *)&ctl->light_iters, 1ull);
+            const BTarget t = target(x, hi, lo);
+            const int buf = (int)(x & 1);
+#pragma unroll
+            for (int tier = 0; tier < 2; ++tier) light_tier(a, buf, a.fq[buf][tier], cnt[tier], tier, t, improved);
+            hub_tier<true>(a, a.fq[buf][2], a.fq_off[buf], a.fq_len[buf], a.fq_done[buf], a.pend[buf], hub, t,
+                           improved);
+            bsync(ctl, 2, __float_as_uint(lo), x);
+            tick(0);
+            ++x;
+        }
+        if (ld_relaxed(&ctl->bar.error)) break;
+        // dense step x: heavy items of [lo, hi), next bucket [hi, hi2) queued
+        if (gtid == 0) atomicAdd((unsigned long long *)&ctl->buckets, 1ull);
+        const f32 hi2 = next_bound(hi, a.delta);
+        {
+            const BTarget t = target(x, hi2, hi);
+            const int hs = (int)((x + 1) % 3);
+            for (u64 tt = gtid; tt < nround; tt += gthreads) {
+                u32 bits;
+                dense_label(tt, bits);
+                bool mine = false, nxt = false;
+                if (bits < kInfBits) {
+                    const f32 d = __uint_as_float(bits);
+                    mine = d >= lo && d < hi;
+                    nxt = d >= hi && d < hi2;
+                }
+                const int sh = (int)(lane & 48);
+                const u32 H = (u32)(ballot(mine) >> sh) & 0xFFFFu, N = (u32)(ballot(nxt) >> sh) & 0xFFFFu;
+                const bool leader = (lane & 15) == 0 && tt < nlab;
+                const u32 v = (u32)(tt / kBatchRoots);
+                push_item(a, v, leader ? N : 0u, t);
+                u64 b = 0, e = 0;
+                if (leader && H) bsegment<false>(a, v, b, e);
+                const u32 len = (u32)(e - b);
+                const bool item = leader && H && len > 0;
+#pragma unroll
+                for (int tier = 0; tier < 2; ++tier) {
+                    const bool want = item && btier(len) == tier;
+                    const u32 s = wave_append(&ctl->hlen[hs][tier], want);
+                    if (want) a.hq[tier][s] = ((u64)H << 32) | v;
+                }
+                const u64 slot = hub_append(&ctl->hhub[hs], a.hqh, a.hq_off, a.hq_len, item && btier(len) == 2, v,
+                                            len, a.n, &ctl->bar.error);
+                if (slot != ~0ull) a.hq_mask[slot] = H;
+            }
+            bsync(ctl, 3, __float_as_uint(lo), x);
+#pragma unroll
+            for (int tier = 0; tier < 2; ++tier) {
+                const u32 c = min(ld_relaxed(&ctl->hlen[hs][tier]), (u32)a.n);
+                heavy_tier(a, a.hq[tier], c, tier, t, improved);
+            }
+            hub_tier<false>(a, a.hqh, a.hq_off, a.hq_len, a.hq_mask, nullptr, ld_relaxed(&ctl->hhub[hs]), t,
+                            improved);
+            bsync(ctl, 4, __float_as_uint(lo), x);
+            tick(1);
+            if (gtid == 0) atomicAdd((unsigned long long *)&ctl->scans, 1ull);
+            ++x;
+        }
+        {
+            u32 cnt[2];
+            u64 hub;
+            if (queue_size(x, cnt, &hub)) {  // next bucket is not empty
+                lo = hi;
+                hi = hi2;
+                continue;
+            }
+        }
+        if (ld_relaxed(&ctl->bar.error)) break;
+        // min step: smallest distance >= hi over all labels
+        clear_ahead(x);
+        for (u64 tt = gtid; tt < nround; tt += gthreads) {
+            u32 bits;
+            dense_label(tt, bits);
+            u32 m = 0xFFFFFFFFu;
+            if (bits < kInfBits && __uint_as_float(bits) >= hi) m = bits;
+            m = wave_min(m);
+            if (lane == 0 && m != 0xFFFFFFFFu) atomicMin(&ctl->nmin[(x + 1) % 3], m);
+        }
+        bsync(ctl, 5, __float_as_uint(lo), x);
+        ++x;
+        const u32 nm = ld_relaxed(&ctl->nmin[x % 3]);
+        if (gtid == 0) atomicAdd((unsigned long long *)&ctl->scans, 1ull);
+        if (nm == 0xFFFFFFFFu) {
+            tick(2);
+            break;
+        }
+        const f32 dmin = __uint_as_float(nm);
+        lo = hi;
+        hi = next_bound(lo, a.delta);
+        while (dmin >= hi) {
+            lo = hi;
+            hi = next_bound(lo, a.delta);
+        }
+        // collect step: queue the bucket [lo, hi) that holds dmin
+        clear_ahead(x);
+        {
+            const BTarget t = target(x, hi, lo);
+            for (u64 tt = gtid; tt < nround; tt += gthreads) {
+                u32 bits;
+                dense_label(tt, bits);
+                bool in = false;
+                if (bits < kInfBits) {
+                    const f32 d = __uint_as_float(bits);
+                    in = d >= lo && d < hi;
+                }
+                const u32 M = (u32)(ballot(in) >> (lane & 48)) & 0xFFFFu;
+                const bool leader = (lane & 15) == 0 && tt < nlab;
+                push_item(a, (u32)(tt / kBatchRoots), leader ? M : 0u, t);
+            }
+        }
+        bsync(ctl, 6, __float_as_uint(lo), x);
+        if (gtid == 0) atomicAdd((unsigned long long *)&ctl->scans, 1ull);
+        tick(2);
+        ++x;
+    }
+    improved = wave_sum(improved);
+    if (lane == 0 && improved) atomicAdd((unsigned long long *)&ctl->improvements, (unsigned long long)improved);
+    if (timer) {
+        for (int i = 0; i < 3; ++i) st_relaxed(&ctl->ticks[i], t_acc[i]);
+        st_relaxed(&ctl->ticks[3], wall_clock64() - t_start);
+    }
+    // export, root-major (traversal.cu:333-340): unreached -> -1.0f / -1
+    for (u64 tt = gtid; tt < nlab; tt += gthreads) {
+        const u64 v = tt / kBatchRoots;
+        const int r = (int)(tt & (kBatchRoots - 1));
+        if (r >= a.nroots) continue;
+        const u64 l = a.lab[tt];
+        const bool reached = (u32)(l >> 32) < kInfBits;
+        a.dist[(u64)r * a.n + v] = reached ? label_dist(l) : -1.0f;
+        a.pred[(u64)r * a.n + v] = reached ? (i64)(u32)l : -1;
+    }
+}
+
+__global__ __launch_bounds__(256) void sssp_batch_clear_kernel(u64 *__restrict__ lab, u32 *__restrict__ pend,
+                                                               u32 *__restrict__ done, u64 n) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x; t < n * kBatchRoots; t += stride) {
+        lab[t] = kInfLabel;
+        if (t < 2 * n) {
+            pend[t] = 0;
+            done[t] = 0;
+        }
+    }
+}
+
+// Per-root summary of a finished batch: reached vertices and the distance sum
+// in 2^-24 fixed point (an integer sum, so it does not depend on the order).
+__global__ __launch_bounds__(256) void sssp_batch_summary_kernel(const u64 *__restrict__ lab, u64 n, i32 nroots,
+                                                                 u64 *__restrict__ out) {
+    __shared__ unsigned long long cnt[kBatchRoots], sum[kBatchRoots];
+    if (threadIdx.x < kBatchRoots) {
+        cnt[threadIdx.x] = 0;
+        sum[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    for (u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x; t < n * kBatchRoots; t += (u64)gridDim.x * blockDim.x) {
+        const int r = (int)(t & (kBatchRoots - 1));
+        if (r >= nroots) continue;
+        const u32 bits = (u32)(lab[t] >> 32);
+        if (bits >= kInfBits) continue;
+        atomicAdd(&cnt[r], 1ull);
+        atomicAdd(&sum[r], (unsigned long long)(__uint_as_float(bits) * 16777216.0f));
+    }
+    __syncthreads();
+    if (threadIdx.x < kBatchRoots && cnt[threadIdx.x]) {
+        atomicAdd((unsigned long long *)&out[threadIdx.x], cnt[threadIdx.x]);
+        atomicAdd((unsigned long long *)&out[kBatchRoots + threadIdx.x], sum[threadIdx.x]);
+    }
+}
+
+void sssp_batch_summary_enqueue(const Oned_Graph *g, const Sssp_Batch_State *s, u64 *out, hipStream_t stream) {
+    Context &c = ctx();
+    const hipStream_t st = stream ? stream : c.stream;
+    G500_HIP(hipMemsetAsync(out, 0, 2 * kBatchRoots * sizeof(u64), st));
+    sssp_batch_summary_kernel<<<(unsigned)c.num_cus * 8, 256, 0, st>>>(s->labels, g->nlocalverts, s->nroots, out);
+    G500_HIP(hipGetLastError());
+}
+
+bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roots, i32 nroots, hipStream_t stream,
+                        int grid) {
+    Context &c = ctx();
+    if (nroots < 1 || nroots > kBatchRoots) {
+        fail("batched SSSP takes 1..%d roots (got %d)", kBatchRoots, nroots);
+        return false;
+    }
+    for (i32 i = 0; i < nroots; ++i)
+        if (roots[i] >= g->nglobalverts) {
+            fail("SSSP root %llu is outside graph vertex range [0, %llu)", (unsigned long long)roots[i],
+                 (unsigned long long)g->nglobalverts);
+            return false;
+        }
+    if (!(s->delta > 0.0f) || !(s->delta < 3.0e38f)) {
+        fail("SSSP delta must be positive and finite (got %g)", (double)s->delta);
+        return false;
+    }
+    const u64 n = g->nlocalverts;
+    if (n > s->max_nlocalverts) {
+        fail("batched SSSP state holds %llu vertices, graph has %llu", (unsigned long long)s->max_nlocalverts,
+             (unsigned long long)n);
+        return false;
+    }
+    const bool library = stream == nullptr;
+    const hipStream_t st = library ? c.stream : stream;
+    // the split depends on the graph too: a state reused across graphs recomputes it
+    if (s->split_delta != s->delta || s->split_graph != (const void *)g->rowstarts) {
+        compute_split(g->rowstarts, g->weights, n, s->delta, s->split, st);
+        s->split_delta = s->delta;
+        s->split_graph = (const void *)g->rowstarts;
+    }
+    sssp_batch_clear_kernel<<<grid_for(n * kBatchRoots, 256, (u64)c.num_cus * 64), 256, 0, st>>>(
+        s->labels, s->pending, s->hub_aux, n);
+    G500_HIP(hipGetLastError());
+    G500_HIP(hipMemsetAsync(s->ctl, 0, sizeof(SsspBatchCtl), st));
+    SsspBatchArgs a;
+    a.n = n;
+    a.rs = g->rowstarts;
+    a.col = g->column;
+    a.w = g->weights;
+    a.split = s->split;
+    a.lab = s->labels;
+    for (int b = 0; b < 2; ++b) {
+        a.pend[b] = s->pending + (u64)b * n;
+        for (int t = 0; t < 3; ++t) a.fq[b][t] = s->frontier + (u64)(b * 3 + t) * n;
+        a.fq_off[b] = s->hub_offsets + (u64)b * n;
+        a.fq_len[b] = s->hub_lengths + (u64)b * n;
+        a.fq_done[b] = s->hub_aux + (u64)b * n;
+        a.hq[b] = s->heavy + (u64)b * n;
+    }
+    a.hqh = s->hub_vertices;
+    a.hq_off = s->hub_offsets + 2 * n;
+    a.hq_len = s->hub_lengths + 2 * n;
+    a.hq_mask = s->hub_aux + 2 * n;
+    a.ctl = (SsspBatchCtl *)s->ctl;
+    a.dist = s->dist;
+    a.pred = s->pred;
+    memset(a.roots, 0, sizeof(a.roots));
+    for (i32 i = 0; i < nroots; ++i) a.roots[i] = roots[i];
+    a.nroots = nroots;
+    a.delta = s->delta;
+    const int block = 512;
+    if (grid <= 0) grid = cooperative_blocks((const void *)sssp_batch_kernel, block, 0);
+    if (failed()) return false;
+    void *args[] = {&a};
+    s->nroots = nroots;
+    if (library) {
+        timer_start();
+        launch_persistent((const void *)sssp_batch_kernel, grid, block, args);
+        timer_stop();
+    } else {
+        launch_persistent((const void *)sssp_batch_kernel, grid, block, args, st, false);
+    }
+    return !failed();
+}
+
+u32 sssp_batch_error(const Sssp_Batch_State *s, hipStream_t stream) {
+    u32 e = 0;
+    const hipStream_t st = stream ? stream : ctx().stream;
+    G500_HIP(hipMemcpyAsync(&e, &((const SsspBatchCtl *)s->ctl)->bar.error, sizeof(u32), hipMemcpyDeviceToHost, st));
+    G500_HIP(hipStreamSynchronize(st));
+    return e;
+}
+
+int sssp_batch_grid() { return cooperative_blocks((const void *)sssp_batch_kernel, 512, 0); }
+
+}  // namespace g500
+
+using namespace g500;
+
+extern "C" {
+
+Sssp_Batch_State oned_graph_sssp_batch_create(Worker_State *ws, const Oned_Graph *g) {
+    Sssp_Batch_State s;
+    memset(&s, 0, sizeof(s));
+    const u64 n = g->nlocalverts;
+    s.max_nlocalverts = n;
+    s.delta = 0.1f;
+    s.dist = (f32 *)arena_alloc(ws, kBatchRoots * n * sizeof(f32), "batched SSSP dist");
+    s.pred = (i64 *)arena_alloc(ws, kBatchRoots * n * sizeof(i64), "batched SSSP pred");
+    s.labels = (u64 *)arena_alloc(ws, kBatchRoots * n * sizeof(u64), "batched SSSP labels");
+    s.pending = (u32 *)arena_alloc(ws, 2 * n * sizeof(u32), "batched SSSP pending masks");
+    s.frontier = (u32 *)arena_alloc(ws, 6 * n * sizeof(u32), "batched SSSP queues");
+    s.heavy = (u64 *)arena_alloc(ws, 2 * n * sizeof(u64), "batched SSSP heavy items");
+    s.hub_vertices = (u32 *)arena_alloc(ws, n * sizeof(u32), "batched SSSP heavy hub vertices");
+    s.hub_offsets = (u64 *)arena_alloc(ws, 3 * n * sizeof(u64), "batched SSSP hub offsets");
+    s.hub_lengths = (u32 *)arena_alloc(ws, 3 * n * sizeof(u32), "batched SSSP hub lengths");
+    s.hub_aux = (u32 *)arena_alloc(ws, 3 * n * sizeof(u32), "batched SSSP hub chunk counts / masks");
+    s.split = (u32 *)arena_alloc(ws, n * sizeof(u32), "batched SSSP light split");
+    s.ctl = arena_alloc(ws, sizeof(SsspBatchCtl), "batched SSSP control");
+    return s;
+}
+
+void oned_graph_sssp_batch_run(Worker_State *ws, const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roots,
+                               i32 nroots) {
+    (void)ws;
+    if (!sssp_batch_enqueue(g, s, roots, nroots, nullptr, 0)) return;
+    const u32 e = sssp_batch_error(s, nullptr);
+    if (e)
+        fail("batched SSSP kernel error %u (1 barrier timeout, 2 hub list overflow, 4 bad queue entry, 8 queue "
+             "overflow, 16 bucket invariant)",
+             e);
+}
+
+void oned_graph_sssp_batch_destroy(Sssp_Batch_State *s) {
+    if (s) memset(s, 0, sizeof(*s));
+}
+
+void g500_sssp_batch_run_stats(const Sssp_Batch_State *s, u64 *buckets, u64 *light_iterations, u64 *relaxations) {
+    static SsspBatchCtl h;
+    g500_copy_to_host(&h, s->ctl, sizeof(h));
+    if (buckets) *buckets = h.buckets;
+    if (light_iterations) *light_iterations = h.light_iters;
+    if (relaxations) *relaxations = h.improvements;
+}
+
+}  // extern "C"

@@ -55,9 +55,25 @@ class SsspState(C.Structure):
 
 
 class BfsBatchState(C.Structure):
-    _fields_ = [("seen", vp), ("fmask", vp * 2), ("next", vp), ("touched", vp), ("log_vertex", vp),
+    _fields_ = [("seen", vp), ("fmask", vp * 2), ("fbits", vp * 2), ("next", vp), ("touched", vp), ("log_vertex", vp),
                 ("log_mask", vp), ("level_start", vp), ("hubs", vp), ("hub_offsets", vp), ("hub_lengths", vp),
                 ("ctl", vp), ("log_capacity", u64), ("max_nlocalverts", u64), ("nroots", i32), ("levels", i32)]
+
+
+class SsspBatchState(C.Structure):
+    _fields_ = [("dist", vp), ("pred", vp), ("labels", vp), ("pending", vp), ("frontier", vp), ("heavy", vp),
+                ("hub_vertices", vp), ("hub_offsets", vp), ("hub_lengths", vp), ("hub_aux", vp), ("split", vp),
+                ("ctl", vp), ("split_graph", vp), ("delta", f32), ("split_delta", f32), ("nroots", i32),
+                ("reserved", i32),
+                ("max_nlocalverts", u64)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("id", u64), ("graph", i32), ("kind", i32), ("root", u64), ("status", i32), ("batch_size", i32),
+                ("reached", u64), ("sum", f64), ("latency_ms", f64), ("queue_ms", f64), ("run_ms", f64)]
+
+
+BFS, SSSP = 0, 1
 
 
 P = C.POINTER
@@ -104,7 +120,20 @@ _SIGS = {
     "oned_graph_bfs_batch_run": (None, [P(WorkerState), P(OnedGraph), P(BfsBatchState), P(u64), i32]),
     "oned_graph_bfs_batch_depth": (None, [P(WorkerState), P(OnedGraph), P(BfsBatchState), i32, vp]),
     "oned_graph_bfs_batch_destroy": (None, [P(BfsBatchState)]),
-    "g500_bfs_batch_level_log": (i32, [P(BfsBatchState), P(i32), P(u64), P(f64), i32]),
+    "g500_bfs_batch_level_log": (i32, [P(BfsBatchState), P(i32), P(u64), P(u64), P(f64), i32]),
+    "oned_graph_sssp_batch_create": (SsspBatchState, [P(WorkerState), P(OnedGraph)]),
+    "oned_graph_sssp_batch_run": (None, [P(WorkerState), P(OnedGraph), P(SsspBatchState), P(u64), i32]),
+    "oned_graph_sssp_batch_destroy": (None, [P(SsspBatchState)]),
+    "g500_sssp_batch_run_stats": (None, [P(SsspBatchState), P(u64), P(u64), P(u64)]),
+    "g500_server_create": (vp, [P(WorkerState), i32, u64]),
+    "g500_server_add_graph": (i32, [vp, P(OnedGraph)]),
+    "g500_server_start": (i32, [vp, f32]),
+    "g500_server_submit": (u64, [vp, i32, i32, u64, vp, vp]),
+    "g500_server_replay": (None, [vp, u64, P(f64), P(i32), P(i32), P(u64)]),
+    "g500_server_poll": (i32, [vp, P(Completion), i32, f64]),
+    "g500_server_drain": (None, [vp]),
+    "g500_server_stats": (None, [vp, P(u64), P(u64), P(u64), P(u64)]),
+    "g500_server_destroy": (None, [vp]),
     "g500_set_error_mode": (None, [i32]),
     "g500_status": (i32, []),
     "g500_last_error": (C.c_char_p, []),
@@ -301,12 +330,39 @@ class Session:
         self.check()
         return self.to_host(p, n, np.int64)
 
-    def batch_level_log(self, b, max_levels=256):
+    def batch_level_log(self, b, max_levels=256, with_edges=False):
         modes = (i32 * max_levels)()
         recs = (u64 * max_levels)()
+        edges = (u64 * max_levels)()
         ms = (f64 * max_levels)()
-        n = self.lib.g500_bfs_batch_level_log(C.byref(b), modes, recs, ms, max_levels)
+        n = self.lib.g500_bfs_batch_level_log(C.byref(b), modes, recs, edges, ms, max_levels)
+        if with_edges:
+            return list(modes[:n]), list(recs[:n]), list(edges[:n]), list(ms[:n])
         return list(modes[:n]), list(recs[:n]), list(ms[:n])
+
+    def sssp_batch_create(self, g, delta=0.1):
+        s = self.lib.oned_graph_sssp_batch_create(C.byref(self.ws), C.byref(g))
+        self.check()
+        s.delta = delta
+        return s
+
+    def sssp_batch(self, g, s, roots):
+        r = np.ascontiguousarray(roots, np.uint64)
+        self.lib.oned_graph_sssp_batch_run(C.byref(self.ws), C.byref(g), C.byref(s), r.ctypes.data_as(P(u64)),
+                                           int(r.size))
+        self.check()
+        return self.lib.g500_last_run_ms()
+
+    def sssp_batch_result(self, g, s, i):
+        """Distances and parents from root index i of the last batched run."""
+        n = g.nlocalverts
+        return (self.to_host(s.dist + 4 * n * i, n, np.float32),
+                self.to_host(s.pred + 8 * n * i, n, np.int64))
+
+    def sssp_batch_stats(self, s):
+        b, li, r = u64(), u64(), u64()
+        self.lib.g500_sssp_batch_run_stats(C.byref(s), C.byref(b), C.byref(li), C.byref(r))
+        return b.value, li.value, r.value
 
     def sssp_phase_ms(self, s):
         out = (f64 * 5)()
@@ -338,6 +394,34 @@ class Session:
         b, li, r = u64(), u64(), u64()
         self.lib.g500_sssp_run_stats(C.byref(s), C.byref(b), C.byref(li), C.byref(r))
         return b.value, li.value, r.value
+
+    # ---- serving
+    def server(self, graphs, slots=2, pool_bytes=1 << 30, delta=0.1):
+        """Starts a server over resident graphs (OnedGraph structs); returns its handle."""
+        h = self.lib.g500_server_create(C.byref(self.ws), slots, pool_bytes)
+        self.check()
+        for g in graphs:
+            self.lib.g500_server_add_graph(h, C.byref(g))
+            self.check()
+        self.lib.g500_server_start(h, delta)
+        self.check()
+        return h
+
+    def server_submit(self, h, graph, kind, root, out0=None, out1=None):
+        rid = self.lib.g500_server_submit(h, graph, kind, int(root), None if out0 is None else _ptr(out0),
+                                          None if out1 is None else _ptr(out1))
+        self.check()
+        return rid
+
+    def server_poll(self, h, max_items=4096, wait_ms=0.0):
+        buf = (Completion * max_items)()
+        k = self.lib.g500_server_poll(h, buf, max_items, wait_ms)
+        return [buf[i] for i in range(k)]
+
+    def server_stats(self, h):
+        a, b, c, d = u64(), u64(), u64(), u64()
+        self.lib.g500_server_stats(h, C.byref(a), C.byref(b), C.byref(c), C.byref(d))
+        return {"submitted": a.value, "completed": b.value, "launches": c.value, "pool_used": d.value}
 
     def set_param(self, name, value):
         if self.lib.g500_set_param(name.encode(), float(value)):

@@ -40,15 +40,17 @@ void get_statistics(const f64 *x, i32 n, f64 r[7]);
 typedef struct Bfs_Batch_State Bfs_Batch_State;
 struct Bfs_Batch_State {
     u64 *seen;         /* [n] roots that reached each vertex */
-    u64 *fmask[2];     /* [n] per-level frontier masks (dense, bottom-up levels) */
+    u64 *fmask[2];     /* [n] per-level frontier masks (bottom-up levels) */
+    u64 *fbits[2];     /* [ceil(n/64)+1] frontier bitmaps matching fmask */
     u64 *next;         /* [n] top-down accumulation of the next level */
     u32 *touched;      /* [n] vertices touched in a top-down level */
     u32 *log_vertex;   /* [log_capacity] discovery log: vertex */
     u64 *log_mask;     /* [log_capacity] discovery log: newly reaching roots */
     u64 *level_start;  /* [n + 2] first log record of each level */
-    u32 *hubs;         /* [2n] hub-tier record indices (two consecutive levels) */
-    u64 *hub_offsets;  /* [2n] */
-    u32 *hub_lengths;  /* [2n] */
+    u32 *hubs;         /* [3n] hub lists: records of two consecutive top-down
+                        * levels, then the heavy vertices of bottom-up levels */
+    u64 *hub_offsets;  /* [3n] */
+    u32 *hub_lengths;  /* [3n] */
     void *ctl;
     u64 log_capacity;
     u64 max_nlocalverts;
@@ -65,9 +67,99 @@ void oned_graph_bfs_batch_depth(Worker_State *ws, const Oned_Graph *g, const Bfs
                                 i64 *depth);
 void oned_graph_bfs_batch_destroy(Bfs_Batch_State *b);
 /* Per-level log of the last batched run: mode (0 top-down, 1 bottom-up),
- * record count and device milliseconds (any pointer may be NULL). Returns
- * the number of levels. */
-i32 g500_bfs_batch_level_log(const Bfs_Batch_State *b, i32 *modes, u64 *records, f64 *ms, i32 max_levels);
+ * record count, degree sum of the records and device milliseconds (any
+ * pointer may be NULL). Returns the number of levels. */
+i32 g500_bfs_batch_level_log(const Bfs_Batch_State *b, i32 *modes, u64 *records, u64 *edges, f64 *ms,
+                             i32 max_levels);
+
+/* ---- batched delta-stepping SSSP ---------------------------------------- */
+/* Up to 16 roots in one persistent kernel with one bucket sequence shared by
+ * all roots; items are (vertex, root mask) pairs and the 16 labels of a vertex
+ * share one 128-byte line. No counterpart in the reference (its runner solves
+ * one root at a time, graph500_runner.cu:393-416); each root's distances equal
+ * the single-root oned_graph_sssp_run's (the FP32 fixed point is unique), and
+ * its parents satisfy the same certificate. */
+typedef struct Sssp_Batch_State Sssp_Batch_State;
+struct Sssp_Batch_State {
+    f32 *dist;          /* [16n] root-major: root index r, vertex v at r * n + v; -1.0f unreached */
+    i64 *pred;          /* [16n] root-major parents; -1 unreached; root -> root */
+    u64 *labels;        /* [16n] vertex-major working labels (v * 16 + r) */
+    u32 *pending;       /* [2n] pending root masks, two steps */
+    u32 *frontier;      /* [6n] light queues: 2 buffers x 3 work tiers */
+    u64 *heavy;         /* [2n] heavy items (mask << 32 | vertex), two work tiers */
+    u32 *hub_vertices;  /* [n] heavy hub tier */
+    u64 *hub_offsets;   /* [3n] hub work offsets (2 light queues, heavy) */
+    u32 *hub_lengths;   /* [3n] hub segment lengths */
+    u32 *hub_aux;       /* [3n] light: finished chunks per hub; heavy: root masks */
+    u32 *split;         /* [n] light entries (w < split_delta) per row */
+    void *ctl;          /* device control block */
+    const void *split_graph; /* row offsets that `split` was computed for */
+    f32 delta;          /* bucket width (default 0.1f) */
+    f32 split_delta;
+    i32 nroots;         /* roots of the last run */
+    i32 reserved;
+    u64 max_nlocalverts;
+};
+Sssp_Batch_State oned_graph_sssp_batch_create(Worker_State *ws, const Oned_Graph *g);
+/* roots: host array of nroots (1..16) vertex ids (repeats allowed). */
+void oned_graph_sssp_batch_run(Worker_State *ws, const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roots,
+                               i32 nroots);
+void oned_graph_sssp_batch_destroy(Sssp_Batch_State *s);
+/* Bucket / light-step / label-improvement counts of the last batched run
+ * (improvements summed over the roots). */
+void g500_sssp_batch_run_stats(const Sssp_Batch_State *s, u64 *buckets, u64 *light_iterations,
+                               u64 *relaxations);
+
+/* ---- serving loop -------------------------------------------------------- */
+/* Resident graphs, (graph, root, BFS | SSSP) requests and continuous batching.
+ * No counterpart in the reference (its runner is a batch program). Execution
+ * slots each own a HIP stream and batched BFS / SSSP states; a free slot takes
+ * the oldest pending request's (graph, kind) plus further pending requests of
+ * that pair (up to 64 BFS / 16 SSSP roots) that the device result pool admits
+ * (footprint per request: BFS n * 8 bytes of depths; SSSP n * 4 bytes of
+ * distances + n * 8 bytes of parents when requested). Slots run concurrently,
+ * each with its share of the device. Results are those of the batched entry
+ * points, so a request's result does not depend on what it was batched with. */
+typedef struct G500_Server G500_Server;
+enum { G500_BFS = 0, G500_SSSP = 1 };
+typedef struct G500_Completion G500_Completion;
+struct G500_Completion {
+    u64 id;            /* from g500_server_submit */
+    i32 graph, kind;
+    u64 root;
+    i32 status;        /* 0: ok; otherwise the kernel's error bits */
+    i32 batch_size;    /* requests in the launch that served it */
+    u64 reached;       /* vertices reached from the root */
+    f64 sum;           /* BFS: sum of depths; SSSP: sum of distances (2^-24 fixed point) */
+    f64 latency_ms;    /* submit -> results in the caller's buffers */
+    f64 queue_ms;      /* submit -> launch */
+    f64 run_ms;        /* launch -> results staged on the device */
+};
+/* slots: 1..8 concurrent launches; result_pool_bytes: device memory for staged
+ * results (requests wait for admission when it is full). */
+G500_Server *g500_server_create(Worker_State *ws, i32 slots, u64 result_pool_bytes);
+/* Registers a resident graph (before start); returns its id. */
+i32 g500_server_add_graph(G500_Server *s, const Oned_Graph *g);
+/* Allocates the slot states (sized for the largest graph) and starts the
+ * threads; delta is the SSSP bucket width. Returns 0, or -1 on error. */
+i32 g500_server_start(G500_Server *s, f32 delta);
+/* Queues a request; thread-safe. out0: host buffer for BFS depths (i64[n]) or
+ * SSSP distances (f32[n]); out1: SSSP parents (i64[n]); either may be NULL
+ * (then only the summary is returned). The buffers must stay valid until the
+ * completion is polled. Returns the request id, 0 on error. */
+u64 g500_server_submit(G500_Server *s, i32 graph, i32 kind, u64 root, void *out0, void *out1);
+/* Submits `count` requests at the given offsets (ms after the call), from the
+ * calling thread, without result buffers; returns when all are submitted. */
+void g500_server_replay(G500_Server *s, u64 count, const f64 *at_ms, const i32 *graph, const i32 *kind,
+                        const u64 *root);
+/* Up to `max` completions, waiting up to wait_ms for the first. Returns the count. */
+i32 g500_server_poll(G500_Server *s, G500_Completion *out, i32 max, f64 wait_ms);
+/* Returns when every submitted request has completed. */
+void g500_server_drain(G500_Server *s);
+void g500_server_stats(const G500_Server *s, u64 *submitted, u64 *completed, u64 *launches, u64 *pool_used);
+/* Drains, stops the threads and frees the host side (device memory returns
+ * with worker_finalize). */
+void g500_server_destroy(G500_Server *s);
 
 /* ---- library control --------------------------------------------------- */
 /* G500_ERRORS_ABORT (default): fatal errors print and exit like the
@@ -95,8 +187,9 @@ void g500_copy_to_device(void *dst_device, const void *src, u64 bytes);
 void g500_copy_device(void *dst_device, const void *src_device, u64 bytes);
 
 /* Tuning knobs (names: "bfs_alpha", "bfs_beta", "bfs_block", "bfs_hub_degree",
- * "sssp_block", "gen_blocks", "coop_launch"; read-only "num_cus"). Returns 0
- * on success, -1 for an unknown name. */
+ * "batch_alpha", "batch_beta", "batch_heavy_degree", "sssp_block",
+ * "gen_blocks", "coop_launch"; read-only "num_cus"). Returns 0 on success, -1
+ * for an unknown name. */
 i32 g500_set_param(const char *name, f64 value);
 i32 g500_get_param(const char *name, f64 *value);
 

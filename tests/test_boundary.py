@@ -39,7 +39,8 @@ def test_library_exports_every_declared_function():
 
 
 STRUCTS = ["Arena", "Worker_State", "Tuple_Graph", "Packed_Edge", "Mrg_State", "Mrg_Transition_Matrix",
-           "Oned_Graph", "Bfs_State", "Sssp_State", "Bfs_Batch_State"]
+           "Oned_Graph", "Bfs_State", "Sssp_State", "Bfs_Batch_State", "Sssp_Batch_State",
+           "G500_Completion"]
 
 
 def test_struct_layout_matches_headers():
@@ -47,7 +48,8 @@ def test_struct_layout_matches_headers():
     import g500
     mirror = {"Arena": g500.Arena, "Worker_State": g500.WorkerState, "Tuple_Graph": g500.TupleGraph,
               "Oned_Graph": g500.OnedGraph, "Bfs_State": g500.BfsState, "Sssp_State": g500.SsspState,
-              "Bfs_Batch_State": g500.BfsBatchState}
+              "Bfs_Batch_State": g500.BfsBatchState, "Sssp_Batch_State": g500.SsspBatchState,
+              "G500_Completion": g500.Completion}
     src = ["#include <stddef.h>", "#include <stdio.h>", '#include "g500_ext.h"', "int main(void){"]
     for s in STRUCTS:
         src.append(f'printf("{s} %zu\\n", sizeof({s}));')

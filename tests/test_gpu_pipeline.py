@@ -192,18 +192,23 @@ def test_bfs_batch_depths(arena, oracle, scale):
 
 
 # a huge alpha switches to bottom-up at once; a tiny one stays top-down; a
-# small hub degree sends most records through the grid-wide hub lists
-@pytest.mark.parametrize("alpha,beta,hub", [(1e-9, 18, 2048), (1e9, 1e9, 2048), (15, 18, 4), (2, 4, 64)])
-def test_bfs_batch_direction_modes(arena, oracle, alpha, beta, hub):
+# small hub degree sends most top-down records through the grid-wide hub
+# lists; a small heavy degree sends most bottom-up vertices through the
+# grid-wide chunk pass, a huge one none
+@pytest.mark.parametrize("alpha,beta,hub,heavy", [(1e-9, 18, 2048, 1024), (1e9, 1e9, 2048, 1024),
+                                                  (1e9, 1e9, 2048, 1), (1e9, 1e9, 2048, 1 << 30),
+                                                  (15, 18, 4, 16), (2, 4, 64, 64)])
+def test_bfs_batch_direction_modes(arena, oracle, alpha, beta, hub, heavy):
     e, w, ref = oracle_graph(14)
     g = check_csr(arena, oracle, 1 << 14, e, w)
     b = arena.batch_create(g)
-    old = {k: arena.get_param(k) for k in ("batch_alpha", "batch_beta", "bfs_hub_degree")}
+    old = {k: arena.get_param(k) for k in ("batch_alpha", "batch_beta", "bfs_hub_degree", "batch_heavy_degree")}
     try:
         arena.set_param("batch_alpha", alpha)
         arena.set_param("batch_beta", beta)
         arena.set_param("bfs_hub_degree", hub)
-        say("batch modes", alpha, beta, hub)
+        arena.set_param("batch_heavy_degree", heavy)
+        say("batch modes", alpha, beta, hub, heavy)
         batch_check(arena, oracle, g, ref, b, oracle.roots(ref))
         modes, recs, ms = arena.batch_level_log(b)
         if alpha < 1e-3:
@@ -301,6 +306,61 @@ def test_sssp_max_error_vs_f64(arena, oracle):
     err = float(np.max(np.abs(d.astype(np.float64) - d64)))
     say("sssp max |f32 - f64| =", err)
     assert err < 1e-5
+
+
+# ------------------------------------------------------- batched SSSP (16 roots)
+def sssp_batch_check(sess, oracle, g, ref, s, roots):
+    sess.sssp_batch(g, s, roots)
+    for i, r in enumerate(roots):
+        r = int(r)
+        d_ref, _ = oracle.dijkstra_f32(ref, r)
+        d, p = sess.sssp_batch_result(g, s, i)
+        assert np.array_equal(d.view(np.uint32), d_ref.view(np.uint32)), (i, r)
+        assert oracle.sssp_check(ref, r, d, p) == 0, (i, r)
+
+
+@pytest.mark.parametrize("scale", [10, 12, 16])
+def test_sssp_batch_bitexact(arena, oracle, scale):
+    e, w, ref = oracle_graph(scale)
+    g = check_csr(arena, oracle, 1 << scale, e, w)
+    s = arena.sssp_batch_create(g)
+    roots = oracle.roots(ref)
+    for delta in (0.1, 0.25, 0.02):
+        s.delta = delta
+        say("sssp batch", scale, delta)
+        sssp_batch_check(arena, oracle, g, ref, s, roots[:16])
+    # fewer roots, repeats, an isolated vertex as a root
+    s.delta = 0.1
+    isolated = int(np.flatnonzero(np.diff(ref.rowstarts.astype(np.int64)) == 0)[0])
+    some = np.array([roots[20], roots[20], isolated, roots[17]], np.uint64)
+    say("sssp batch repeats", scale, some.tolist())
+    sssp_batch_check(arena, oracle, g, ref, s, some)
+
+
+def test_sssp_batch_matches_plain(arena, oracle):
+    """Batched and single-root kernels give the same distances on a larger graph."""
+    e, w, ref = oracle_graph(18)
+    g = arena.csr(arena.upload_tuples(e, w), 1 << 18)
+    sb = arena.sssp_batch_create(g)
+    sp = arena.sssp_create(g)
+    roots = arena.roots(g)[:16]
+    arena.sssp_batch(g, sb, roots)
+    for i, r in enumerate(roots):
+        arena.sssp(g, sp, int(r))
+        d1, _ = arena.sssp_result(sp)
+        d, p = arena.sssp_batch_result(g, sb, i)
+        assert np.array_equal(d.view(np.uint32), d1.view(np.uint32)), i
+        assert oracle.sssp_check(ref, int(r), d, p) == 0
+
+
+def test_sssp_batch_errors_return(arena):
+    import g500
+    e, w, ref = oracle_graph(10)
+    g = arena.csr(arena.upload_tuples(e, w), 1 << 10)
+    s = arena.sssp_batch_create(g)
+    for roots in ([], list(range(17)), [1 << 10]):
+        with pytest.raises(g500.LibraryError):
+            arena.sssp_batch(g, s, np.array(roots, np.uint64))
 
 
 # --------------------------------------------------------------------- runner
