@@ -159,6 +159,30 @@ __device__ __forceinline__ T wave_inclusive_scan(T v) {
     return v;
 }
 
+// Segmented walk over the concatenated work ranges of one wave's 64 items:
+// lane j's item owns positions [off, off + deg) of the concatenation. For the
+// 64-position chunk starting at c, returns the item whose range holds
+// position c + lane (valid when c + lane < total), and sets `inter`/`last`
+// for the lane's own item: whether its range meets the chunk and the lane
+// that holds its last position in the chunk. `slot` is the wave's 64 LDS
+// words; LDS operations of one wave complete in order.
+__device__ __forceinline__ int chunk_owner(volatile int *slot, u32 off, u32 deg, u32 c, bool &inter, int &last) {
+    const u32 lane = lane_id();
+    const u32 lo = off > c ? off : c;
+    const u32 hi = off + deg < c + kWave ? off + deg : c + kWave;
+    inter = lo < hi;
+    last = inter ? (int)(hi - 1 - c) : (int)lane;
+    slot[lane] = -1;
+    if (inter) slot[lo - c] = (int)lane;
+    int j = slot[lane];
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int y = __shfl_up(j, o);
+        if ((int)lane >= o && y > j) j = y;
+    }
+    return j;
+}
+
 // ------------------------------------------------------------ hub work lists
 // A hub list holds high-degree vertices together with the offset of each
 // one's edge range in a flattened work space where every range is padded to a

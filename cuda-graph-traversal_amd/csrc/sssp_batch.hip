@@ -115,25 +115,22 @@ __device__ __forceinline__ void push_item(const SsspBatchArgs &a, u32 v, u32 mas
                a.n, &ctl->bar.error);
 }
 
-// Relaxes entry k of u's row for the roots of M; returns the roots whose label
-// of the target went below t.below, and the target in v.
-__device__ __forceinline__ u32 relax_entry(const SsspBatchArgs &a, u32 u, u32 M, u64 k, const BTarget &t, u32 &v,
-                                           u64 &improved) {
+// One relaxation: entry k of u's row for root r. Returns the root's bit when
+// the target's label went below t.below, and the target in v.
+__device__ __forceinline__ u32 relax_pair(const SsspBatchArgs &a, u32 u, int r, u64 k, const BTarget &t, u32 &v,
+                                          u64 &improved) {
     v = a.col[k];
-    const f32 wk = a.w[k];
-    const u64 *lu = a.lab + (u64)u * kBatchRoots;
-    u64 *lv = a.lab + (u64)v * kBatchRoots;
-    u32 out = 0;
-    for (u32 m = M; m; m &= m - 1) {
-        const int r = __ffs(m) - 1;
-        const f32 cand = label_dist(lu[r]) + wk;
-        if (relax(&lv[r], cand, u)) {
-            ++improved;
-            if (cand < t.below) out |= 1u << r;
-            if (cand < t.floor) atomicOr(&a.ctl->bar.error, 16u);
-        }
-    }
-    return out;
+    const f32 cand = label_dist(a.lab[(u64)u * kBatchRoots + r]) + a.w[k];
+    if (!relax(&a.lab[(u64)v * kBatchRoots + r], cand, u)) return 0;
+    ++improved;
+    if (cand < t.floor) atomicOr(&a.ctl->bar.error, 16u);
+    return cand < t.below ? 1u << r : 0u;
+}
+
+// Index of the i-th set bit of M (M has more than i bits).
+__device__ __forceinline__ int nth_root(u32 M, u32 i) {
+    for (; i; --i) M &= M - 1;
+    return __ffs(M) - 1;
 }
 
 template <bool LIGHT>
@@ -143,112 +140,125 @@ __device__ __forceinline__ void bsegment(const SsspBatchArgs &a, u32 u, u64 &b, 
     e = LIGHT ? r0 + s : a.rs[u + 1];
 }
 
-// Light queue tiers 0 (a lane per item) and 1 (a wave per item); the item's
-// mask is read from pend and cleared.
-__device__ void light_tier(const SsspBatchArgs &a, int buf, const u32 *list, u32 count, int tier, const BTarget &t,
-                           u64 &improved) {
+// The work of an item (u, M) over a segment is its (entry, root) pairs,
+// entry-major: pair q = entry * k + i, i-th root of M, k = |M|. The k lanes of
+// one entry read the same column and weight and relax k labels in the same
+// 128-byte line of the target.
+
+// Tier 0 (segments <= 32 entries): a wave takes 64 items and walks their
+// concatenated pair ranges 64 pairs at a time. Light items read their mask
+// from pend and clear it; heavy items carry it in the list entry.
+template <bool LIGHT>
+__device__ void group_tier(const SsspBatchArgs &a, int buf, const u32 *lq, const u64 *hq, u32 count,
+                           const BTarget &t, u64 &improved, volatile int *slot) {
     const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
     const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
-    u32 *pend = a.pend[buf];
-    if (tier == 0) {
-        for (u64 i0 = wave_gid * kWave; i0 < count; i0 += nwaves * kWave) {
-            const u64 i = i0 + lane;
-            u64 b = 0, e = 0;
-            u32 u = 0, M = 0;
-            if (i < count) {
-                u = list[i];
+    u32 *pend = LIGHT ? a.pend[buf] : nullptr;
+    for (u64 i0 = wave_gid * kWave; i0 < count; i0 += nwaves * kWave) {
+        const u64 i = i0 + lane;
+        u32 u = 0, M = 0;
+        u64 b = 0, e = 0;
+        if (i < count) {
+            if (LIGHT) {
+                u = lq[i];
                 if (u < a.n) {
                     M = pend[u];
                     pend[u] = 0;
-                    bsegment<true>(a, u, b, e);
                 } else {
                     atomicOr(&a.ctl->bar.error, 4u);
                 }
-            }
-            const u32 len = M ? (u32)(e - b) : 0;
-            u32 mx = len;
-            for (int o = 32; o > 0; o >>= 1) mx = max(mx, (u32)__shfl_xor((int)mx, o));
-            for (u32 j = 0; j < mx; ++j) {
-                u32 v = 0, out = 0;
-                if (j < len) out = relax_entry(a, u, M, b + j, t, v, improved);
-                push_item(a, v, out, t);
-            }
-        }
-    } else {
-        for (u64 i = wave_gid; i < count; i += nwaves) {
-            const u32 u = list[i];
-            if (u >= a.n) {
-                if (lane == 0) atomicOr(&a.ctl->bar.error, 4u);
-                continue;
-            }
-            const u32 M = __builtin_amdgcn_readfirstlane(pend[u]);
-            if (lane == 0) pend[u] = 0;
-            u64 b, e;
-            bsegment<true>(a, u, b, e);
-            for (u64 k0 = b; k0 < e; k0 += kWave) {
-                const u64 k = k0 + lane;
-                u32 v = 0, out = 0;
-                if (k < e) out = relax_entry(a, u, M, k, t, v, improved);
-                push_item(a, v, out, t);
-            }
-        }
-    }
-}
-
-// Heavy item tiers 0 and 1: entries (mask << 32 | v).
-__device__ void heavy_tier(const SsspBatchArgs &a, const u64 *list, u32 count, int tier, const BTarget &t,
-                           u64 &improved) {
-    const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
-    const u32 lane = lane_id();
-    const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
-    if (tier == 0) {
-        for (u64 i0 = wave_gid * kWave; i0 < count; i0 += nwaves * kWave) {
-            const u64 i = i0 + lane;
-            u64 b = 0, e = 0;
-            u32 u = 0, M = 0;
-            if (i < count) {
-                const u64 it = list[i];
+            } else {
+                const u64 it = hq[i];
                 u = (u32)it;
                 M = (u32)(it >> 32);
-                if (u < a.n) {
-                    bsegment<false>(a, u, b, e);
-                } else {
+                if (u >= a.n) {
                     atomicOr(&a.ctl->bar.error, 4u);
                     M = 0;
                 }
             }
-            const u32 len = M ? (u32)(e - b) : 0;
-            u32 mx = len;
-            for (int o = 32; o > 0; o >>= 1) mx = max(mx, (u32)__shfl_xor((int)mx, o));
-            for (u32 j = 0; j < mx; ++j) {
-                u32 v = 0, out = 0;
-                if (j < len) out = relax_entry(a, u, M, b + j, t, v, improved);
-                push_item(a, v, out, t);
-            }
+            if (M) bsegment<LIGHT>(a, u, b, e);
         }
-    } else {
-        for (u64 i = wave_gid; i < count; i += nwaves) {
-            const u64 it = list[i];
-            const u32 u = (u32)it, M = (u32)(it >> 32);
+        const u32 pairs = (u32)(e - b) * (u32)__popc(M);
+        const u32 incl = wave_inclusive_scan(pairs);
+        const u32 off = incl - pairs, total = __shfl(incl, kWave - 1);
+        for (u32 c = 0; c < total; c += kWave) {
+            bool inter;
+            int last;
+            int j = chunk_owner(slot, off, pairs, c, inter, last);
+            if (j < 0) j = 0;
+            const u32 uj = __shfl(u, j), Mj = __shfl(M, j), offj = __shfl(off, j);
+            const u64 bj = shfl64(b, j);
+            const u32 pos = c + lane;
+            u32 v = 0, out = 0;
+            if (pos < total) {
+                const u32 k = (u32)__popc(Mj), q = pos - offj;
+                const u32 entry = q / k;
+                out = relax_pair(a, uj, nth_root(Mj, q - entry * k), bj + entry, t, v, improved);
+            }
+            push_item(a, v, out, t);
+        }
+    }
+}
+
+// Lane roles for a wave-uniform mask M: `per` entries per step, lane -> entry
+// offset and root.
+struct PairLanes {
+    u32 per, eoff;
+    int r;
+    bool active;
+};
+__device__ __forceinline__ PairLanes pair_lanes(u32 M) {
+    const u32 lane = lane_id(), k = (u32)__popc(M);
+    PairLanes p;
+    p.per = kWave / k;
+    p.eoff = lane / k;
+    p.active = lane < p.per * k;
+    p.r = p.active ? nth_root(M, lane - p.eoff * k) : 0;
+    return p;
+}
+
+// Tier 1 (segments <= 1024 entries): a wave per item.
+template <bool LIGHT>
+__device__ void wave_tier(const SsspBatchArgs &a, int buf, const u32 *lq, const u64 *hq, u32 count, const BTarget &t,
+                          u64 &improved) {
+    const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
+    const u32 lane = lane_id();
+    const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
+    for (u64 i = wave_gid; i < count; i += nwaves) {
+        u32 u, M;
+        if (LIGHT) {
+            u = lq[i];
             if (u >= a.n) {
                 if (lane == 0) atomicOr(&a.ctl->bar.error, 4u);
                 continue;
             }
-            u64 b, e;
-            bsegment<false>(a, u, b, e);
-            for (u64 k0 = b; k0 < e; k0 += kWave) {
-                const u64 k = k0 + lane;
-                u32 v = 0, out = 0;
-                if (k < e) out = relax_entry(a, u, M, k, t, v, improved);
-                push_item(a, v, out, t);
+            M = __builtin_amdgcn_readfirstlane(a.pend[buf][u]);
+            if (lane == 0) a.pend[buf][u] = 0;
+        } else {
+            const u64 it = hq[i];
+            u = (u32)it;
+            M = (u32)(it >> 32);
+            if (u >= a.n) {
+                if (lane == 0) atomicOr(&a.ctl->bar.error, 4u);
+                continue;
             }
+        }
+        if (!M) continue;
+        u64 b, e;
+        bsegment<LIGHT>(a, u, b, e);
+        const PairLanes p = pair_lanes(M);
+        for (u64 e0 = b; e0 < e; e0 += p.per) {
+            const u64 k = e0 + p.eoff;
+            u32 v = 0, out = 0;
+            if (p.active && k < e) out = relax_pair(a, u, p.r, k, t, v, improved);
+            push_item(a, v, out, t);
         }
     }
 }
 
 // Hub tier: chunks of the flattened work space. Light hubs read their mask
-// from pend (cleared by the item's last chunk); heavy hubs from `masks`.
+// from pend (cleared by the item's last chunk); heavy hubs from `aux`.
 template <bool LIGHT>
 __device__ void hub_tier(const SsspBatchArgs &a, const u32 *hv, const u64 *hoff, const u32 *hlen, u32 *aux,
                          u32 *pend, u64 packed, const BTarget &t, u64 &improved) {
@@ -258,24 +268,26 @@ __device__ void hub_tier(const SsspBatchArgs &a, const u32 *hv, const u64 *hoff,
     const u64 count = hub_count(packed), total = hub_total(packed);
     u64 i = 0;
     for (u64 c = wave_gid; c * kHubChunk < total; c += nwaves) {
-        const u64 p = c * kHubChunk;
-        i = hub_locate(hoff, i, count, p);
+        const u64 p0 = c * kHubChunk;
+        i = hub_locate(hoff, i, count, p0);
         const u32 u = hv[i];
         if (u >= a.n) {
             if (lane == 0) atomicOr(&a.ctl->bar.error, 4u);
             continue;
         }
         const u32 M = __builtin_amdgcn_readfirstlane(LIGHT ? pend[u] : aux[i]);
-        const u64 local = p - hoff[i];
+        const u64 local = p0 - hoff[i];
         const u32 len = hlen[i];
         u64 b, e;
         bsegment<LIGHT>(a, u, b, e);
-#pragma unroll
-        for (u64 j = 0; j < kHubChunk; j += kWave) {
-            const u64 off = local + j + lane;
-            u32 v = 0, out = 0;
-            if (off < len) out = relax_entry(a, u, M, b + off, t, v, improved);
-            push_item(a, v, out, t);
+        if (M) {
+            const PairLanes p = pair_lanes(M);
+            for (u32 j = 0; j < (u32)kHubChunk; j += p.per) {
+                const u64 off = local + j + p.eoff;
+                u32 v = 0, out = 0;
+                if (p.active && j + p.eoff < (u32)kHubChunk && off < len) out = relax_pair(a, u, p.r, b + off, t, v, improved);
+                push_item(a, v, out, t);
+            }
         }
         if (LIGHT && lane == 0) {
             const u32 chunks = (len + (u32)kHubChunk - 1) / (u32)kHubChunk;
@@ -291,6 +303,8 @@ __global__ __launch_bounds__(512) void sssp_batch_kernel(SsspBatchArgs a) {
     SsspBatchCtl *ctl = a.ctl;
     const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
+    __shared__ int slots[512 / kWave][kWave];
+    volatile int *slot = slots[threadIdx.x / kWave];
     // dense passes: 16 lanes per vertex, uniform trip count
     const u64 nlab = a.n * kBatchRoots;
     const u64 nround = (nlab + gthreads - 1) / gthreads * gthreads;
@@ -367,8 +381,8 @@ __global__ __launch_bounds__(512) void sssp_batch_kernel(SsspBatchArgs a) {
             if (gtid == 0) atomicAdd((unsigned long long *)&ctl->light_iters, 1ull);
             const BTarget t = target(x, hi, lo);
             const int buf = (int)(x & 1);
-#pragma unroll
-            for (int tier = 0; tier < 2; ++tier) light_tier(a, buf, a.fq[buf][tier], cnt[tier], tier, t, improved);
+            group_tier<true>(a, buf, a.fq[buf][0], nullptr, cnt[0], t, improved, slot);
+            wave_tier<true>(a, buf, a.fq[buf][1], nullptr, cnt[1], t, improved);
             hub_tier<true>(a, a.fq[buf][2], a.fq_off[buf], a.fq_len[buf], a.fq_done[buf], a.pend[buf], hub, t,
                            improved);
             bsync(ctl, 2, __float_as_uint(lo), x);
@@ -411,11 +425,8 @@ __global__ __launch_bounds__(512) void sssp_batch_kernel(SsspBatchArgs a) {
                 if (slot != ~0ull) a.hq_mask[slot] = H;
             }
             bsync(ctl, 3, __float_as_uint(lo), x);
-#pragma unroll
-            for (int tier = 0; tier < 2; ++tier) {
-                const u32 c = min(ld_relaxed(&ctl->hlen[hs][tier]), (u32)a.n);
-                heavy_tier(a, a.hq[tier], c, tier, t, improved);
-            }
+            group_tier<false>(a, 0, nullptr, a.hq[0], min(ld_relaxed(&ctl->hlen[hs][0]), (u32)a.n), t, improved, slot);
+            wave_tier<false>(a, 0, nullptr, a.hq[1], min(ld_relaxed(&ctl->hlen[hs][1]), (u32)a.n), t, improved);
             hub_tier<false>(a, a.hqh, a.hq_off, a.hq_len, a.hq_mask, nullptr, ld_relaxed(&ctl->hhub[hs]), t,
                             improved);
             bsync(ctl, 4, __float_as_uint(lo), x);

@@ -84,9 +84,10 @@ def test_serve_alone_vs_mix(sess, oracle):
             assert c.status == 0
             (a0, a1), reached, ssum = alone[i]
             b0, b1 = bufs[i]
+            # depths and distances are equal bit for bit; SSSP parents may differ
+            # between runs only where two parents give the same distance (the
+            # first strict decrease wins), so they are checked by the certificate
             assert np.array_equal(b0.view(np.uint8), a0.view(np.uint8)), reqs[i]
-            if b1 is not None:
-                assert np.array_equal(b1, a1)
             assert (c.reached, c.sum) == (reached, ssum)
             ref = graphs[gi][1]
             if kind == g500.BFS:
