@@ -48,8 +48,9 @@ struct BatchCtl {
 
 struct BatchArgs {
     u64 n, m_total;
-    const u64 *rs;
+    const u64 *rs;    // traversal view
     const u32 *col;
+    const u32 *perm;
     u64 *seen;
     u64 *fm[2];
     u64 *fb[2];       // frontier bitmaps matching fm
@@ -342,7 +343,7 @@ __global__ __launch_bounds__(512) void batch_bfs_kernel(BatchArgs a) {
         bool first = false;
         u32 v = 0;
         if ((i32)gtid < a.nroots) {
-            v = (u32)a.roots[gtid];
+            v = trav_id(a.perm, a.roots[gtid]);
             first = atomicOr((unsigned long long *)&a.next[v], 1ull << gtid) == 0;
         }
         const u32 s = wave_append(&ctl->ntouched[0], first);
@@ -465,7 +466,7 @@ __global__ __launch_bounds__(256) void batch_depth_fill(i64 *__restrict__ depth,
 __global__ __launch_bounds__(256) void batch_depth_scatter(const u32 *__restrict__ lv, const u64 *__restrict__ lm,
                                                            const u64 *__restrict__ level_start,
                                                            const BatchCtl *__restrict__ ctl, int r,
-                                                           i64 *__restrict__ depth) {
+                                                           const u32 *__restrict__ iperm, i64 *__restrict__ depth) {
     const u32 levels = ctl->levels;
     const u64 nrec = level_start[levels];
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < nrec; i += (u64)gridDim.x * blockDim.x) {
@@ -476,7 +477,7 @@ __global__ __launch_bounds__(256) void batch_depth_scatter(const u32 *__restrict
             if (level_start[mid] <= i) lo = mid;
             else hi = mid;
         }
-        depth[lv[i]] = lo;
+        depth[orig_id(iperm, lv[i])] = lo;
     }
 }
 
@@ -507,8 +508,10 @@ bool bfs_batch_enqueue(const Oned_Graph *g, Bfs_Batch_State *b, const u64 *roots
     BatchArgs a;
     a.n = n;
     a.m_total = g->nlocaledges;
-    a.rs = g->rowstarts;
-    a.col = g->column;
+    const TraversalView tv = traversal_view(g);
+    a.rs = tv.rs;
+    a.col = tv.col;
+    a.perm = tv.perm;
     a.seen = b->seen;
     a.fm[0] = b->fmask[0];
     a.fm[1] = b->fmask[1];
@@ -552,7 +555,8 @@ void bfs_batch_depth_enqueue(const Oned_Graph *g, const Bfs_Batch_State *b, i32 
     const u64 n = g->nlocalverts;
     batch_depth_fill<<<grid_for(n, 256, (u64)c.num_cus * 32), 256, 0, s>>>(depth, n);
     batch_depth_scatter<<<(unsigned)c.num_cus * 32, 256, 0, s>>>(b->log_vertex, b->log_mask, b->level_start,
-                                                                  (const BatchCtl *)b->ctl, r, depth);
+                                                                  (const BatchCtl *)b->ctl, r,
+                                                                  traversal_view(g).iperm, depth);
     G500_HIP(hipGetLastError());
 }
 

@@ -49,9 +49,10 @@ __device__ __forceinline__ void bfs_sync(BfsCtl *ctl, u32 phase, u32 level) {
 
 struct BfsArgs {
     u64 n, nwords, m_total;
-    const u64 *rs;
+    const u64 *rs;      // traversal view
     const u32 *col;
-    i64 *pred;
+    const u32 *perm, *iperm;
+    i64 *pred;          // original ids
     u64 *visited;
     u64 *fb[2];
     u32 *q[2];
@@ -82,7 +83,7 @@ __device__ __forceinline__ void td_visit(const BfsArgs &a, u32 u, u32 v, bool ac
     }
     u32 deg = 0;
     if (won) {
-        a.pred[v] = u;
+        a.pred[orig_id(a.iperm, v)] = orig_id(a.iperm, u);
         deg = (u32)(a.rs[v + 1] - a.rs[v]);
         ++cnt;
         edges += deg;
@@ -101,8 +102,8 @@ __global__ __launch_bounds__(512) void bfs_kernel(BfsArgs a) {
     const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
 
     if (gtid == 0) {  // level 0: the root
-        const u64 r = a.root;
-        a.pred[r] = (i64)r;
+        const u64 r = trav_id(a.perm, a.root);
+        a.pred[a.root] = (i64)a.root;
         atomicOr((unsigned long long *)&a.visited[r >> 6], 1ull << (r & 63));
         const u32 deg = (u32)(a.rs[r + 1] - a.rs[r]);
         if (deg > a.hub_degree) {
@@ -227,7 +228,7 @@ __global__ __launch_bounds__(512) void bfs_kernel(BfsArgs a) {
                     for (u64 k = b; k < e; ++k) {
                         const u32 u = a.col[k];
                         if ((fbc[u >> 6] >> (u & 63)) & 1) {
-                            a.pred[v] = u;
+                            a.pred[orig_id(a.iperm, (u32)v)] = orig_id(a.iperm, u);
                             found = true;
                             edges += e - b;
                             ++cnt;
@@ -357,8 +358,11 @@ void oned_graph_bfs_run(Worker_State *ws, const Oned_Graph *g, Bfs_State *bfs, u
     a.n = g->nlocalverts;
     a.nwords = (a.n + 63) / 64;
     a.m_total = g->nlocaledges;
-    a.rs = g->rowstarts;
-    a.col = g->column;
+    const TraversalView tv = traversal_view(g);
+    a.rs = tv.rs;
+    a.col = tv.col;
+    a.perm = tv.perm;
+    a.iperm = tv.iperm;
     a.pred = bfs->pred;
     a.visited = bfs->visited;
     a.fb[0] = bfs->frontier_bits[0];

@@ -36,10 +36,11 @@ struct Context {
     int bfs_hub_degree = 2048;    // frontier vertices above this degree are split grid-wide
     double batch_alpha = 15.0;    // the same two thresholds for the batched multi-root BFS
     double batch_beta = 18.0;
-    int batch_heavy_degree = 1024; // batched bottom-up: vertices above this degree are split grid-wide
+    int batch_heavy_degree = 1 << 20; // batched bottom-up: vertices above this degree are split grid-wide
     int sssp_block = 512;
     int coop_launch = 1;          // 1: cooperative launch first; 0: plain launch of the co-resident grid
     int gen_blocks = 0;           // 0: one block per 256-edge tile (capped)
+    int relabel = 1;              // 1: CSR builds add the degree-ordered traversal copy
 };
 
 Context &ctx();
@@ -78,6 +79,22 @@ int cooperative_blocks(const void *kernel, int block, size_t dyn_lds);
 // ends with an error instead of waiting forever. stream 0: the library stream.
 void launch_persistent(const void *kernel, int grid, int block, void **args, hipStream_t stream = nullptr,
                        bool cooperative = true);
+
+// Traversal view of a graph: the degree-ordered copy when it exists, else the
+// original arrays. Kernels run in traversal ids and report original ids
+// through orig_id (identity without the copy).
+struct TraversalView {
+    const u64 *rs;
+    const u32 *col;
+    const f32 *w;
+    const u32 *perm, *iperm;
+};
+inline TraversalView traversal_view(const Oned_Graph *g) {
+    if (g->perm) return {g->rowstarts_t, g->column_t, g->weights_t, g->perm, g->iperm};
+    return {g->rowstarts, g->column, g->weights, nullptr, nullptr};
+}
+__device__ __forceinline__ u32 orig_id(const u32 *iperm, u32 v) { return iperm ? iperm[v] : v; }
+__device__ __forceinline__ u32 trav_id(const u32 *perm, u64 v) { return perm ? perm[v] : (u32)v; }
 
 // Asynchronous batched launches on a caller's stream with a caller's grid (0:
 // the full co-resident grid, cooperative, on the library stream with timing).

@@ -365,6 +365,157 @@ __global__ __launch_bounds__(kRadixBlock) void sort_rows_radix(
     }
 }
 
+// ---------------------------------------------------------------- relabeling
+// Traversal copy of the CSR with vertices renumbered by decreasing degree
+// (exact up to kDegKeys - 1, above that in one class). In Kronecker graphs a
+// few percent of the vertices hold most edge endpoints; numbering them first
+// packs the labels, bitmap words and masks that traversals probe most into a
+// small, cache-resident range. Entry order inside each row is unchanged, so
+// rows stay sorted by weight.
+constexpr u32 kDegKeys = 4096;
+constexpr int kRelabelBlock = 1024;
+constexpr int kRelabelItems = 8;
+
+__device__ __forceinline__ u32 degree_key(const u64 *rs, u64 v) {
+    const u64 d = rs[v + 1] - rs[v];
+    return kDegKeys - 1 - (u32)(d < kDegKeys - 1 ? d : kDegKeys - 1);
+}
+
+__global__ __launch_bounds__(kRelabelBlock) void relabel_count(const u64 *__restrict__ rs, u64 n,
+                                                               u32 *__restrict__ counts) {
+    __shared__ u32 h[kDegKeys];
+    for (u32 k = threadIdx.x; k < kDegKeys; k += blockDim.x) h[k] = 0;
+    __syncthreads();
+    const u64 base = (u64)blockIdx.x * kRelabelBlock * kRelabelItems;
+    for (int j = 0; j < kRelabelItems; ++j) {
+        const u64 v = base + (u64)j * kRelabelBlock + threadIdx.x;
+        if (v < n) atomicAdd(&h[degree_key(rs, v)], 1u);
+    }
+    __syncthreads();
+    for (u32 k = threadIdx.x; k < kDegKeys; k += blockDim.x)
+        if (h[k]) atomicAdd(&counts[k], h[k]);
+}
+
+// Single block: exclusive scan of the key counts into cursors.
+__global__ __launch_bounds__(1024) void relabel_offsets(const u32 *__restrict__ counts, u32 *__restrict__ cursor) {
+    u64 carry = 0;
+    for (u32 b = 0; b < kDegKeys; b += 1024) {
+        const u32 k = b + threadIdx.x;
+        const u64 v = counts[k];
+        u64 tot;
+        const u64 ex = block_exclusive_scan<1024>(v, &tot);
+        cursor[k] = (u32)(carry + ex);
+        carry += tot;
+    }
+}
+
+// Each block reserves one range per key for its tile, then ranks its
+// vertices inside the ranges (order inside a key class is not specified).
+__global__ __launch_bounds__(kRelabelBlock) void relabel_scatter(const u64 *__restrict__ rs, u64 n,
+                                                                 u32 *__restrict__ cursor, u32 *__restrict__ perm,
+                                                                 u32 *__restrict__ iperm, u32 *__restrict__ deg_t) {
+    __shared__ u32 h[kDegKeys], base[kDegKeys];
+    for (u32 k = threadIdx.x; k < kDegKeys; k += blockDim.x) h[k] = 0;
+    __syncthreads();
+    const u64 tile = (u64)blockIdx.x * kRelabelBlock * kRelabelItems;
+    u32 key[kRelabelItems];
+    for (int j = 0; j < kRelabelItems; ++j) {
+        const u64 v = tile + (u64)j * kRelabelBlock + threadIdx.x;
+        key[j] = v < n ? degree_key(rs, v) : 0;
+        if (v < n) atomicAdd(&h[key[j]], 1u);
+    }
+    __syncthreads();
+    for (u32 k = threadIdx.x; k < kDegKeys; k += blockDim.x) {
+        base[k] = h[k] ? atomicAdd(&cursor[k], h[k]) : 0;
+        h[k] = 0;
+    }
+    __syncthreads();
+    for (int j = 0; j < kRelabelItems; ++j) {
+        const u64 v = tile + (u64)j * kRelabelBlock + threadIdx.x;
+        if (v >= n) continue;
+        const u32 pos = base[key[j]] + atomicAdd(&h[key[j]], 1u);
+        perm[v] = pos;
+        iperm[pos] = (u32)v;
+        deg_t[pos] = (u32)(rs[v + 1] - rs[v]);
+    }
+}
+
+// Rows of the traversal copy: a lane per row up to kWave entries, a block per
+// longer row (they come first: rows are in decreasing degree order).
+__global__ __launch_bounds__(256) void relabel_rows_lane(const u64 *__restrict__ rs, const u32 *__restrict__ col,
+                                                         const f32 *__restrict__ w, const u32 *__restrict__ perm,
+                                                         const u32 *__restrict__ iperm, const u64 *__restrict__ rs_t,
+                                                         u64 first, u64 n, u32 *__restrict__ col_t,
+                                                         f32 *__restrict__ w_t) {
+    for (u64 p = first + (u64)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (u64)gridDim.x * blockDim.x) {
+        const u32 v = iperm[p];
+        const u64 b = rs[v], e = rs[v + 1], o = rs_t[p];
+        for (u64 k = b; k < e; ++k) {
+            col_t[o + (k - b)] = perm[col[k]];
+            w_t[o + (k - b)] = w[k];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void relabel_rows_block(const u64 *__restrict__ rs, const u32 *__restrict__ col,
+                                                          const f32 *__restrict__ w, const u32 *__restrict__ perm,
+                                                          const u32 *__restrict__ iperm, const u64 *__restrict__ rs_t,
+                                                          u64 count, u32 *__restrict__ col_t, f32 *__restrict__ w_t) {
+    for (u64 p = blockIdx.x; p < count; p += gridDim.x) {
+        const u32 v = iperm[p];
+        const u64 b = rs[v], e = rs[v + 1], o = rs_t[p];
+        for (u64 k = b + threadIdx.x; k < e; k += blockDim.x) {
+            col_t[o + (k - b)] = perm[col[k]];
+            w_t[o + (k - b)] = w[k];
+        }
+    }
+}
+
+// Builds g's traversal copy (perm, iperm, rowstarts_t, column_t, weights_t)
+// on the library stream; scratch is taken above the arena mark and released.
+void build_traversal_copy(Worker_State *ws, Oned_Graph *g) {
+    Context &c = ctx();
+    const hipStream_t s = c.stream;
+    const u64 n = g->nlocalverts, m = g->nlocaledges;
+    g->perm = (u32 *)arena_alloc(ws, n * sizeof(u32), "traversal ids");
+    g->iperm = (u32 *)arena_alloc(ws, n * sizeof(u32), "original ids");
+    g->rowstarts_t = (u64 *)arena_alloc(ws, (n + 1) * sizeof(u64), "traversal row offsets");
+    g->column_t = (u32 *)arena_alloc(ws, m * sizeof(u32), "traversal columns");
+    g->weights_t = (f32 *)arena_alloc(ws, m * sizeof(f32), "traversal weights");
+    const u64 mark = ws->gpu_arena.used;
+    u32 *counts = (u32 *)arena_alloc(ws, 2 * kDegKeys * sizeof(u32), "relabel counts");
+    u32 *deg_t = (u32 *)arena_alloc(ws, n * sizeof(u32), "relabel degrees");
+    const u64 ntiles = (n + kScanTile - 1) / kScanTile;
+    u64 *tiles = (u64 *)arena_alloc(ws, (ntiles + 1) * sizeof(u64), "relabel scan tiles");
+    if (failed()) return;
+    u32 *cursor = counts + kDegKeys;
+    const unsigned nblk = (unsigned)((n + (u64)kRelabelBlock * kRelabelItems - 1) / ((u64)kRelabelBlock * kRelabelItems));
+    G500_HIP(hipMemsetAsync(counts, 0, kDegKeys * sizeof(u32), s));
+    relabel_count<<<nblk, kRelabelBlock, 0, s>>>(g->rowstarts, n, counts);
+    relabel_offsets<<<1, 1024, 0, s>>>(counts, cursor);
+    relabel_scatter<<<nblk, kRelabelBlock, 0, s>>>(g->rowstarts, n, cursor, g->perm, g->iperm, deg_t);
+    scan_tile_sums<<<(unsigned)ntiles, kScanBlock, 0, s>>>(deg_t, n, tiles);
+    scan_tile_offsets<<<1, 1024, 0, s>>>(tiles, ntiles, g->rowstarts_t, n);
+    scan_write<<<(unsigned)ntiles, kScanBlock, 0, s>>>(deg_t, n, tiles, g->rowstarts_t);
+    G500_HIP(hipGetLastError());
+    // rows longer than kWave entries: keys below kDegKeys - 1 - kWave
+    u32 hc[kDegKeys];
+    G500_HIP(hipMemcpyAsync(hc, counts, sizeof(hc), hipMemcpyDeviceToHost, s));
+    sync();
+    u64 nlong = 0;
+    for (u32 k = 0; k < kDegKeys - 1 - (u32)kWave; ++k) nlong += hc[k];
+    if (nlong)
+        relabel_rows_block<<<(unsigned)MIN(nlong, (u64)c.num_cus * 8), 256, 0, s>>>(
+            g->rowstarts, g->column, g->weights, g->perm, g->iperm, g->rowstarts_t, nlong, g->column_t,
+            g->weights_t);
+    relabel_rows_lane<<<grid_for(n - nlong, 256, (u64)c.num_cus * 32), 256, 0, s>>>(
+        g->rowstarts, g->column, g->weights, g->perm, g->iperm, g->rowstarts_t, nlong, n, g->column_t,
+        g->weights_t);
+    G500_HIP(hipGetLastError());
+    sync();
+    ws->gpu_arena.used = mark;
+}
+
 }  // namespace g500
 
 using namespace g500;
@@ -454,8 +605,9 @@ Oned_Graph oned_graph_from_tuple_graph(Worker_State *ws, Tuple_Graph *tg, u64 ng
             g.rowstarts, t3, t3_off, L, g.column, (u32 *)g.weights, s_lo, s_hi, col_passes);
     }
     G500_HIP(hipGetLastError());
-    timer_stop();
     ws->gpu_arena.used = mark_sort;  // release the scratch
+    if (c.relabel) build_traversal_copy(ws, &g);
+    timer_stop();
     (void)mark;
     return g;
 }

@@ -76,6 +76,7 @@ struct SsspArgs {
     const u32 *col;
     const f32 *w;
     const u32 *split;
+    const u32 *perm, *iperm;  // traversal <-> original ids (null: identity)
     u64 *lab;
     u32 *fq[2][3];   // light queues [buffer][tier]; tier 2 holds hub vertices
     u64 *fq_off[2];  // hub-tier work offsets per buffer
@@ -274,11 +275,13 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
     };
 
     if (gtid == 0) {
-        st_relaxed(&a.lab[(u32)a.root], (u64)(u32)a.root);  // distance 0.0f, parent = root
+        const u32 r = trav_id(a.perm, a.root);
+        st_relaxed(&a.lab[r], (u64)r);  // distance 0.0f, parent = root
         for (int i = 0; i < 3; ++i) st_relaxed(&ctl->nmin[i], 0xFFFFFFFFu);
     }
     sync_traced(ctl, kPhaseInit, 0, 0);
-    if (gtid < kWave) push_light(a, lane == 0, (u32)a.root, target(0, 0.f, 0.f));  // step 0 queues the root
+    if (gtid < kWave)  // step 0 queues the root
+        push_light(a, lane == 0, trav_id(a.perm, a.root), target(0, 0.f, 0.f));
     sync_traced(ctl, kPhaseInit, 1, 0);
     x = 1;
 
@@ -413,12 +416,13 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
         for (int i = 0; i < 3; ++i) st_relaxed(&ctl->ticks[i], t_acc[i]);
         st_relaxed(&ctl->ticks[3], wall_clock64() - t_start);
     }
-    // export (traversal.cu:333-340): unreached -> -1.0f / -1
+    // export in original ids (traversal.cu:333-340): unreached -> -1.0f / -1
     for (u64 v = gtid; v < a.n; v += gthreads) {
         const u64 l = a.lab[v];
         const bool reached = (u32)(l >> 32) < kInfBits;
-        a.dist[v] = reached ? label_dist(l) : -1.0f;
-        a.pred[v] = reached ? (i64)(u32)l : -1;
+        const u32 o = orig_id(a.iperm, (u32)v);
+        a.dist[o] = reached ? label_dist(l) : -1.0f;
+        a.pred[o] = reached ? (i64)orig_id(a.iperm, (u32)l) : -1;
     }
 }
 
@@ -504,15 +508,19 @@ void oned_graph_sssp_run(Worker_State *ws, const Oned_Graph *g, Sssp_State *s, u
         return;
     }
     const u64 n = g->nlocalverts;
-    if (s->split_delta != s->delta) {  // light/heavy split of every row for this delta
-        compute_split(g->rowstarts, g->weights, n, s->delta, s->split);
+    const TraversalView tv = traversal_view(g);
+    if (s->split_delta != s->delta || s->split_graph != (const void *)tv.rs) {  // light/heavy split of every row
+        compute_split(tv.rs, tv.w, n, s->delta, s->split);
         s->split_delta = s->delta;
+        s->split_graph = (const void *)tv.rs;
     }
     SsspArgs a;
     a.n = n;
-    a.rs = g->rowstarts;
-    a.col = g->column;
-    a.w = g->weights;
+    a.rs = tv.rs;
+    a.col = tv.col;
+    a.w = tv.w;
+    a.perm = tv.perm;
+    a.iperm = tv.iperm;
     a.split = s->split;
     a.lab = s->dist_pred;
     for (int b = 0; b < 2; ++b) {

@@ -55,6 +55,7 @@ struct SsspBatchArgs {
     const u32 *col;
     const f32 *w;
     const u32 *split;
+    const u32 *perm, *iperm;  // traversal <-> original ids (null: identity)
     u64 *lab;          // [n * 16]
     u32 *pend[2];      // [n] pending root masks, by step parity
     u32 *fq[2][3];     // light queues [buffer][tier]; tier 2 holds hub vertices
@@ -358,7 +359,7 @@ __global__ __launch_bounds__(512) void sssp_batch_kernel(SsspBatchArgs a) {
     };
 
     if (gtid < (u64)a.nroots) {
-        const u64 root = a.roots[gtid];
+        const u64 root = trav_id(a.perm, a.roots[gtid]);
         st_relaxed(&a.lab[root * kBatchRoots + gtid], root);  // distance 0.0f, parent = root
     }
     if (gtid == 0)
@@ -366,7 +367,7 @@ __global__ __launch_bounds__(512) void sssp_batch_kernel(SsspBatchArgs a) {
     bsync(ctl, 1, 0, 0);
     if (gtid < kWave) {  // step 0 queues the roots (repeated roots share one item)
         const bool r = (i32)gtid < a.nroots;
-        push_item(a, r ? (u32)a.roots[gtid] : 0u, r ? 1u << gtid : 0u, target(0, 0.f, 0.f));
+        push_item(a, r ? trav_id(a.perm, a.roots[gtid]) : 0u, r ? 1u << gtid : 0u, target(0, 0.f, 0.f));
     }
     bsync(ctl, 1, 1, 0);
     x = 1;
@@ -504,8 +505,9 @@ __global__ __launch_bounds__(512) void sssp_batch_kernel(SsspBatchArgs a) {
         if (r >= a.nroots) continue;
         const u64 l = a.lab[tt];
         const bool reached = (u32)(l >> 32) < kInfBits;
-        a.dist[(u64)r * a.n + v] = reached ? label_dist(l) : -1.0f;
-        a.pred[(u64)r * a.n + v] = reached ? (i64)(u32)l : -1;
+        const u64 o = (u64)r * a.n + orig_id(a.iperm, (u32)v);
+        a.dist[o] = reached ? label_dist(l) : -1.0f;
+        a.pred[o] = reached ? (i64)orig_id(a.iperm, (u32)l) : -1;
     }
 }
 
@@ -580,10 +582,11 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
     const bool library = stream == nullptr;
     const hipStream_t st = library ? c.stream : stream;
     // the split depends on the graph too: a state reused across graphs recomputes it
-    if (s->split_delta != s->delta || s->split_graph != (const void *)g->rowstarts) {
-        compute_split(g->rowstarts, g->weights, n, s->delta, s->split, st);
+    const TraversalView tv = traversal_view(g);
+    if (s->split_delta != s->delta || s->split_graph != (const void *)tv.rs) {
+        compute_split(tv.rs, tv.w, n, s->delta, s->split, st);
         s->split_delta = s->delta;
-        s->split_graph = (const void *)g->rowstarts;
+        s->split_graph = (const void *)tv.rs;
     }
     sssp_batch_clear_kernel<<<grid_for(n * kBatchRoots, 256, (u64)c.num_cus * 64), 256, 0, st>>>(
         s->labels, s->pending, s->hub_aux, n);
@@ -591,9 +594,11 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
     G500_HIP(hipMemsetAsync(s->ctl, 0, sizeof(SsspBatchCtl), st));
     SsspBatchArgs a;
     a.n = n;
-    a.rs = g->rowstarts;
-    a.col = g->column;
-    a.w = g->weights;
+    a.rs = tv.rs;
+    a.col = tv.col;
+    a.w = tv.w;
+    a.perm = tv.perm;
+    a.iperm = tv.iperm;
     a.split = s->split;
     a.lab = s->labels;
     for (int b = 0; b < 2; ++b) {

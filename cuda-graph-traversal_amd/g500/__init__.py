@@ -39,7 +39,8 @@ class TupleGraph(C.Structure):
 
 class OnedGraph(C.Structure):
     _fields_ = [("nlocalverts", u64), ("nlocaledges", u64), ("nglobalverts", u64), ("notisolated", u64),
-                ("rowstarts", vp), ("column", vp), ("weights", vp)]
+                ("rowstarts", vp), ("column", vp), ("weights", vp), ("perm", vp), ("iperm", vp),
+                ("rowstarts_t", vp), ("column_t", vp), ("weights_t", vp)]
 
 
 class BfsState(C.Structure):
@@ -51,7 +52,7 @@ class BfsState(C.Structure):
 class SsspState(C.Structure):
     _fields_ = [("pred", vp), ("dist", vp), ("dist_pred", vp), ("frontier", vp), ("heavy", vp),
                 ("hub_offsets", vp), ("hub_lengths", vp), ("split", vp), ("stamp", vp), ("settled", vp), ("ctl", vp),
-                ("delta", f32), ("split_delta", f32), ("max_nlocalverts", u64)]
+                ("split_graph", vp), ("delta", f32), ("split_delta", f32), ("max_nlocalverts", u64)]
 
 
 class BfsBatchState(C.Structure):

@@ -13,6 +13,8 @@
  *     u64; row offsets stay u64. Each row is stored in ascending
  *     (weight, column) order, so the CSR is deterministic (the reference's
  *     in-row order depends on atomic arrival, traversal.cu:51).
+ *   - Oned_Graph also carries a degree-ordered traversal copy of the CSR
+ *     (fields after `weights`); results are reported in the original ids.
  *   - Bfs_State keeps pred/dist/max_nlocalverts; the frontier counters and
  *     grid-barrier words (traversal.h:23-26) live in the device control block
  *     `ctl`; frontiers are u32 queues plus bitmaps for bottom-up levels.
@@ -41,6 +43,16 @@ struct Oned_Graph {
     u64 *rowstarts; /* [nlocalverts + 1] */
     u32 *column;    /* [nlocaledges] */
     f32 *weights;   /* [nlocaledges] */
+    /* Traversal copy built with the CSR: vertices renumbered by decreasing
+     * degree (so the most-probed labels share cache lines); same rows and
+     * entry order. BFS/SSSP run on it and report in the original ids. All
+     * NULL when the copy is switched off (g500_set_param("relabel", 0)): the
+     * traversals then run on the arrays above. */
+    u32 *perm;          /* [n] original id -> traversal id */
+    u32 *iperm;         /* [n] traversal id -> original id */
+    u64 *rowstarts_t;   /* [n + 1] */
+    u32 *column_t;      /* [nlocaledges] traversal ids */
+    f32 *weights_t;     /* [nlocaledges] */
 };
 
 /* traversal.h:18-27 */
@@ -72,6 +84,7 @@ struct Sssp_State {
     u32 *stamp;         /* [n] light-queue de-duplication */
     u64 *settled;       /* [ceil(n/64)] vertices whose distance is final */
     void *ctl;          /* device control block */
+    const void *split_graph; /* row offsets that `split` was computed for */
     f32 delta;          /* bucket width (default 0.1f) */
     f32 split_delta;    /* delta that `split` was computed for (0: none yet) */
     u64 max_nlocalverts;

@@ -116,6 +116,53 @@ def test_csr_edge_cases(arena, oracle):
         check_csr(arena, oracle, n, e, w)
 
 
+def test_traversal_copy_is_a_relabeling(arena, oracle):
+    """The degree-ordered traversal copy holds the same rows as the CSR under
+    a permutation, in decreasing degree order, entry order unchanged."""
+    e, w, ref = oracle_graph(14)
+    n = 1 << 14
+    g = check_csr(arena, oracle, n, e, w)
+    perm = arena.to_host(g.perm, n, np.uint32).astype(np.int64)
+    iperm = arena.to_host(g.iperm, n, np.uint32).astype(np.int64)
+    rs_t = arena.to_host(g.rowstarts_t, n + 1, np.uint64).astype(np.int64)
+    col_t = arena.to_host(g.column_t, g.nlocaledges, np.uint32).astype(np.int64)
+    w_t = arena.to_host(g.weights_t, g.nlocaledges, np.float32)
+    assert np.array_equal(np.sort(perm), np.arange(n)) and np.array_equal(iperm[perm], np.arange(n))
+    rs = ref.rowstarts.astype(np.int64)
+    deg = np.diff(rs)
+    deg_t = np.diff(rs_t)
+    assert np.array_equal(deg_t, deg[iperm])
+    assert np.all(np.diff(np.minimum(deg_t, 4095)) <= 0)
+    src = np.repeat(rs[iperm] - rs_t[:-1], deg_t) + np.arange(g.nlocaledges)
+    assert np.array_equal(col_t, perm[ref.column.astype(np.int64)[src]])
+    assert np.array_equal(w_t.view(np.uint32), ref.weights[src].view(np.uint32))
+
+
+def test_without_traversal_copy(arena, oracle):
+    """With the copy switched off the kernels run on the CSR itself."""
+    e, w, ref = oracle_graph(12)
+    arena.set_param("relabel", 0)
+    try:
+        g = check_csr(arena, oracle, 1 << 12, e, w)
+        assert not g.perm and not g.column_t
+        arena.bfs_state = arena.bfs_create(g)
+        s = arena.sssp_create(g)
+        sb = arena.sssp_batch_create(g)
+        bb = arena.batch_create(g)
+        roots = oracle.roots(ref)
+        for r in roots[:3]:
+            bfs_check(arena, oracle, g, ref, int(r))
+            arena.sssp(g, s, int(r))
+            d, p = arena.sssp_result(s)
+            d_ref, _ = oracle.dijkstra_f32(ref, int(r))
+            assert np.array_equal(d.view(np.uint32), d_ref.view(np.uint32))
+            assert oracle.sssp_check(ref, int(r), d, p) == 0
+        batch_check(arena, oracle, g, ref, bb, roots)
+        sssp_batch_check(arena, oracle, g, ref, sb, roots[:16])
+    finally:
+        arena.set_param("relabel", 1)
+
+
 # ------------------------------------------------------------------------ BFS
 def bfs_check(sess, oracle, g, ref, root):
     sess.bfs(g, sess.bfs_state, root)
