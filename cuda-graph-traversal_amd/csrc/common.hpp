@@ -284,10 +284,20 @@ __device__ __forceinline__ void st_relaxed(i32 *p, i32 v) {
 // Grid barrier for cooperative launches: generation counter + arrival count.
 // Every wait is bounded; on timeout it raises *error and returns so that all
 // waves still leave the kernel.
-struct alignas(256) GridBarrier {
+// Arrivals are counted in kBarrierGroups group counters, each in its own
+// 256-byte line (block b counts in group b % groups); the last arrival of a
+// group counts in the top counter, and the last group arrival opens the next
+// generation. One counter for the whole grid would serialize every block's
+// atomic on one address.
+constexpr u32 kBarrierGroups = 16;
+struct alignas(256) BarrierLine {
     u32 count;
+};
+struct alignas(256) GridBarrier {
+    u32 count;   // groups arrived
     u32 gen;
     u32 error;   // set on timeout or by kernels that detect an inconsistency
+    BarrierLine group[kBarrierGroups];
 };
 
 // Per-block trace of the last barrier passed (count, phase, two tags), for
@@ -300,13 +310,24 @@ struct BlockTrace {
 __device__ __forceinline__ void grid_sync(GridBarrier *b, u32 *error) {
     __syncthreads();
     if (threadIdx.x == 0) {
-        u32 g = __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const u32 g = __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        u32 arrived = __hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == gridDim.x - 1) {
-            __hip_atomic_store(&b->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&b->gen, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
+        const u32 ngroups = gridDim.x < kBarrierGroups ? gridDim.x : kBarrierGroups;
+        const u32 grp = blockIdx.x % ngroups;
+        const u32 members = (gridDim.x - grp + ngroups - 1) / ngroups;
+        bool opened = false;
+        const u32 arrived =
+            __hip_atomic_fetch_add(&b->group[grp].count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == members - 1) {
+            __hip_atomic_store(&b->group[grp].count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const u32 top = __hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (top == ngroups - 1) {
+                __hip_atomic_store(&b->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&b->gen, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                opened = true;
+            }
+        }
+        if (!opened) {
             // poll with relaxed loads (an acquire load would invalidate this XCD's L2
             // on every poll); one acquire fence follows the wait
             const u64 t0 = wall_clock64();

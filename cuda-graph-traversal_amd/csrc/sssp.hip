@@ -43,6 +43,8 @@ namespace g500 {
 
 constexpr u32 kSmall = 32;
 constexpr u32 kHub = 1024;
+constexpr u32 kStepLog = 1024;
+
 // Error bits: 1 barrier timeout, 2 hub list overflow, 4 queue entry out of
 // range, 8 queue length above n, 16 bucket invariant violated.
 struct SsspCtl {
@@ -52,8 +54,14 @@ struct SsspCtl {
     u32 hlen[3];                   // heavy wave-tier list length [slot]
     u64 hhub[3];                   // heavy hub-tier counters [slot]
     u32 nmin[3];                   // smallest distance bits >= hi, per slot
+    u32 nnext[3];                  // labels that entered [hi, hi + delta), per slot (nonzero: bucket not empty)
     alignas(64) u64 buckets, light_iters, improvements, scans;
     u64 ticks[4];                  // constant-clock ticks of block 0: light, fused, min+collect, kernel
+    // per-step log (block 0): kind (0 light, 1 fused, 2 min, 3 collect), queue
+    // entries of tiers 0/1, packed hub counter, constant-clock ticks
+    u32 step_kind[kStepLog], step_n0[kStepLog], step_n1[kStepLog];
+    u64 step_hub[kStepLog], step_ticks[kStepLog];
+    u32 nsteps;
     alignas(256) BlockTrace trace[kMaxTraceBlocks];
 };
 
@@ -87,6 +95,7 @@ struct SsspArgs {
     u32 *hq_len;
     u32 *stamp;
     u64 *settled;    // bit per vertex: distance final (its bucket is done)
+    u64 *reached;    // bit per vertex: label finite (set on the first relaxation)
     SsspCtl *ctl;
     i64 *pred;
     f32 *dist;
@@ -108,14 +117,17 @@ struct Target {
 // Wave-converged: queues v (lanes with `want`) into the light queue of `t`,
 // once per queue (stamp tag), in the work tier of its light segment.
 __device__ __forceinline__ void push_light(const SsspArgs &a, bool want, u32 v, const Target &t) {
-    if (!ballot(want)) return;
+    const u64 wm = ballot(want);
+    if (!wm) return;
+    SsspCtl *ctl = a.ctl;
+    if (lane_id() == (u32)(__ffsll((unsigned long long)wm) - 1)) atomicAdd(&ctl->nnext[t.slot], (u32)__popcll(wm));
+    // a vertex without light entries has nothing to do in light steps
     bool q = false;
     u32 lw = 0;
     if (want) {
-        q = atomicExch(&a.stamp[v], t.tag) != t.tag;
-        if (q) lw = a.split[v];
+        lw = a.split[v];
+        if (lw) q = atomicExch(&a.stamp[v], t.tag) != t.tag;
     }
-    SsspCtl *ctl = a.ctl;
 #pragma unroll
     for (int tier = 0; tier < 2; ++tier) {
         const bool qt = q && tier_of(lw) == tier;
@@ -126,22 +138,62 @@ __device__ __forceinline__ void push_light(const SsspArgs &a, bool want, u32 v, 
                a.n, &ctl->bar.error);
 }
 
-// One relaxation per active lane, then the wave-converged pushes.
-__device__ __forceinline__ void visit_edge(const SsspArgs &a, const Target &t, bool act, u32 u, f32 du, u64 k,
-                                           u64 &improved) {
-    bool push = false;
-    u32 v = 0;
-    if (act) {
-        v = a.col[k];
-        const f32 cand = du + a.w[k];
-        // a settled vertex has d_v < floor <= cand: skip its label
-        if (!((a.settled[v >> 6] >> (v & 63)) & 1) && relax(&a.lab[v], cand, u)) {
+// Lowers *p from o (its value as last read) to `want` while the distance
+// bits stay above cb; true with the replaced value when this call did it.
+__device__ __forceinline__ bool cas_lower(u64 *p, u64 o, u64 want, u32 cb, u64 &replaced) {
+    while ((u32)(o >> 32) > cb) {
+        const u64 prev = atomicCAS((unsigned long long *)p, (unsigned long long)o, (unsigned long long)want);
+        if (prev == o) {
+            replaced = o;
+            return true;
+        }
+        o = prev;
+    }
+    return false;
+}
+
+// kQ relaxations per lane, in phases (columns and weights, settled bits,
+// labels, compare-and-swap, pushes) so that each wave keeps kQ independent
+// chains of loads in flight; then the wave-converged pushes.
+constexpr int kQ = 2;
+
+__device__ __forceinline__ void visit_edges(const SsspArgs &a, const Target &t, const bool (&act)[kQ],
+                                            const u32 (&u)[kQ], const f32 (&du)[kQ], const u64 (&k)[kQ],
+                                            u64 &improved) {
+    u32 v[kQ];
+    f32 cand[kQ];
+    bool live[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        live[q] = act[q];
+        v[q] = live[q] ? a.col[k[q]] : 0u;
+        cand[q] = live[q] ? du[q] + a.w[k[q]] : 0.f;
+    }
+    // a settled vertex has d_v < floor <= cand: skip its label
+#pragma unroll
+    for (int q = 0; q < kQ; ++q)
+        if (live[q]) live[q] = !((a.settled[v[q] >> 6] >> (v[q] & 63)) & 1);
+    u64 old[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) old[q] = live[q] ? a.lab[v[q]] : 0ull;
+    bool push[kQ], first[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const u32 cb = __float_as_uint(cand[q]);
+        u64 replaced = 0;
+        const bool ok = live[q] && cas_lower(&a.lab[v[q]], old[q], ((u64)cb << 32) | u[q], cb, replaced);
+        first[q] = ok && (u32)(replaced >> 32) >= kInfBits;
+        push[q] = ok && cand[q] < t.below;
+        if (ok) {
             ++improved;
-            push = cand < t.below;
-            if (cand < t.floor) atomicOr(&a.ctl->bar.error, 16u);
+            if (cand[q] < t.floor) atomicOr(&a.ctl->bar.error, 16u);
         }
     }
-    push_light(a, push, v, t);
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        if (first[q]) atomicOr((unsigned long long *)&a.reached[v[q] >> 6], 1ull << (v[q] & 63));
+        push_light(a, push[q], v[q], t);
+    }
 }
 
 // Segment of u's row: the light prefix or the heavy suffix.
@@ -152,10 +204,50 @@ __device__ __forceinline__ void segment(const SsspArgs &a, u32 u, u64 &b, u64 &e
     e = LIGHT ? r0 + s : a.rs[u + 1];
 }
 
-// Vertices list[0, count) of work tier 0 (a lane each) or 1 (a wave each).
+// Relaxes the concatenation of the wave's 64 segments (lane j: entries
+// [b, b + len) of u's row, u at distance du), 64 entries per step, so short
+// and uneven segments keep every lane busy.
+__device__ __forceinline__ void walk_segments(const SsspArgs &a, const Target &t, u32 u, f32 du, u64 b, u32 len,
+                                              u64 &improved, volatile int *slot) {
+    if (!ballot(len != 0)) return;
+    const u32 incl = wave_inclusive_scan(len);
+    const u32 off = incl - len, total = __shfl(incl, kWave - 1);
+    const u32 lane = lane_id();
+    for (u32 c = 0; c < total; c += kQ * kWave) {
+        bool act[kQ];
+        u32 uu[kQ];
+        f32 dd[kQ];
+        u64 kk[kQ];
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {
+            const u32 cq = c + (u32)q * kWave;
+            act[q] = false;
+            uu[q] = 0;
+            dd[q] = 0.f;
+            kk[q] = 0;
+            if (cq < total) {
+                bool inter;
+                int last;
+                int j = chunk_owner(slot, off, len, cq, inter, last);
+                if (j < 0) j = 0;
+                const u32 offj = __shfl(off, j);
+                uu[q] = __shfl(u, j);
+                dd[q] = __shfl(du, j);
+                const u64 bj = shfl64(b, j);
+                const u32 pos = cq + lane;
+                act[q] = pos < total;
+                kk[q] = bj + (pos - offj);
+            }
+        }
+        visit_edges(a, t, act, uu, dd, kk, improved);
+    }
+}
+
+// Vertices list[0, count) of work tier 0 (64 vertices per wave, segments
+// walked together) or 1 (a wave each).
 template <bool LIGHT>
 __device__ void process_tier(const SsspArgs &a, const u32 *list, u32 count, int tier, const Target &t,
-                             u64 &improved) {
+                             u64 &improved, volatile int *slot) {
     const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
     const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
@@ -174,10 +266,7 @@ __device__ void process_tier(const SsspArgs &a, const u32 *list, u32 count, int 
                     atomicOr(&a.ctl->bar.error, 4u);
                 }
             }
-            const u32 len = (u32)(e - b);
-            u32 mx = len;
-            for (int o = 32; o > 0; o >>= 1) mx = max(mx, (u32)__shfl_xor((int)mx, o));
-            for (u32 j = 0; j < mx; ++j) visit_edge(a, t, j < len, u, du, b + j, improved);
+            walk_segments(a, t, u, du, b, (u32)(e - b), improved, slot);
         }
     } else {
         for (u64 i = wave_gid; i < count; i += nwaves) {
@@ -189,7 +278,20 @@ __device__ void process_tier(const SsspArgs &a, const u32 *list, u32 count, int 
             u64 b, e;
             segment<LIGHT>(a, u, b, e);
             const f32 du = label_dist(a.lab[u]);
-            for (u64 k0 = b; k0 < e; k0 += kWave) visit_edge(a, t, k0 + lane < e, u, du, k0 + lane, improved);
+            for (u64 k0 = b; k0 < e; k0 += kQ * kWave) {
+                bool act[kQ];
+                u32 uu[kQ];
+                f32 dd[kQ];
+                u64 kk[kQ];
+#pragma unroll
+                for (int q = 0; q < kQ; ++q) {
+                    kk[q] = k0 + (u64)q * kWave + lane;
+                    act[q] = kk[q] < e;
+                    uu[q] = u;
+                    dd[q] = du;
+                }
+                visit_edges(a, t, act, uu, dd, kk, improved);
+            }
         }
     }
 }
@@ -217,10 +319,60 @@ __device__ void process_hubs(const SsspArgs &a, const u32 *hv, const u64 *hoff, 
         u64 b, e;
         segment<LIGHT>(a, u, b, e);
         const f32 du = label_dist(a.lab[u]);
+        static_assert(kHubChunk % ((u64)kQ * kWave) == 0, "a hub chunk is whole rounds of visit_edges");
 #pragma unroll
-        for (u64 j = 0; j < kHubChunk; j += kWave) {
-            const u64 off = local + j + lane;
-            visit_edge(a, t, off < len, u, du, b + off, improved);
+        for (u64 j = 0; j < kHubChunk; j += (u64)kQ * kWave) {
+            bool act[kQ];
+            u32 uu[kQ];
+            f32 dd[kQ];
+            u64 kk[kQ];
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) {
+                const u64 off = local + j + (u64)q * kWave + lane;
+                act[q] = off < len;
+                kk[q] = b + off;
+                uu[q] = u;
+                dd[q] = du;
+            }
+            visit_edges(a, t, act, uu, dd, kk, improved);
+        }
+    }
+}
+
+// Calls f(v, label distance bits) for every reached vertex that is not
+// settled, with the whole wave converged: each lane reads one 64-vertex word
+// of reached & ~settled, a ballot picks the words with such vertices, and
+// the wave walks them one word at a time (lane = vertex of the word; lanes of
+// other vertices get +inf bits). Words in which everything is
+// settled or unreached cost one load per 64 of them.
+template <typename F>
+__device__ __forceinline__ void scan_open(const SsspArgs &a, F &&f) {
+    const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
+    const u32 lane = lane_id();
+    const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
+    const u64 nwords = (a.n + 63) / 64;
+    for (u64 w0 = wave_gid * kWave; w0 < nwords; w0 += nwaves * kWave) {
+        const u64 w = w0 + lane;
+        const u64 open_l = w < nwords ? (a.reached[w] & ~a.settled[w]) : 0ull;
+        u64 todo = ballot(open_l != 0);
+        // software pipelined: the next word's labels load while f runs
+        auto take = [&](int &j, u32 &lbits) {
+            j = todo ? __ffsll((unsigned long long)todo) - 1 : -1;
+            todo &= todo - 1;
+            const u64 open = shfl64(open_l, j < 0 ? 0 : j);
+            const u64 v = (w0 + (u64)(j < 0 ? 0 : j)) * 64 + lane;
+            lbits = (j >= 0 && ((open >> lane) & 1)) ? (u32)(a.lab[v] >> 32) : kInfBits;
+        };
+        int j;
+        u32 lbits;
+        take(j, lbits);
+        while (j >= 0) {
+            int jn;
+            u32 ln;
+            take(jn, ln);
+            f((w0 + (u64)j) * 64 + lane, lbits);
+            j = jn;
+            lbits = ln;
         }
     }
 }
@@ -229,7 +381,8 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
     SsspCtl *ctl = a.ctl;
     const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
-    const u64 nround = (a.n + gthreads - 1) / gthreads * gthreads;  // uniform trip count of dense passes
+    __shared__ int slots[1024 / kWave][kWave];
+    volatile int *slot = slots[threadIdx.x / kWave];
     u64 improved = 0;
     u32 x = 0;                       // step
     f32 lo = 0.0f, hi = next_bound(0.0f, a.delta);
@@ -240,7 +393,16 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
         if (timer) {
             const u64 t = wall_clock64();
             t_acc[phase] += t - t_prev;
+            if (x < kStepLog) st_relaxed(&ctl->step_ticks[x], t - t_prev);
             t_prev = t;
+        }
+    };
+    auto log_step = [&](u32 kind, u32 n0, u32 n1, u64 hubp) {
+        if (timer && x < kStepLog) {
+            st_relaxed(&ctl->step_kind[x], kind);
+            st_relaxed(&ctl->step_n0[x], n0);
+            st_relaxed(&ctl->step_n1[x], n1);
+            st_relaxed(&ctl->step_hub[x], hubp);
         }
     };
     auto clear_ahead = [&](u32 step) {
@@ -252,6 +414,7 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
             st_relaxed(&ctl->hlen[s], 0u);
             st_relaxed(&ctl->hhub[s], 0ull);
             st_relaxed(&ctl->nmin[s], 0xFFFFFFFFu);
+            st_relaxed(&ctl->nnext[s], 0u);
         }
     };
     // Light queue of step `step`: counts per tier and the packed hub counter.
@@ -277,6 +440,7 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
     if (gtid == 0) {
         const u32 r = trav_id(a.perm, a.root);
         st_relaxed(&a.lab[r], (u64)r);  // distance 0.0f, parent = root
+        atomicOr((unsigned long long *)&a.reached[r >> 6], 1ull << (r & 63));
         for (int i = 0; i < 3; ++i) st_relaxed(&ctl->nmin[i], 0xFFFFFFFFu);
     }
     sync_traced(ctl, kPhaseInit, 0, 0);
@@ -293,9 +457,11 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
             clear_ahead(x);
             if (total == 0 || ld_relaxed(&ctl->bar.error)) break;
             if (gtid == 0) atomicAdd((unsigned long long *)&ctl->light_iters, 1ull);
+            log_step(0, cnt[0], cnt[1], hub);
             const Target t = target(x, hi, lo);
 #pragma unroll
-            for (int tier = 0; tier < 2; ++tier) process_tier<true>(a, a.fq[x & 1][tier], cnt[tier], tier, t, improved);
+            for (int tier = 0; tier < 2; ++tier)
+                process_tier<true>(a, a.fq[x & 1][tier], cnt[tier], tier, t, improved, slot);
             process_hubs<true>(a, a.fq[x & 1][2], a.fq_off[x & 1], a.fq_len[x & 1], hub, t, improved);
             sync_traced(ctl, kPhaseLight, __float_as_uint(lo), x);
             tick(0);
@@ -305,29 +471,28 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
         // fused step x (its light queue was empty): heavy edges of [lo, hi) and
         // the next bucket [hi, hi2) queued
         if (gtid == 0) atomicAdd((unsigned long long *)&ctl->buckets, 1ull);
+        log_step(1, 0, 0, 0);
         const f32 hi2 = next_bound(hi, a.delta);
         {
             const Target t = target(x, hi2, hi);
             const int hs = (int)((x + 1) % 3);
-            for (u64 v = gtid; v < nround; v += gthreads) {
+            scan_open(a, [&](u64 v, u32 lbits) {
                 bool mine = false, next = false;
                 u64 b = 0, e = 0;
                 f32 dv = 0.f;
-                if (v < a.n) {
-                    const u32 bits = (u32)(a.lab[v] >> 32);
-                    if (bits < kInfBits) {
-                        const f32 d = __uint_as_float(bits);
-                        if (d >= lo && d < hi) {
-                            mine = true;
-                            dv = d;
-                            segment<false>(a, (u32)v, b, e);
-                        } else if (d >= hi && d < hi2) {
-                            next = true;
-                        }
+                if (lbits < kInfBits) {
+                    const f32 d = __uint_as_float(lbits);
+                    if (d >= lo && d < hi) {
+                        mine = true;
+                        dv = d;
+                        segment<false>(a, (u32)v, b, e);
+                    } else if (d >= hi && d < hi2) {
+                        next = true;
                     }
                 }
+                if (!ballot(mine || next)) return;  // the word holds only later buckets
                 push_light(a, next, (u32)v, t);
-                // the bucket's vertices are final now: one word per wave (64 consecutive v)
+                // the bucket's vertices are final now: the wave owns the word of v
                 const u64 sm = ballot(mine);
                 if (sm && lane == 0) a.settled[v >> 6] |= sm;
                 const u32 len = (u32)(e - b);
@@ -336,16 +501,13 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
                 if (medium) a.hq1[s1] = (u32)v;
                 hub_append(&ctl->hhub[hs], a.hqh, a.hq_off, a.hq_len, mine && len > kHub, (u32)v, len, a.n,
                            &ctl->bar.error);
-                const u32 slen = mine && len <= kSmall ? len : 0;
-                u32 mx = slen;
-                for (int o = 32; o > 0; o >>= 1) mx = max(mx, (u32)__shfl_xor((int)mx, o));
-                for (u32 j = 0; j < mx; ++j) visit_edge(a, t, j < slen, (u32)v, dv, b + j, improved);
-            }
+                walk_segments(a, t, (u32)v, dv, b, mine && len <= kSmall ? len : 0, improved, slot);
+            });
             sync_traced(ctl, kPhaseFused, __float_as_uint(lo), x);
             const u32 nmed = ld_relaxed(&ctl->hlen[hs]);
             const u64 hh = ld_relaxed(&ctl->hhub[hs]);
             if (nmed || hub_count(hh)) {  // second phase of the same step: longer heavy rows
-                process_tier<false>(a, a.hq1, min(nmed, (u32)a.n), 1, t, improved);
+                process_tier<false>(a, a.hq1, min(nmed, (u32)a.n), 1, t, improved, slot);
                 process_hubs<false>(a, a.hqh, a.hq_off, a.hq_len, hh, t, improved);
                 sync_traced(ctl, kPhaseHeavy, __float_as_uint(lo), x);
             }
@@ -356,7 +518,8 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
         {
             u32 cnt[2];
             u64 hub;
-            if (queue_size(x, cnt, &hub)) {  // next bucket is not empty
+            // next bucket not empty: queued light work, or labels that entered it
+            if (queue_size(x, cnt, &hub) || ld_relaxed(&ctl->nnext[x % 3])) {
                 lo = hi;
                 hi = hi2;
                 continue;
@@ -364,13 +527,14 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
         }
         if (ld_relaxed(&ctl->bar.error)) break;
         // min step: smallest distance >= hi among all labels (nothing changes now)
+        log_step(2, 0, 0, 0);
         clear_ahead(x);
-        for (u64 v = gtid; v < nround; v += gthreads) {
+        {
             u32 bits = 0xFFFFFFFFu;
-            if (v < a.n) {
-                const u32 lb = (u32)(a.lab[v] >> 32);
-                if (lb < kInfBits && __uint_as_float(lb) >= hi) bits = lb;
-            }
+            scan_open(a, [&](u64 v, u32 lbits) {
+                (void)v;
+                if (lbits < kInfBits && __uint_as_float(lbits) >= hi) bits = min(bits, lbits);
+            });
             bits = wave_min(bits);
             if (lane == 0 && bits != 0xFFFFFFFFu) atomicMin(&ctl->nmin[(x + 1) % 3], bits);
         }
@@ -390,20 +554,19 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
             hi = next_bound(lo, a.delta);
         }
         // collect step: queue the bucket [lo, hi) that holds dmin
+        log_step(3, 0, 0, 0);
         clear_ahead(x);
         {
             const Target t = target(x, hi, lo);
-            for (u64 v = gtid; v < nround; v += gthreads) {
+            scan_open(a, [&](u64 v, u32 lbits) {
                 bool want = false;
-                if (v < a.n) {
-                    const u32 lb = (u32)(a.lab[v] >> 32);
-                    if (lb < kInfBits) {
-                        const f32 d = __uint_as_float(lb);
-                        want = d >= lo && d < hi;
-                    }
+                if (lbits < kInfBits) {
+                    const f32 d = __uint_as_float(lbits);
+                    want = d >= lo && d < hi;
                 }
+                if (!ballot(want)) return;
                 push_light(a, want, (u32)v, t);
-            }
+            });
         }
         sync_traced(ctl, kPhaseCollect, __float_as_uint(lo), x);
         if (gtid == 0) atomicAdd((unsigned long long *)&ctl->scans, 1ull);
@@ -413,6 +576,7 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
     improved = wave_sum(improved);
     if (lane == 0 && improved) atomicAdd((unsigned long long *)&ctl->improvements, (unsigned long long)improved);
     if (timer) {
+        st_relaxed(&ctl->nsteps, x);
         for (int i = 0; i < 3; ++i) st_relaxed(&ctl->ticks[i], t_acc[i]);
         st_relaxed(&ctl->ticks[3], wall_clock64() - t_start);
     }
@@ -441,11 +605,15 @@ __global__ __launch_bounds__(256) void sssp_split_kernel(const u64 *__restrict__
 }
 
 __global__ __launch_bounds__(256) void sssp_clear_kernel(u64 *__restrict__ lab, u32 *__restrict__ stamp,
-                                                         u64 *__restrict__ settled, u64 n) {
+                                                         u64 *__restrict__ settled, u64 *__restrict__ reached,
+                                                         u64 n) {
     for (u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (u64)gridDim.x * blockDim.x) {
         lab[v] = kInfLabel;
         stamp[v] = 0;
-        if ((v & 63) == 0) settled[v >> 6] = 0;
+        if ((v & 63) == 0) {
+            settled[v >> 6] = 0;
+            reached[v >> 6] = 0;
+        }
     }
 }
 
@@ -478,6 +646,7 @@ Sssp_State oned_graph_sssp_create(Worker_State *ws, const Oned_Graph *g) {
     s.split = (u32 *)arena_alloc(ws, n * sizeof(u32), "SSSP light split");
     s.stamp = (u32 *)arena_alloc(ws, n * sizeof(u32), "SSSP stamps");
     s.settled = (u64 *)arena_alloc(ws, (n + 63) / 64 * sizeof(u64), "SSSP settled bitmap");
+    s.reached = (u64 *)arena_alloc(ws, (n + 63) / 64 * sizeof(u64), "SSSP reached bitmap");
     s.ctl = arena_alloc(ws, sizeof(SsspCtl), "SSSP control");
     if (failed()) return s;
     oned_graph_sssp_clear(ws, &s);
@@ -489,7 +658,7 @@ void oned_graph_sssp_clear(Worker_State *ws, Sssp_State *s) {
     Context &c = ctx();
     const u64 n = s->max_nlocalverts;
     sssp_clear_kernel<<<grid_for(n, 256, (u64)c.num_cus * 32), 256, 0, c.stream>>>(s->dist_pred, s->stamp,
-                                                                                  s->settled, n);
+                                                                                  s->settled, s->reached, n);
     G500_HIP(hipGetLastError());
     G500_HIP(hipMemsetAsync(s->ctl, 0, sizeof(SsspCtl), c.stream));
     sync();
@@ -534,6 +703,7 @@ void oned_graph_sssp_run(Worker_State *ws, const Oned_Graph *g, Sssp_State *s, u
     a.hq_len = s->hub_lengths + 2 * n;
     a.stamp = s->stamp;
     a.settled = s->settled;
+    a.reached = s->reached;
     a.ctl = (SsspCtl *)s->ctl;
     a.pred = s->pred;
     a.dist = s->dist;
@@ -574,6 +744,21 @@ void g500_sssp_run_stats(const Sssp_State *s, u64 *buckets, u64 *light_iteration
     if (buckets) *buckets = h.buckets;
     if (light_iterations) *light_iterations = h.light_iters;
     if (relaxations) *relaxations = h.improvements;
+}
+
+i32 g500_sssp_step_log(const Sssp_State *s, i32 *kind, u64 *items, u64 *hub_entries, f64 *ms, i32 max_steps) {
+    static SsspCtl h;
+    g500_copy_to_host(&h, s->ctl, sizeof(h));
+    const f64 khz = ctx().wall_clock_khz;
+    i32 k = 0;
+    for (u32 x = 1; x < MIN(h.nsteps, kStepLog) && k < max_steps; ++x) {
+        if (kind) kind[k] = (i32)h.step_kind[x];
+        if (items) items[k] = (u64)h.step_n0[x] + h.step_n1[x] + (h.step_hub[x] >> kHubCountShift);
+        if (hub_entries) hub_entries[k] = h.step_hub[x] & kHubTotalMask;
+        if (ms) ms[k] = (f64)h.step_ticks[x] / khz;
+        ++k;
+    }
+    return k;
 }
 
 void g500_sssp_phase_ms(const Sssp_State *s, f64 *ms) {

@@ -34,6 +34,7 @@ namespace g500 {
 constexpr int kBatchRoots = 16;  // label stride and largest batch
 constexpr u32 kBSmall = 32;
 constexpr u32 kBHub = 1024;
+constexpr int kBDense = 4;  // labels per lane per round of a dense pass
 
 // Error bits: 1 barrier timeout, 2 hub list overflow, 4 queue entry out of
 // range, 8 queue length above n, 16 bucket invariant violated.
@@ -44,6 +45,7 @@ struct SsspBatchCtl {
     u32 hlen[3][2];               // heavy item list lengths [slot][tier 0-1]
     u64 hhub[3];                  // heavy hub counters [slot]
     u32 nmin[3];                  // smallest distance bits >= hi, per slot
+    u32 nnext[3];                 // labels that entered [hi, hi + delta), per slot (nonzero: bucket not empty)
     alignas(64) u64 buckets, light_iters, improvements, scans;
     u64 ticks[4];                 // constant-clock ticks of block 0: light, dense+heavy, min+collect, kernel
     alignas(256) BlockTrace trace[kMaxTraceBlocks];
@@ -98,14 +100,18 @@ __device__ __forceinline__ int btier(u32 len) { return len <= kBSmall ? 0 : (len
 // Wave-converged: ORs `mask` into v's pending mask; the first OR of the step
 // queues v in the tier of its light segment.
 __device__ __forceinline__ void push_item(const SsspBatchArgs &a, u32 v, u32 mask, const BTarget &t) {
+    const u64 wm = ballot(mask != 0);
+    if (!wm) return;
+    SsspBatchCtl *ctl = a.ctl;
+    if (lane_id() == (u32)(__ffsll((unsigned long long)wm) - 1)) atomicAdd(&ctl->nnext[t.slot], (u32)__popcll(wm));
+    // a vertex without light entries has nothing to do in light steps
     bool q = false;
     u32 lw = 0;
     if (mask) {
-        q = atomicOr(&a.pend[t.buf][v], mask) == 0;
-        if (q) lw = a.split[v];
+        lw = a.split[v];
+        if (lw) q = atomicOr(&a.pend[t.buf][v], mask) == 0;
     }
     if (!ballot(q)) return;
-    SsspBatchCtl *ctl = a.ctl;
 #pragma unroll
     for (int tier = 0; tier < 2; ++tier) {
         const bool qt = q && btier(lw) == tier;
@@ -308,7 +314,8 @@ __global__ __launch_bounds__(512) void sssp_batch_kernel(SsspBatchArgs a) {
     volatile int *slot = slots[threadIdx.x / kWave];
     // dense passes: 16 lanes per vertex, uniform trip count
     const u64 nlab = a.n * kBatchRoots;
-    const u64 nround = (nlab + gthreads - 1) / gthreads * gthreads;
+    const u64 dstride = (u64)kBDense * gthreads;  // kBDense labels per lane per round, loads in flight together
+    const u64 nround = (nlab + dstride - 1) / dstride * dstride;
     u64 improved = 0;
     u32 x = 0;
     f32 lo = 0.0f, hi = next_bound(0.0f, a.delta);
@@ -332,6 +339,7 @@ __global__ __launch_bounds__(512) void sssp_batch_kernel(SsspBatchArgs a) {
             st_relaxed(&ctl->hlen[s][1], 0u);
             st_relaxed(&ctl->hhub[s], 0ull);
             st_relaxed(&ctl->nmin[s], 0xFFFFFFFFu);
+            st_relaxed(&ctl->nnext[s], 0u);
         }
     };
     auto queue_size = [&](u32 step, u32 *cnt, u64 *hub) {
@@ -397,9 +405,14 @@ __global__ __launch_bounds__(512) void sssp_batch_kernel(SsspBatchArgs a) {
         {
             const BTarget t = target(x, hi2, hi);
             const int hs = (int)((x + 1) % 3);
-            for (u64 tt = gtid; tt < nround; tt += gthreads) {
-                u32 bits;
-                dense_label(tt, bits);
+            for (u64 t0 = gtid; t0 < nround; t0 += dstride) {
+              u32 lbq[kBDense];
+#pragma unroll
+              for (int q = 0; q < kBDense; ++q) dense_label(t0 + (u64)q * gthreads, lbq[q]);
+#pragma unroll
+              for (int q = 0; q < kBDense; ++q) {
+                const u64 tt = t0 + (u64)q * gthreads;
+                const u32 bits = lbq[q];
                 bool mine = false, nxt = false;
                 if (bits < kInfBits) {
                     const f32 d = __uint_as_float(bits);
@@ -424,6 +437,7 @@ __global__ __launch_bounds__(512) void sssp_batch_kernel(SsspBatchArgs a) {
                 const u64 slot = hub_append(&ctl->hhub[hs], a.hqh, a.hq_off, a.hq_len, item && btier(len) == 2, v,
                                             len, a.n, &ctl->bar.error);
                 if (slot != ~0ull) a.hq_mask[slot] = H;
+              }
             }
             bsync(ctl, 3, __float_as_uint(lo), x);
             group_tier<false>(a, 0, nullptr, a.hq[0], min(ld_relaxed(&ctl->hlen[hs][0]), (u32)a.n), t, improved, slot);
@@ -438,7 +452,8 @@ __global__ __launch_bounds__(512) void sssp_batch_kernel(SsspBatchArgs a) {
         {
             u32 cnt[2];
             u64 hub;
-            if (queue_size(x, cnt, &hub)) {  // next bucket is not empty
+            // next bucket not empty: queued light work, or labels that entered it
+            if (queue_size(x, cnt, &hub) || ld_relaxed(&ctl->nnext[x % 3])) {
                 lo = hi;
                 hi = hi2;
                 continue;
@@ -447,11 +462,14 @@ __global__ __launch_bounds__(512) void sssp_batch_kernel(SsspBatchArgs a) {
         if (ld_relaxed(&ctl->bar.error)) break;
         // min step: smallest distance >= hi over all labels
         clear_ahead(x);
-        for (u64 tt = gtid; tt < nround; tt += gthreads) {
-            u32 bits;
-            dense_label(tt, bits);
+        for (u64 t0 = gtid; t0 < nround; t0 += dstride) {
+            u32 lbq[kBDense];
+#pragma unroll
+            for (int q = 0; q < kBDense; ++q) dense_label(t0 + (u64)q * gthreads, lbq[q]);
             u32 m = 0xFFFFFFFFu;
-            if (bits < kInfBits && __uint_as_float(bits) >= hi) m = bits;
+#pragma unroll
+            for (int q = 0; q < kBDense; ++q)
+                if (lbq[q] < kInfBits && __uint_as_float(lbq[q]) >= hi) m = min(m, lbq[q]);
             m = wave_min(m);
             if (lane == 0 && m != 0xFFFFFFFFu) atomicMin(&ctl->nmin[(x + 1) % 3], m);
         }
@@ -474,17 +492,22 @@ __global__ __launch_bounds__(512) void sssp_batch_kernel(SsspBatchArgs a) {
         clear_ahead(x);
         {
             const BTarget t = target(x, hi, lo);
-            for (u64 tt = gtid; tt < nround; tt += gthreads) {
-                u32 bits;
-                dense_label(tt, bits);
-                bool in = false;
-                if (bits < kInfBits) {
-                    const f32 d = __uint_as_float(bits);
-                    in = d >= lo && d < hi;
+            for (u64 t0 = gtid; t0 < nround; t0 += dstride) {
+                u32 lbq[kBDense];
+#pragma unroll
+                for (int q = 0; q < kBDense; ++q) dense_label(t0 + (u64)q * gthreads, lbq[q]);
+#pragma unroll
+                for (int q = 0; q < kBDense; ++q) {
+                    const u64 tt = t0 + (u64)q * gthreads;
+                    bool in = false;
+                    if (lbq[q] < kInfBits) {
+                        const f32 d = __uint_as_float(lbq[q]);
+                        in = d >= lo && d < hi;
+                    }
+                    const u32 M = (u32)(ballot(in) >> (lane & 48)) & 0xFFFFu;
+                    const bool leader = (lane & 15) == 0 && tt < nlab;
+                    push_item(a, (u32)(tt / kBatchRoots), leader ? M : 0u, t);
                 }
-                const u32 M = (u32)(ballot(in) >> (lane & 48)) & 0xFFFFu;
-                const bool leader = (lane & 15) == 0 && tt < nlab;
-                push_item(a, (u32)(tt / kBatchRoots), leader ? M : 0u, t);
             }
         }
         bsync(ctl, 6, __float_as_uint(lo), x);

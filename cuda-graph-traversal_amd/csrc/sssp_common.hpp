@@ -31,6 +31,23 @@ __device__ __forceinline__ bool relax(u64 *lab, f32 cand, u32 parent) {
     return false;
 }
 
+// relax() that also tells whether the replaced label was unreached (+inf),
+// i.e. whether this call reached the vertex for the first time.
+__device__ __forceinline__ bool relax_first(u64 *lab, f32 cand, u32 parent, bool &first) {
+    const u32 cb = __float_as_uint(cand);
+    const u64 want = ((u64)cb << 32) | parent;
+    u64 old = *lab;
+    while ((u32)(old >> 32) > cb) {
+        const u64 prev = atomicCAS((unsigned long long *)lab, (unsigned long long)old, (unsigned long long)want);
+        if (prev == old) {
+            first = (u32)(old >> 32) >= kInfBits;
+            return true;
+        }
+        old = prev;
+    }
+    return false;
+}
+
 // split[v] = number of entries of row v with weight < delta (rows are sorted
 // by weight), enqueued on `stream` (0: the library stream).
 void compute_split(const u64 *rs, const f32 *w, u64 n, f32 delta, u32 *split, hipStream_t stream = nullptr);

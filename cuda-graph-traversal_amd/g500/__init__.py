@@ -51,8 +51,8 @@ class BfsState(C.Structure):
 
 class SsspState(C.Structure):
     _fields_ = [("pred", vp), ("dist", vp), ("dist_pred", vp), ("frontier", vp), ("heavy", vp),
-                ("hub_offsets", vp), ("hub_lengths", vp), ("split", vp), ("stamp", vp), ("settled", vp), ("ctl", vp),
-                ("split_graph", vp), ("delta", f32), ("split_delta", f32), ("max_nlocalverts", u64)]
+                ("hub_offsets", vp), ("hub_lengths", vp), ("split", vp), ("stamp", vp), ("settled", vp), ("reached", vp),
+                ("ctl", vp), ("split_graph", vp), ("delta", f32), ("split_delta", f32), ("max_nlocalverts", u64)]
 
 
 class BfsBatchState(C.Structure):
@@ -148,6 +148,7 @@ _SIGS = {
     "g500_bfs_level_log": (i32, [P(BfsState), P(i32), P(u64), P(f64), i32]),
     "g500_sssp_run_stats": (None, [P(SsspState), P(u64), P(u64), P(u64)]),
     "g500_sssp_phase_ms": (None, [P(SsspState), P(f64)]),
+    "g500_sssp_step_log": (i32, [P(SsspState), P(i32), P(u64), P(u64), P(f64), i32]),
 }
 
 _LIB = None
@@ -364,6 +365,14 @@ class Session:
         b, li, r = u64(), u64(), u64()
         self.lib.g500_sssp_batch_run_stats(C.byref(s), C.byref(b), C.byref(li), C.byref(r))
         return b.value, li.value, r.value
+
+    def sssp_step_log(self, s, max_steps=1023):
+        kind = (i32 * max_steps)()
+        items = (u64 * max_steps)()
+        hubs = (u64 * max_steps)()
+        ms = (f64 * max_steps)()
+        k = self.lib.g500_sssp_step_log(C.byref(s), kind, items, hubs, ms, max_steps)
+        return list(kind[:k]), list(items[:k]), list(hubs[:k]), list(ms[:k])
 
     def sssp_phase_ms(self, s):
         out = (f64 * 5)()

@@ -197,6 +197,10 @@ i32 g500_get_param(const char *name, f64 *value);
  * frontier size and device milliseconds of each level (any pointer may be
  * NULL). Returns the number of levels. */
 i32 g500_bfs_level_log(const Bfs_State *bfs, i32 *modes, u64 *frontier, f64 *ms, i32 max_levels);
+/* Per-step log of the last SSSP run (first 1023 steps): kind (0 light step,
+ * 1 fused heavy step, 2 min step, 3 collect step), queued vertices, padded
+ * hub-tier entries and device milliseconds. Returns the number of steps. */
+i32 g500_sssp_step_log(const Sssp_State *sssp, i32 *kind, u64 *items, u64 *hub_entries, f64 *ms, i32 max_steps);
 /* Bucket / light-iteration / label-improvement counts of the last SSSP run. */
 void g500_sssp_run_stats(const Sssp_State *sssp, u64 *buckets, u64 *light_iterations,
                          u64 *relaxations);

@@ -83,6 +83,7 @@ struct Sssp_State {
     u32 *split;         /* [n] number of light entries (w < split_delta) per row */
     u32 *stamp;         /* [n] light-queue de-duplication */
     u64 *settled;       /* [ceil(n/64)] vertices whose distance is final */
+    u64 *reached;       /* [ceil(n/64)] vertices with a finite label */
     void *ctl;          /* device control block */
     const void *split_graph; /* row offsets that `split` was computed for */
     f32 delta;          /* bucket width (default 0.1f) */
