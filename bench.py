@@ -78,7 +78,9 @@ def main():
                     help="roots of the plain reference pass in batched mode (0: skip)")
     ap.add_argument("--scale", type=int, default=26)
     ap.add_argument("--edgefactor", type=int, default=16)
-    ap.add_argument("--delta", type=float, default=0.1)
+    ap.add_argument("--delta", type=float, default=0.1, help="SSSP delta when not tuned")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (knobs at their defaults, delta from --delta)")
     ap.add_argument("--arena-gib", type=int, default=160)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -113,10 +115,19 @@ def main():
         lib.g500_copy_device(dst, src, nb)
         log(f"[bench] calibration copy of {args.calibrate_mib} MiB: {lib.g500_last_run_ms():.3f} ms")
         sess.release(m)
+    tuned = None
+    delta_plain = delta_batch = args.delta
+    if not args.no_tune:
+        from g500 import autotune
+        t0 = time.perf_counter()
+        tuned = autotune.tune(sess, g, batch=args.mode == "batched", log=log)["params"]
+        delta_plain = tuned["sssp_delta"]
+        delta_batch = tuned.get("batch_sssp_delta", delta_plain)
+        log(f"[bench] autotuner ({time.perf_counter() - t0:.1f} s, untimed): {tuned}")
     roots = sess.roots(g)
     nroots = len(roots)
     bfs = sess.bfs_create(g)
-    sssp = sess.sssp_create(g, args.delta)
+    sssp = sess.sssp_create(g, delta_plain)
 
     # per-root TEPS numerators and reached counts (untimed, as in the runner)
     m_c = np.zeros(nroots, np.float64)
@@ -160,7 +171,7 @@ def main():
                     f"{args.scale} edgefactor {args.edgefactor}, roots from the Graph500 root rule")
     else:
         bb = sess.batch_create(g)
-        sb = sess.sssp_batch_create(g, args.delta)
+        sb = sess.sssp_batch_create(g, delta_batch)
         rsets = [list(range(k, min(k + BFS_GROUP, nroots))) for k in range(0, nroots, BFS_GROUP)]
 
         def batched_step(grp):
@@ -243,7 +254,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic: Graph500 Kronecker generator on the GPU, user seed (2, 3), U[0,1] f32 weights",
         "config": {"workload": workload, "scale": args.scale, "edgefactor": args.edgefactor,
-                   "delta": args.delta, "roots": nroots, "mode": args.mode},
+                   "delta": delta_batch if args.mode == "batched" else delta_plain,
+                   "plain_delta": delta_plain, "roots": nroots, "mode": args.mode, "tuned": tuned},
         "bfs_gteps": round(me.sum() / tb.sum() / 1e9, 4),
         "sssp_gteps": round(me.sum() / ts.sum() / 1e9, 4),
         "plain": plain,

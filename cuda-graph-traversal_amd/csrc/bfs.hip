@@ -379,7 +379,7 @@ void oned_graph_bfs_run(Worker_State *ws, const Oned_Graph *g, Bfs_State *bfs, u
     a.alpha = (f32)c.bfs_alpha;
     a.beta = (f32)c.bfs_beta;
     a.hub_degree = (u32)c.bfs_hub_degree;
-    const int block = 512;
+    const int block = ctx().bfs_block;
     int grid = cooperative_blocks((const void *)bfs_kernel, block, 0);
     if (failed()) return;
     void *args[] = {&a};

@@ -258,9 +258,13 @@ i32 g500_set_param(const char *name, f64 value) {
     else if (n == "batch_alpha") c.batch_alpha = value;
     else if (n == "batch_beta") c.batch_beta = value;
     else if (n == "batch_heavy_degree") c.batch_heavy_degree = (int)value;
-    else if (n == "bfs_block") c.bfs_block = (int)value;
+    else if (n == "bfs_block" || n == "sssp_block") {
+        // persistent kernels are built for at most 512 threads per block (wave multiples)
+        const int v = (int)value;
+        if (v != 128 && v != 256 && v != 512) return -1;
+        (n == "bfs_block" ? c.bfs_block : c.sssp_block) = v;
+    }
     else if (n == "bfs_hub_degree") c.bfs_hub_degree = (int)value;
-    else if (n == "sssp_block") c.sssp_block = (int)value;
     else if (n == "gen_blocks") c.gen_blocks = (int)value;
     else if (n == "coop_launch") c.coop_launch = (int)value;
     else if (n == "relabel") c.relabel = (int)value;

@@ -709,7 +709,7 @@ void oned_graph_sssp_run(Worker_State *ws, const Oned_Graph *g, Sssp_State *s, u
     a.dist = s->dist;
     a.root = root;
     a.delta = s->delta;
-    const int block = 512;
+    const int block = ctx().sssp_block;
     int grid = cooperative_blocks((const void *)sssp_kernel, block, 0);
     if (failed()) return;
     void *args[] = {&a};

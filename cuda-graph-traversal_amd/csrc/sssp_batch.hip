@@ -34,7 +34,6 @@ namespace g500 {
 constexpr int kBatchRoots = 16;  // label stride and largest batch
 constexpr u32 kBSmall = 32;
 constexpr u32 kBHub = 1024;
-constexpr int kBDense = 4;  // labels per lane per round of a dense pass
 
 // Error bits: 1 barrier timeout, 2 hub list overflow, 4 queue entry out of
 // range, 8 queue length above n, 16 bucket invariant violated.
@@ -59,6 +58,9 @@ struct SsspBatchArgs {
     const u32 *split;
     const u32 *perm, *iperm;  // traversal <-> original ids (null: identity)
     u64 *lab;          // [n * 16]
+    u32 *rmask;        // [n] roots with a finite label of the vertex
+    u32 *smask;        // [n] roots whose label of the vertex is final
+    u64 *openw;        // [ceil(n/64)] vertex has a root in rmask & ~smask
     u32 *pend[2];      // [n] pending root masks, by step parity
     u32 *fq[2][3];     // light queues [buffer][tier]; tier 2 holds hub vertices
     u64 *fq_off[2];    // light hub tier: work offsets
@@ -128,7 +130,12 @@ __device__ __forceinline__ u32 relax_pair(const SsspBatchArgs &a, u32 u, int r, 
                                           u64 &improved) {
     v = a.col[k];
     const f32 cand = label_dist(a.lab[(u64)u * kBatchRoots + r]) + a.w[k];
-    if (!relax(&a.lab[(u64)v * kBatchRoots + r], cand, u)) return 0;
+    bool first = false;
+    if (!relax_first(&a.lab[(u64)v * kBatchRoots + r], cand, u, first)) return 0;
+    if (first) {  // v reached from root r: it has open work again
+        atomicOr(&a.rmask[v], 1u << r);
+        atomicOr((unsigned long long *)&a.openw[v >> 6], 1ull << (v & 63));
+    }
     ++improved;
     if (cand < t.floor) atomicOr(&a.ctl->bar.error, 16u);
     return cand < t.below ? 1u << r : 0u;
@@ -306,16 +313,15 @@ __device__ void hub_tier(const SsspBatchArgs &a, const u32 *hv, const u64 *hoff,
     }
 }
 
-__global__ __launch_bounds__(512) void sssp_batch_kernel(SsspBatchArgs a) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void sssp_batch_kernel(SsspBatchArgs a) {
     SsspBatchCtl *ctl = a.ctl;
     const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
     __shared__ int slots[512 / kWave][kWave];
     volatile int *slot = slots[threadIdx.x / kWave];
     // dense passes: 16 lanes per vertex, uniform trip count
-    const u64 nlab = a.n * kBatchRoots;
-    const u64 dstride = (u64)kBDense * gthreads;  // kBDense labels per lane per round, loads in flight together
-    const u64 nround = (nlab + dstride - 1) / dstride * dstride;
+    const u64 nwords = (a.n + 63) / 64;
+    const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
     u64 improved = 0;
     u32 x = 0;
     f32 lo = 0.0f, hi = next_bound(0.0f, a.delta);
@@ -361,14 +367,43 @@ __global__ __launch_bounds__(512) void sssp_batch_kernel(SsspBatchArgs a) {
         return BTarget{(int)((step + 1) & 1), (int)((step + 1) % 3), below, floor};
     };
     // this lane's label in a dense pass: vertex t / 16, root t % 16
-    auto dense_label = [&](u64 t, u32 &bits) {
-        bits = kInfBits;
-        if (t < nlab && (int)(t & (kBatchRoots - 1)) < a.nroots) bits = (u32)(a.lab[t] >> 32);
+    // Word-parallel pass over the vertices with open work: each lane reads one
+    // word of openw, the wave walks the words that have open vertices, one
+    // lane per vertex (its 16 labels share one line). f(word, v, A, B, dmin)
+    // gets the roots whose open label is in [x, y) (A) and in [y, z) (B), and
+    // the smallest open distance bits >= y.
+    auto scan_open = [&](f32 x, f32 y, f32 z, auto &&f) {
+        for (u64 w0 = wave_gid * kWave; w0 < nwords; w0 += nwaves * kWave) {
+            const u64 wl = w0 + lane;
+            const u64 open_l = wl < nwords ? a.openw[wl] : 0ull;
+            u64 todo = ballot(open_l != 0);
+            while (todo) {
+                const int j = __ffsll((unsigned long long)todo) - 1;
+                todo &= todo - 1;
+                const u64 word = w0 + (u64)j;
+                const u64 v = word * 64 + lane;
+                const bool isopen = (shfl64(open_l, j) >> lane) & 1;
+                const u32 live = isopen ? a.rmask[v] & ~a.smask[v] : 0u;
+                u32 A = 0, B = 0, dmin = 0xFFFFFFFFu;
+#pragma unroll
+                for (int r = 0; r < kBatchRoots; ++r) {
+                    if (!((live >> r) & 1)) continue;
+                    const u32 bits = (u32)(a.lab[v * kBatchRoots + r] >> 32);
+                    const f32 d = __uint_as_float(bits);
+                    if (d >= x && d < y) A |= 1u << r;
+                    if (d >= y && d < z) B |= 1u << r;
+                    if (d >= y) dmin = min(dmin, bits);
+                }
+                f(word, v, A, B, dmin);
+            }
+        }
     };
 
     if (gtid < (u64)a.nroots) {
         const u64 root = trav_id(a.perm, a.roots[gtid]);
         st_relaxed(&a.lab[root * kBatchRoots + gtid], root);  // distance 0.0f, parent = root
+        atomicOr(&a.rmask[root], 1u << gtid);
+        atomicOr((unsigned long long *)&a.openw[root >> 6], 1ull << (root & 63));
     }
     if (gtid == 0)
         for (int i = 0; i < 3; ++i) st_relaxed(&ctl->nmin[i], 0xFFFFFFFFu);
@@ -405,40 +440,32 @@ __global__ __launch_bounds__(512) void sssp_batch_kernel(SsspBatchArgs a) {
         {
             const BTarget t = target(x, hi2, hi);
             const int hs = (int)((x + 1) % 3);
-            for (u64 t0 = gtid; t0 < nround; t0 += dstride) {
-              u32 lbq[kBDense];
-#pragma unroll
-              for (int q = 0; q < kBDense; ++q) dense_label(t0 + (u64)q * gthreads, lbq[q]);
-#pragma unroll
-              for (int q = 0; q < kBDense; ++q) {
-                const u64 tt = t0 + (u64)q * gthreads;
-                const u32 bits = lbq[q];
-                bool mine = false, nxt = false;
-                if (bits < kInfBits) {
-                    const f32 d = __uint_as_float(bits);
-                    mine = d >= lo && d < hi;
-                    nxt = d >= hi && d < hi2;
+            scan_open(lo, hi, hi2, [&](u64 word, u64 v, u32 H, u32 N, u32 dmin) {
+                (void)dmin;
+                // the bucket's roots of v are final now; v closes when no reached root is open
+                bool closed = false;
+                if (H) {
+                    const u32 sm = a.smask[v] | H;
+                    a.smask[v] = sm;
+                    closed = (a.rmask[v] & ~sm) == 0;
                 }
-                const int sh = (int)(lane & 48);
-                const u32 H = (u32)(ballot(mine) >> sh) & 0xFFFFu, N = (u32)(ballot(nxt) >> sh) & 0xFFFFu;
-                const bool leader = (lane & 15) == 0 && tt < nlab;
-                const u32 v = (u32)(tt / kBatchRoots);
-                push_item(a, v, leader ? N : 0u, t);
+                const u64 cm = ballot(closed);
+                if (cm && lane == 0) atomicAnd((unsigned long long *)&a.openw[word], ~(unsigned long long)cm);
+                push_item(a, (u32)v, N, t);
                 u64 b = 0, e = 0;
-                if (leader && H) bsegment<false>(a, v, b, e);
+                if (H) bsegment<false>(a, (u32)v, b, e);
                 const u32 len = (u32)(e - b);
-                const bool item = leader && H && len > 0;
+                const bool item = H && len > 0;
 #pragma unroll
                 for (int tier = 0; tier < 2; ++tier) {
                     const bool want = item && btier(len) == tier;
-                    const u32 s = wave_append(&ctl->hlen[hs][tier], want);
-                    if (want) a.hq[tier][s] = ((u64)H << 32) | v;
+                    const u32 sl = wave_append(&ctl->hlen[hs][tier], want);
+                    if (want) a.hq[tier][sl] = ((u64)H << 32) | (u32)v;
                 }
-                const u64 slot = hub_append(&ctl->hhub[hs], a.hqh, a.hq_off, a.hq_len, item && btier(len) == 2, v,
-                                            len, a.n, &ctl->bar.error);
-                if (slot != ~0ull) a.hq_mask[slot] = H;
-              }
-            }
+                const u64 hslot = hub_append(&ctl->hhub[hs], a.hqh, a.hq_off, a.hq_len, item && btier(len) == 2,
+                                             (u32)v, len, a.n, &ctl->bar.error);
+                if (hslot != ~0ull) a.hq_mask[hslot] = H;
+            });
             bsync(ctl, 3, __float_as_uint(lo), x);
             group_tier<false>(a, 0, nullptr, a.hq[0], min(ld_relaxed(&ctl->hlen[hs][0]), (u32)a.n), t, improved, slot);
             wave_tier<false>(a, 0, nullptr, a.hq[1], min(ld_relaxed(&ctl->hlen[hs][1]), (u32)a.n), t, improved);
@@ -462,14 +489,15 @@ __global__ __launch_bounds__(512) void sssp_batch_kernel(SsspBatchArgs a) {
         if (ld_relaxed(&ctl->bar.error)) break;
         // min step: smallest distance >= hi over all labels
         clear_ahead(x);
-        for (u64 t0 = gtid; t0 < nround; t0 += dstride) {
-            u32 lbq[kBDense];
-#pragma unroll
-            for (int q = 0; q < kBDense; ++q) dense_label(t0 + (u64)q * gthreads, lbq[q]);
+        {
             u32 m = 0xFFFFFFFFu;
-#pragma unroll
-            for (int q = 0; q < kBDense; ++q)
-                if (lbq[q] < kInfBits && __uint_as_float(lbq[q]) >= hi) m = min(m, lbq[q]);
+            scan_open(hi, hi, hi, [&](u64 word, u64 v, u32 A, u32 B, u32 dmin) {
+                (void)word;
+                (void)v;
+                (void)A;
+                (void)B;
+                m = min(m, dmin);
+            });
             m = wave_min(m);
             if (lane == 0 && m != 0xFFFFFFFFu) atomicMin(&ctl->nmin[(x + 1) % 3], m);
         }
@@ -492,23 +520,12 @@ __global__ __launch_bounds__(512) void sssp_batch_kernel(SsspBatchArgs a) {
         clear_ahead(x);
         {
             const BTarget t = target(x, hi, lo);
-            for (u64 t0 = gtid; t0 < nround; t0 += dstride) {
-                u32 lbq[kBDense];
-#pragma unroll
-                for (int q = 0; q < kBDense; ++q) dense_label(t0 + (u64)q * gthreads, lbq[q]);
-#pragma unroll
-                for (int q = 0; q < kBDense; ++q) {
-                    const u64 tt = t0 + (u64)q * gthreads;
-                    bool in = false;
-                    if (lbq[q] < kInfBits) {
-                        const f32 d = __uint_as_float(lbq[q]);
-                        in = d >= lo && d < hi;
-                    }
-                    const u32 M = (u32)(ballot(in) >> (lane & 48)) & 0xFFFFu;
-                    const bool leader = (lane & 15) == 0 && tt < nlab;
-                    push_item(a, (u32)(tt / kBatchRoots), leader ? M : 0u, t);
-                }
-            }
+            scan_open(lo, hi, hi, [&](u64 word, u64 v, u32 M, u32 B, u32 dmin) {
+                (void)word;
+                (void)B;
+                (void)dmin;
+                push_item(a, (u32)v, M, t);
+            });
         }
         bsync(ctl, 6, __float_as_uint(lo), x);
         if (gtid == 0) atomicAdd((unsigned long long *)&ctl->scans, 1ull);
@@ -522,6 +539,7 @@ __global__ __launch_bounds__(512) void sssp_batch_kernel(SsspBatchArgs a) {
         st_relaxed(&ctl->ticks[3], wall_clock64() - t_start);
     }
     // export, root-major (traversal.cu:333-340): unreached -> -1.0f / -1
+    const u64 nlab = a.n * kBatchRoots;
     for (u64 tt = gtid; tt < nlab; tt += gthreads) {
         const u64 v = tt / kBatchRoots;
         const int r = (int)(tt & (kBatchRoots - 1));
@@ -535,14 +553,17 @@ __global__ __launch_bounds__(512) void sssp_batch_kernel(SsspBatchArgs a) {
 }
 
 __global__ __launch_bounds__(256) void sssp_batch_clear_kernel(u64 *__restrict__ lab, u32 *__restrict__ pend,
-                                                               u32 *__restrict__ done, u64 n) {
+                                                               u32 *__restrict__ done, u32 *__restrict__ masks,
+                                                               u64 *__restrict__ openw, u64 n) {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x; t < n * kBatchRoots; t += stride) {
         lab[t] = kInfLabel;
         if (t < 2 * n) {
             pend[t] = 0;
             done[t] = 0;
+            masks[t] = 0;
         }
+        if (t < (n + 63) / 64) openw[t] = 0;
     }
 }
 
@@ -612,7 +633,7 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
         s->split_graph = (const void *)tv.rs;
     }
     sssp_batch_clear_kernel<<<grid_for(n * kBatchRoots, 256, (u64)c.num_cus * 64), 256, 0, st>>>(
-        s->labels, s->pending, s->hub_aux, n);
+        s->labels, s->pending, s->hub_aux, s->root_masks, s->open_words, n);
     G500_HIP(hipGetLastError());
     G500_HIP(hipMemsetAsync(s->ctl, 0, sizeof(SsspBatchCtl), st));
     SsspBatchArgs a;
@@ -624,6 +645,9 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
     a.iperm = tv.iperm;
     a.split = s->split;
     a.lab = s->labels;
+    a.rmask = s->root_masks;
+    a.smask = s->root_masks + n;
+    a.openw = s->open_words;
     for (int b = 0; b < 2; ++b) {
         a.pend[b] = s->pending + (u64)b * n;
         for (int t = 0; t < 3; ++t) a.fq[b][t] = s->frontier + (u64)(b * 3 + t) * n;
@@ -643,7 +667,7 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
     for (i32 i = 0; i < nroots; ++i) a.roots[i] = roots[i];
     a.nroots = nroots;
     a.delta = s->delta;
-    const int block = 512;
+    const int block = ctx().sssp_block;
     if (grid <= 0) grid = cooperative_blocks((const void *)sssp_batch_kernel, block, 0);
     if (failed()) return false;
     void *args[] = {&a};
@@ -666,7 +690,7 @@ u32 sssp_batch_error(const Sssp_Batch_State *s, hipStream_t stream) {
     return e;
 }
 
-int sssp_batch_grid() { return cooperative_blocks((const void *)sssp_batch_kernel, 512, 0); }
+int sssp_batch_grid() { return cooperative_blocks((const void *)sssp_batch_kernel, ctx().sssp_block, 0); }
 
 }  // namespace g500
 
@@ -691,6 +715,8 @@ Sssp_Batch_State oned_graph_sssp_batch_create(Worker_State *ws, const Oned_Graph
     s.hub_lengths = (u32 *)arena_alloc(ws, 3 * n * sizeof(u32), "batched SSSP hub lengths");
     s.hub_aux = (u32 *)arena_alloc(ws, 3 * n * sizeof(u32), "batched SSSP hub chunk counts / masks");
     s.split = (u32 *)arena_alloc(ws, n * sizeof(u32), "batched SSSP light split");
+    s.root_masks = (u32 *)arena_alloc(ws, 2 * n * sizeof(u32), "batched SSSP reached/settled root masks");
+    s.open_words = (u64 *)arena_alloc(ws, ((n + 63) / 64) * sizeof(u64), "batched SSSP open-vertex bitmap");
     s.ctl = arena_alloc(ws, sizeof(SsspBatchCtl), "batched SSSP control");
     return s;
 }
@@ -708,6 +734,14 @@ void oned_graph_sssp_batch_run(Worker_State *ws, const Oned_Graph *g, Sssp_Batch
 
 void oned_graph_sssp_batch_destroy(Sssp_Batch_State *s) {
     if (s) memset(s, 0, sizeof(*s));
+}
+
+void g500_sssp_batch_phase_ms(const Sssp_Batch_State *s, f64 *ms) {
+    static SsspBatchCtl h;
+    g500_copy_to_host(&h, s->ctl, sizeof(h));
+    const f64 khz = ctx().wall_clock_khz;
+    for (int i = 0; i < 4; ++i) ms[i] = (f64)h.ticks[i] / khz;
+    ms[4] = (f64)h.scans;
 }
 
 void g500_sssp_batch_run_stats(const Sssp_Batch_State *s, u64 *buckets, u64 *light_iterations, u64 *relaxations) {

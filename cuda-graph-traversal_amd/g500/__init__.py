@@ -64,7 +64,7 @@ class BfsBatchState(C.Structure):
 class SsspBatchState(C.Structure):
     _fields_ = [("dist", vp), ("pred", vp), ("labels", vp), ("pending", vp), ("frontier", vp), ("heavy", vp),
                 ("hub_vertices", vp), ("hub_offsets", vp), ("hub_lengths", vp), ("hub_aux", vp), ("split", vp),
-                ("ctl", vp), ("split_graph", vp), ("delta", f32), ("split_delta", f32), ("nroots", i32),
+                ("root_masks", vp), ("open_words", vp), ("ctl", vp), ("split_graph", vp), ("delta", f32), ("split_delta", f32), ("nroots", i32),
                 ("reserved", i32),
                 ("max_nlocalverts", u64)]
 
@@ -126,6 +126,7 @@ _SIGS = {
     "oned_graph_sssp_batch_run": (None, [P(WorkerState), P(OnedGraph), P(SsspBatchState), P(u64), i32]),
     "oned_graph_sssp_batch_destroy": (None, [P(SsspBatchState)]),
     "g500_sssp_batch_run_stats": (None, [P(SsspBatchState), P(u64), P(u64), P(u64)]),
+    "g500_sssp_batch_phase_ms": (None, [P(SsspBatchState), P(f64)]),
     "g500_server_create": (vp, [P(WorkerState), i32, u64]),
     "g500_server_add_graph": (i32, [vp, P(OnedGraph)]),
     "g500_server_start": (i32, [vp, f32]),
@@ -360,6 +361,11 @@ class Session:
         n = g.nlocalverts
         return (self.to_host(s.dist + 4 * n * i, n, np.float32),
                 self.to_host(s.pred + 8 * n * i, n, np.int64))
+
+    def sssp_batch_phase_ms(self, s):
+        out = (f64 * 5)()
+        self.lib.g500_sssp_batch_phase_ms(C.byref(s), out)
+        return list(out)
 
     def sssp_batch_stats(self, s):
         b, li, r = u64(), u64(), u64()

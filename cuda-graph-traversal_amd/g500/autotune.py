@@ -1,0 +1,124 @@
+"""Autotuner for the traversal knobs of one resident graph.
+
+It times each candidate setting through the C-ABI on tuning roots and keeps
+the fastest one. The roots are non-isolated vertices of the giant component,
+drawn from a fixed numpy seed, so they are distinct from the Graph500 roots
+that a benchmark measures. Candidates:
+  plain BFS:     (bfs_alpha, bfs_beta) switch thresholds, bfs_block
+  batched BFS:   batch_alpha, batch_heavy_degree
+  plain SSSP:    delta, sssp_block
+  batched SSSP:  delta
+Every candidate gives the same depths and distances (the tuner changes only
+schedules), so the choice is by time alone.
+"""
+import numpy as np
+
+BFS_SWITCH = [(15.0, 18.0), (4.0, 18.0), (60.0, 18.0), (15.0, 60.0)]
+BLOCKS = [256, 512]
+BATCH_ALPHA = [4.0, 15.0, 60.0]
+BATCH_HEAVY = [4096, 1 << 20]
+SSSP_DELTA = [0.025, 0.05, 0.1]
+BATCH_SSSP_DELTA = [0.0125, 0.025, 0.05]
+
+
+def tuning_roots(sess, g, count, seed=2024):
+    """`count` giant-component roots from a fixed seed."""
+    n = int(g.nlocalverts)
+    deg = np.diff(sess.to_host(g.rowstarts, n + 1, np.uint64).astype(np.int64))
+    cand = np.flatnonzero(deg > 0)
+    rng = np.random.default_rng(seed)
+    b = sess.bfs_create(g)
+    sess.bfs(g, b, int(np.argmax(deg)))
+    n_giant = sum(sess.level_log(b)[1])
+    out = []
+    for v in rng.permutation(cand)[: 8 * count]:
+        sess.bfs(g, b, int(v))
+        if sum(sess.level_log(b)[1]) == n_giant:
+            out.append(int(v))
+            if len(out) == count:
+                break
+    return np.array(out, np.uint64)
+
+
+def _best(results):
+    return min(results, key=lambda r: r["ms"])
+
+
+def tune(sess, g, batch=True, roots_plain=3, seed=2024, log=None):
+    """Tunes the knobs on graph g, applies the best settings, and returns
+    them with the measurements: {"params": ..., "trials": [...]}."""
+    say = log or (lambda *a: None)
+    mark = sess.mark()  # the tuner's traversal states are released at the end
+    try:
+        return _tune(sess, g, batch, roots_plain, seed, say)
+    finally:
+        sess.release(mark)
+
+
+def _tune(sess, g, batch, roots_plain, seed, say):
+    roots = tuning_roots(sess, g, max(64, roots_plain), seed)
+    plain = roots[:roots_plain]
+    trials = []
+    old = {k: sess.get_param(k) for k in ("bfs_alpha", "bfs_beta", "bfs_block", "sssp_block", "batch_alpha",
+                                          "batch_heavy_degree")}
+
+    b = sess.bfs_create(g)
+    res = []
+    for (al, be) in BFS_SWITCH:
+        for blk in BLOCKS:
+            sess.set_param("bfs_alpha", al)
+            sess.set_param("bfs_beta", be)
+            sess.set_param("bfs_block", blk)
+            ms = float(np.mean([sess.bfs(g, b, r) for r in plain]))
+            res.append({"kind": "bfs", "bfs_alpha": al, "bfs_beta": be, "bfs_block": blk, "ms": ms})
+    best_bfs = _best(res)
+    trials += res
+    for k in ("bfs_alpha", "bfs_beta", "bfs_block"):
+        sess.set_param(k, best_bfs[k])
+    say(f"[tune] bfs {best_bfs}")
+
+    s = sess.sssp_create(g)
+    res = []
+    for d in SSSP_DELTA:
+        for blk in BLOCKS:
+            sess.set_param("sssp_block", blk)
+            s.delta = d
+            ms = float(np.mean([sess.sssp(g, s, r) for r in plain]))
+            res.append({"kind": "sssp", "delta": d, "sssp_block": blk, "ms": ms})
+    best_sssp = _best(res)
+    trials += res
+    sess.set_param("sssp_block", best_sssp["sssp_block"])
+    say(f"[tune] sssp {best_sssp}")
+
+    params = {"bfs_alpha": best_bfs["bfs_alpha"], "bfs_beta": best_bfs["bfs_beta"],
+              "bfs_block": best_bfs["bfs_block"], "sssp_block": best_sssp["sssp_block"],
+              "sssp_delta": best_sssp["delta"]}
+    if batch:
+        bb = sess.batch_create(g)
+        res = []
+        for al in BATCH_ALPHA:
+            for hv in BATCH_HEAVY:
+                sess.set_param("batch_alpha", al)
+                sess.set_param("batch_heavy_degree", hv)
+                ms = sess.batch(g, bb, roots[:64])
+                res.append({"kind": "batch_bfs", "batch_alpha": al, "batch_heavy_degree": hv, "ms": ms})
+        best_bb = _best(res)
+        trials += res
+        sess.set_param("batch_alpha", best_bb["batch_alpha"])
+        sess.set_param("batch_heavy_degree", best_bb["batch_heavy_degree"])
+        say(f"[tune] batched bfs {best_bb}")
+        sb = sess.sssp_batch_create(g)
+        res = []
+        for d in BATCH_SSSP_DELTA:
+            sb.delta = d
+            ms = sess.sssp_batch(g, sb, roots[:16])
+            res.append({"kind": "batch_sssp", "delta": d, "ms": ms})
+        best_bs = _best(res)
+        trials += res
+        say(f"[tune] batched sssp {best_bs}")
+        params.update({"batch_alpha": best_bb["batch_alpha"], "batch_heavy_degree": best_bb["batch_heavy_degree"],
+                       "batch_sssp_delta": best_bs["delta"]})
+    else:
+        for k in ("batch_alpha", "batch_heavy_degree"):
+            sess.set_param(k, old[k])
+    return {"params": params, "trials": trials, "tuning_roots": len(roots)}

@@ -92,6 +92,8 @@ struct Sssp_Batch_State {
     u32 *hub_lengths;   /* [3n] hub segment lengths */
     u32 *hub_aux;       /* [3n] light: finished chunks per hub; heavy: root masks */
     u32 *split;         /* [n] light entries (w < split_delta) per row */
+    u32 *root_masks;    /* [2n] per vertex: roots with a finite label, roots whose label is final */
+    u64 *open_words;    /* [ceil(n/64)] vertices with a reached root not yet final */
     void *ctl;          /* device control block */
     const void *split_graph; /* row offsets that `split` was computed for */
     f32 delta;          /* bucket width (default 0.1f) */
@@ -109,6 +111,9 @@ void oned_graph_sssp_batch_destroy(Sssp_Batch_State *s);
  * (improvements summed over the roots). */
 void g500_sssp_batch_run_stats(const Sssp_Batch_State *s, u64 *buckets, u64 *light_iterations,
                                u64 *relaxations);
+/* Device milliseconds of the last batched run: light steps, dense + heavy
+ * steps, min + collect steps, whole kernel; ms[4] = dense passes. */
+void g500_sssp_batch_phase_ms(const Sssp_Batch_State *s, f64 ms[5]);
 
 /* ---- serving loop -------------------------------------------------------- */
 /* Resident graphs, (graph, root, BFS | SSSP) requests and continuous batching.
@@ -186,10 +191,11 @@ void g500_copy_to_device(void *dst_device, const void *src, u64 bytes);
  * workload for calibrating HBM byte counters. */
 void g500_copy_device(void *dst_device, const void *src_device, u64 bytes);
 
-/* Tuning knobs (names: "bfs_alpha", "bfs_beta", "bfs_block", "bfs_hub_degree",
- * "batch_alpha", "batch_beta", "batch_heavy_degree", "sssp_block",
- * "gen_blocks", "coop_launch"; read-only "num_cus"). Returns 0 on success, -1
- * for an unknown name. */
+/* Tuning knobs (names: "bfs_alpha", "bfs_beta", "bfs_hub_degree", "batch_alpha",
+ * "batch_beta", "batch_heavy_degree", "bfs_block" and "sssp_block" (threads
+ * per block of the persistent BFS / SSSP kernels, single and batched: 128,
+ * 256 or 512), "gen_blocks", "coop_launch", "relabel"; read-only "num_cus").
+ * Returns 0 on success, -1 for an unknown name or a value out of range. */
 i32 g500_set_param(const char *name, f64 value);
 i32 g500_get_param(const char *name, f64 *value);
 

@@ -410,6 +410,59 @@ def test_sssp_batch_errors_return(arena):
             arena.sssp_batch(g, s, np.array(roots, np.uint64))
 
 
+# ------------------------------------------------------------- knobs / tuner
+@pytest.mark.parametrize("block", [128, 256])
+def test_block_shapes(arena, oracle, block):
+    """Persistent kernels with smaller blocks (more blocks in the grid barrier)."""
+    e, w, ref = oracle_graph(13)
+    g = check_csr(arena, oracle, 1 << 13, e, w)
+    old = {k: arena.get_param(k) for k in ("bfs_block", "sssp_block")}
+    try:
+        arena.set_param("bfs_block", block)
+        arena.set_param("sssp_block", block)
+        arena.bfs_state = arena.bfs_create(g)
+        s = arena.sssp_create(g)
+        roots = oracle.roots(ref)
+        for r in roots[:2]:
+            bfs_check(arena, oracle, g, ref, int(r))
+            arena.sssp(g, s, int(r))
+            d, p = arena.sssp_result(s)
+            d_ref, _ = oracle.dijkstra_f32(ref, int(r))
+            assert np.array_equal(d.view(np.uint32), d_ref.view(np.uint32))
+        batch_check(arena, oracle, g, ref, arena.batch_create(g), roots)
+        sssp_batch_check(arena, oracle, g, ref, arena.sssp_batch_create(g), roots[:16])
+    finally:
+        for k, v in old.items():
+            arena.set_param(k, v)
+    import g500
+    with pytest.raises(KeyError):
+        arena.set_param("bfs_block", 300)
+
+
+def test_autotuner_applies_candidates(arena, oracle):
+    from g500 import autotune
+    e, w, ref = oracle_graph(14)
+    g = check_csr(arena, oracle, 1 << 14, e, w)
+    keys = ("bfs_alpha", "bfs_beta", "bfs_block", "sssp_block", "batch_alpha", "batch_heavy_degree")
+    old = {k: arena.get_param(k) for k in keys}
+    try:
+        out = autotune.tune(arena, g, batch=True)
+        p = out["params"]
+        assert (p["bfs_alpha"], p["bfs_beta"]) in autotune.BFS_SWITCH and p["bfs_block"] in autotune.BLOCKS
+        assert p["sssp_delta"] in autotune.SSSP_DELTA and p["batch_sssp_delta"] in autotune.BATCH_SSSP_DELTA
+        assert p["batch_alpha"] in autotune.BATCH_ALPHA and p["batch_heavy_degree"] in autotune.BATCH_HEAVY
+        for k in keys:
+            assert arena.get_param(k) == p[k]
+        roots = oracle.roots(ref)
+        arena.bfs_state = arena.bfs_create(g)
+        bfs_check(arena, oracle, g, ref, int(roots[0]))
+        sb = arena.sssp_batch_create(g, p["batch_sssp_delta"])
+        sssp_batch_check(arena, oracle, g, ref, sb, roots[:16])
+    finally:
+        for k, v in old.items():
+            arena.set_param(k, v)
+
+
 # --------------------------------------------------------------------- runner
 def test_runner_small():
     import g500
