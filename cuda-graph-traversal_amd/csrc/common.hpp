@@ -17,6 +17,7 @@
 namespace g500 {
 
 constexpr int kWave = 64;
+constexpr int kSsspBatchRoots = 32;  // roots (and label stride) of a batched SSSP launch
 
 struct Context {
     int device = -1;

@@ -287,6 +287,7 @@ i32 g500_get_param(const char *name, f64 *value) {
     else if (n == "coop_launch") *value = c.coop_launch;
     else if (n == "relabel") *value = c.relabel;
     else if (n == "num_cus") *value = c.num_cus;
+    else if (n == "sssp_batch_max") *value = kSsspBatchRoots;
     else return -1;
     return 0;
 }

@@ -6,7 +6,7 @@
 // stream, and one batched BFS and one batched SSSP state sized for the largest
 // graph) repeatedly takes the oldest pending request's (graph, kind) and as
 // many further pending requests of that pair as a launch holds (64 BFS roots,
-// 16 SSSP roots) and as the result pool admits: each request reserves its
+// 32 SSSP roots) and as the result pool admits: each request reserves its
 // result footprint (BFS: i64 depths; SSSP: f32 distances + i64 parents) in a
 // device pool, and a request that does not fit waits until completions free
 // memory. Requests that arrive while launches run join the next launch of a
@@ -147,7 +147,7 @@ std::vector<Request *> take_batch(G500_Server *s) {
     std::vector<Request *> batch;
     if (s->pending.empty()) return batch;
     const i32 graph = s->pending.front()->graph, kind = s->pending.front()->kind;
-    const size_t cap = kind == G500_BFS ? 64 : 16;
+    const size_t cap = kind == G500_BFS ? 64 : kSsspBatchRoots;
     for (auto it = s->pending.begin(); it != s->pending.end() && batch.size() < cap;) {
         Request *r = *it;
         if (r->graph != graph || r->kind != kind) {
@@ -219,7 +219,7 @@ void slot_loop(G500_Server *s, int k) {
         G500_HIP(hipMemcpyAsync(summary, slot.summary, sizeof(summary), hipMemcpyDeviceToHost, slot.stream));
         G500_HIP(hipStreamSynchronize(slot.stream));
         const double t_staged = now_ms();
-        const int R = bfs ? 64 : 16;
+        const int R = bfs ? 64 : kSsspBatchRoots;
         {
             std::lock_guard<std::mutex> lk(s->mu);
             for (size_t i = 0; i < batch.size(); ++i) {

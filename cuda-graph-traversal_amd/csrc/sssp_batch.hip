@@ -1,12 +1,12 @@
-// Batched delta-stepping SSSP: up to 16 roots in one persistent cooperative
+// Batched delta-stepping SSSP: up to 32 roots in one persistent cooperative
 // kernel with one bucket sequence shared by all roots. No counterpart in the
 // reference, whose runner solves one root at a time
 // (examples/graph500_runner.cu:393-416); each root's distances equal the
 // single-root kernel's, because every terminating label-correcting schedule
 // reaches the same FP32 fixed point.
 //
-// Labels are vertex-major: lab[v * 16 + r] = (distance bits << 32) | parent of
-// vertex v from root r, so the 16 labels of a vertex share one 128-byte line.
+// Labels are vertex-major: lab[v * 32 + r] = (distance bits << 32) | parent of
+// vertex v from root r, so the labels of a vertex share two 128-byte lines.
 // Buckets are [lo, hi) with hi = fl(lo + delta) (traversal.cu:693-694), the
 // same sequence for every root. Work items are (vertex, root mask) pairs: one
 // pass over a row serves every root of the mask, and relaxing an edge for
@@ -16,8 +16,9 @@
 //     the weight-sorted row) are relaxed for every root of M; a label lowered
 //     below hi ORs its root into the target's pending mask for the next step,
 //     and the first OR queues the target;
-//   dense step: 16 lanes per vertex read its labels; roots with d in [lo, hi)
-//     form the vertex's heavy item, roots with d in [hi, fl(hi + delta)) are
+//   dense step: a lane per vertex with open work reads its labels; roots with
+//     d in [lo, hi) form the vertex's heavy item (and become final), roots with
+//     d in [hi, fl(hi + delta)) are
 //     queued for the next bucket; after a barrier the heavy items relax their
 //     heavy edges (never into [lo, hi), as for one root);
 //   when the next bucket is empty: a min step over all labels and a collect
@@ -31,7 +32,7 @@
 
 namespace g500 {
 
-constexpr int kBatchRoots = 16;  // label stride and largest batch
+constexpr int kBatchRoots = kSsspBatchRoots;  // label stride and largest batch
 constexpr u32 kBSmall = 32;
 constexpr u32 kBHub = 1024;
 
@@ -57,7 +58,7 @@ struct SsspBatchArgs {
     const f32 *w;
     const u32 *split;
     const u32 *perm, *iperm;  // traversal <-> original ids (null: identity)
-    u64 *lab;          // [n * 16]
+    u64 *lab;          // [n * kBatchRoots]
     u32 *rmask;        // [n] roots with a finite label of the vertex
     u32 *smask;        // [n] roots whose label of the vertex is final
     u64 *openw;        // [ceil(n/64)] vertex has a root in rmask & ~smask
@@ -319,7 +320,6 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
     const u32 lane = lane_id();
     __shared__ int slots[512 / kWave][kWave];
     volatile int *slot = slots[threadIdx.x / kWave];
-    // dense passes: 16 lanes per vertex, uniform trip count
     const u64 nwords = (a.n + 63) / 64;
     const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
     u64 improved = 0;
@@ -366,10 +366,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
     auto target = [&](u32 step, f32 below, f32 floor) {
         return BTarget{(int)((step + 1) & 1), (int)((step + 1) % 3), below, floor};
     };
-    // this lane's label in a dense pass: vertex t / 16, root t % 16
     // Word-parallel pass over the vertices with open work: each lane reads one
     // word of openw, the wave walks the words that have open vertices, one
-    // lane per vertex (its 16 labels share one line). f(word, v, A, B, dmin)
+    // lane per vertex (its labels are adjacent). f(word, v, A, B, dmin)
     // gets the roots whose open label is in [x, y) (A) and in [y, z) (B), and
     // the smallest open distance bits >= y.
     auto scan_open = [&](f32 x, f32 y, f32 z, auto &&f) {
@@ -385,7 +384,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                 const bool isopen = (shfl64(open_l, j) >> lane) & 1;
                 const u32 live = isopen ? a.rmask[v] & ~a.smask[v] : 0u;
                 u32 A = 0, B = 0, dmin = 0xFFFFFFFFu;
-#pragma unroll
+#pragma unroll 8
                 for (int r = 0; r < kBatchRoots; ++r) {
                     if (!((live >> r) & 1)) continue;
                     const u32 bits = (u32)(a.lab[v * kBatchRoots + r] >> 32);

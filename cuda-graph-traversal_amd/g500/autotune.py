@@ -111,7 +111,7 @@ def _tune(sess, g, batch, roots_plain, seed, say):
         res = []
         for d in BATCH_SSSP_DELTA:
             sb.delta = d
-            ms = sess.sssp_batch(g, sb, roots[:16])
+            ms = sess.sssp_batch(g, sb, roots[:int(sess.get_param("sssp_batch_max"))])
             res.append({"kind": "batch_sssp", "delta": d, "ms": ms})
         best_bs = _best(res)
         trials += res

@@ -73,17 +73,17 @@ i32 g500_bfs_batch_level_log(const Bfs_Batch_State *b, i32 *modes, u64 *records,
                              i32 max_levels);
 
 /* ---- batched delta-stepping SSSP ---------------------------------------- */
-/* Up to 16 roots in one persistent kernel with one bucket sequence shared by
- * all roots; items are (vertex, root mask) pairs and the 16 labels of a vertex
- * share one 128-byte line. No counterpart in the reference (its runner solves
+/* Up to 32 roots in one persistent kernel with one bucket sequence shared by
+ * all roots; items are (vertex, root mask) pairs and the labels of a vertex
+ * share two 128-byte lines. No counterpart in the reference (its runner solves
  * one root at a time, graph500_runner.cu:393-416); each root's distances equal
  * the single-root oned_graph_sssp_run's (the FP32 fixed point is unique), and
  * its parents satisfy the same certificate. */
 typedef struct Sssp_Batch_State Sssp_Batch_State;
 struct Sssp_Batch_State {
-    f32 *dist;          /* [16n] root-major: root index r, vertex v at r * n + v; -1.0f unreached */
-    i64 *pred;          /* [16n] root-major parents; -1 unreached; root -> root */
-    u64 *labels;        /* [16n] vertex-major working labels (v * 16 + r) */
+    f32 *dist;          /* [32n] root-major: root index r, vertex v at r * n + v; -1.0f unreached */
+    i64 *pred;          /* [32n] root-major parents; -1 unreached; root -> root */
+    u64 *labels;        /* [32n] vertex-major working labels (v * 32 + r) */
     u32 *pending;       /* [2n] pending root masks, two steps */
     u32 *frontier;      /* [6n] light queues: 2 buffers x 3 work tiers */
     u64 *heavy;         /* [2n] heavy items (mask << 32 | vertex), two work tiers */
@@ -103,7 +103,7 @@ struct Sssp_Batch_State {
     u64 max_nlocalverts;
 };
 Sssp_Batch_State oned_graph_sssp_batch_create(Worker_State *ws, const Oned_Graph *g);
-/* roots: host array of nroots (1..16) vertex ids (repeats allowed). */
+/* roots: host array of nroots (1..32, g500_get_param("sssp_batch_max")) vertex ids (repeats allowed). */
 void oned_graph_sssp_batch_run(Worker_State *ws, const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roots,
                                i32 nroots);
 void oned_graph_sssp_batch_destroy(Sssp_Batch_State *s);
@@ -120,7 +120,7 @@ void g500_sssp_batch_phase_ms(const Sssp_Batch_State *s, f64 ms[5]);
  * No counterpart in the reference (its runner is a batch program). Execution
  * slots each own a HIP stream and batched BFS / SSSP states; a free slot takes
  * the oldest pending request's (graph, kind) plus further pending requests of
- * that pair (up to 64 BFS / 16 SSSP roots) that the device result pool admits
+ * that pair (up to 64 BFS / 32 SSSP roots) that the device result pool admits
  * (footprint per request: BFS n * 8 bytes of depths; SSSP n * 4 bytes of
  * distances + n * 8 bytes of parents when requested). Slots run concurrently,
  * each with its share of the device. Results are those of the batched entry
@@ -194,7 +194,7 @@ void g500_copy_device(void *dst_device, const void *src_device, u64 bytes);
 /* Tuning knobs (names: "bfs_alpha", "bfs_beta", "bfs_hub_degree", "batch_alpha",
  * "batch_beta", "batch_heavy_degree", "bfs_block" and "sssp_block" (threads
  * per block of the persistent BFS / SSSP kernels, single and batched: 128,
- * 256 or 512), "gen_blocks", "coop_launch", "relabel"; read-only "num_cus").
+ * 256 or 512), "gen_blocks", "coop_launch", "relabel"; read-only "num_cus", "sssp_batch_max").
  * Returns 0 on success, -1 for an unknown name or a value out of range. */
 i32 g500_set_param(const char *name, f64 value);
 i32 g500_get_param(const char *name, f64 *value);

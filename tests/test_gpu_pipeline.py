@@ -372,10 +372,11 @@ def test_sssp_batch_bitexact(arena, oracle, scale):
     g = check_csr(arena, oracle, 1 << scale, e, w)
     s = arena.sssp_batch_create(g)
     roots = oracle.roots(ref)
+    width = int(arena.get_param("sssp_batch_max"))
     for delta in (0.1, 0.25, 0.02):
         s.delta = delta
         say("sssp batch", scale, delta)
-        sssp_batch_check(arena, oracle, g, ref, s, roots[:16])
+        sssp_batch_check(arena, oracle, g, ref, s, roots[:width])
     # fewer roots, repeats, an isolated vertex as a root
     s.delta = 0.1
     isolated = int(np.flatnonzero(np.diff(ref.rowstarts.astype(np.int64)) == 0)[0])
@@ -405,7 +406,8 @@ def test_sssp_batch_errors_return(arena):
     e, w, ref = oracle_graph(10)
     g = arena.csr(arena.upload_tuples(e, w), 1 << 10)
     s = arena.sssp_batch_create(g)
-    for roots in ([], list(range(17)), [1 << 10]):
+    width = int(arena.get_param("sssp_batch_max"))
+    for roots in ([], list(range(width + 1)), [1 << 10]):
         with pytest.raises(g500.LibraryError):
             arena.sssp_batch(g, s, np.array(roots, np.uint64))
 
