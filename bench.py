@@ -80,7 +80,7 @@ def main():
     ap.add_argument("--delta", type=float, default=0.1, help="SSSP delta when not tuned")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (knobs at their defaults, delta from --delta)")
-    ap.add_argument("--arena-gib", type=int, default=160)
+    ap.add_argument("--arena-gib", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--serve-requests", type=int, default=512,
@@ -127,6 +127,7 @@ def main():
         log(f"[bench] autotuner ({time.perf_counter() - t0:.1f} s, untimed): {tuned}")
     roots = sess.roots(g)
     nroots = len(roots)
+    states = sess.mark()  # traversal states below are released before the serving leg
     bfs = sess.bfs_create(g)
     sssp = sess.sssp_create(g, delta_plain)
 
@@ -274,6 +275,7 @@ def main():
     if args.mode == "plain":
         out["bfs_hm_gteps"] = round(hm(tb, me) / 1e9, 6)
         out["sssp_hm_gteps"] = round(hm(ts, me) / 1e9, 6)
+    sess.release(states)
     if args.serve_requests:
         from g500 import serving
         log("[bench] serving leg: loading 8 resident graphs")
