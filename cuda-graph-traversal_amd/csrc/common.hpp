@@ -282,6 +282,14 @@ __device__ __forceinline__ void st_relaxed(i32 *p, i32 v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// A counter alone in a 256-byte line: atomics on one line are serialized by
+// the L2 channel that owns it, so counters that many waves hit at once each
+// get a line.
+template <typename T>
+struct alignas(256) CounterLine {
+    T v;
+};
+
 // Grid barrier for cooperative launches: generation counter + arrival count.
 // Every wait is bounded; on timeout it raises *error and returns so that all
 // waves still leave the kernel.

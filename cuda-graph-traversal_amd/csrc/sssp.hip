@@ -2,13 +2,19 @@
 // Reference: src/traversal.cu:209-340 (kernels), :570-716 (state, host bucket
 // loop with delta = 0.1f).
 //
-// Labels: one u64 per vertex, (f32 distance bits << 32) | parent. Distances
-// are non-negative, so their bit patterns order like the floats; a relaxation
-// lowers a label with a compare-and-swap loop only while the candidate is
-// strictly smaller (the reference's CAS plus per-vertex spin lock,
-// traversal.cu:211-240, becomes one lock-free 64-bit CAS). Every terminating
-// label-correcting schedule reaches the same FP32 fixed point, so the schedule
-// below does not change any distance.
+// Labels: one u64 per vertex, (f32 distance bits << 32) | parent, and a u32
+// probe per vertex with the distance bits alone (the state's dist array, in
+// traversal ids while the kernel runs). Distances are non-negative, so their
+// bit patterns order like the floats. A relaxation reads the probe, lowers it
+// with a 32-bit atomic min, and only when that strictly lowered it commits
+// (distance, parent) to the label with a 64-bit atomic min (the reference's
+// CAS plus per-vertex spin lock, traversal.cu:211-240). Each strict lowering
+// of a probe is by a distinct value, so two label commits never tie and the
+// 64-bit min keeps the commit with the smaller distance: the label ends as
+// (final distance, a parent that achieved it). Failed relaxations, the large
+// majority, touch only the probe array, half the label array's size. Every
+// terminating label-correcting schedule reaches the same FP32 fixed point, so
+// the schedule below does not change any distance.
 //
 // Buckets follow the reference's bounds: [lo, hi) with hi = fl(lo + delta)
 // (traversal.cu:693-694). Rows are sorted by weight, so the light edges
@@ -86,6 +92,7 @@ struct SsspArgs {
     const u32 *split;
     const u32 *perm, *iperm;  // traversal <-> original ids (null: identity)
     u64 *lab;
+    u32 *dl;         // probe: distance bits per vertex (aliases the dist output)
     u32 *fq[2][3];   // light queues [buffer][tier]; tier 2 holds hub vertices
     u64 *fq_off[2];  // hub-tier work offsets per buffer
     u32 *fq_len[2];  // hub-tier segment lengths per buffer
@@ -138,23 +145,9 @@ __device__ __forceinline__ void push_light(const SsspArgs &a, bool want, u32 v, 
                a.n, &ctl->bar.error);
 }
 
-// Lowers *p from o (its value as last read) to `want` while the distance
-// bits stay above cb; true with the replaced value when this call did it.
-__device__ __forceinline__ bool cas_lower(u64 *p, u64 o, u64 want, u32 cb, u64 &replaced) {
-    while ((u32)(o >> 32) > cb) {
-        const u64 prev = atomicCAS((unsigned long long *)p, (unsigned long long)o, (unsigned long long)want);
-        if (prev == o) {
-            replaced = o;
-            return true;
-        }
-        o = prev;
-    }
-    return false;
-}
-
 // kQ relaxations per lane, in phases (columns and weights, settled bits,
-// labels, compare-and-swap, pushes) so that each wave keeps kQ independent
-// chains of loads in flight; then the wave-converged pushes.
+// probes, atomic mins, pushes) so that each wave keeps kQ independent chains
+// of loads in flight; then the wave-converged pushes.
 constexpr int kQ = 2;
 
 __device__ __forceinline__ void visit_edges(const SsspArgs &a, const Target &t, const bool (&act)[kQ],
@@ -173,16 +166,24 @@ __device__ __forceinline__ void visit_edges(const SsspArgs &a, const Target &t, 
 #pragma unroll
     for (int q = 0; q < kQ; ++q)
         if (live[q]) live[q] = !((a.settled[v[q] >> 6] >> (v[q] & 63)) & 1);
-    u64 old[kQ];
+    // a plain read may be stale, i.e. larger: it only lets an atomic through
+    u32 old[kQ];
 #pragma unroll
-    for (int q = 0; q < kQ; ++q) old[q] = live[q] ? a.lab[v[q]] : 0ull;
+    for (int q = 0; q < kQ; ++q) old[q] = live[q] ? a.dl[v[q]] : 0u;
     bool push[kQ], first[kQ];
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {
         const u32 cb = __float_as_uint(cand[q]);
-        u64 replaced = 0;
-        const bool ok = live[q] && cas_lower(&a.lab[v[q]], old[q], ((u64)cb << 32) | u[q], cb, replaced);
-        first[q] = ok && (u32)(replaced >> 32) >= kInfBits;
+        bool ok = false;
+        if (live[q] && old[q] > cb) {
+            const u32 prev = atomicMin(&a.dl[v[q]], cb);
+            ok = prev > cb;
+            if (ok) {
+                atomicMin((unsigned long long *)&a.lab[v[q]], ((unsigned long long)cb << 32) | u[q]);
+                old[q] = prev;
+            }
+        }
+        first[q] = ok && old[q] >= kInfBits;
         push[q] = ok && cand[q] < t.below;
         if (ok) {
             ++improved;
@@ -261,7 +262,7 @@ __device__ void process_tier(const SsspArgs &a, const u32 *list, u32 count, int 
                 u = list[i];
                 if (u < a.n) {
                     segment<LIGHT>(a, u, b, e);
-                    du = label_dist(a.lab[u]);
+                    du = __uint_as_float(a.dl[u]);
                 } else {
                     atomicOr(&a.ctl->bar.error, 4u);
                 }
@@ -277,7 +278,7 @@ __device__ void process_tier(const SsspArgs &a, const u32 *list, u32 count, int 
             }
             u64 b, e;
             segment<LIGHT>(a, u, b, e);
-            const f32 du = label_dist(a.lab[u]);
+            const f32 du = __uint_as_float(a.dl[u]);
             for (u64 k0 = b; k0 < e; k0 += kQ * kWave) {
                 bool act[kQ];
                 u32 uu[kQ];
@@ -318,7 +319,7 @@ __device__ void process_hubs(const SsspArgs &a, const u32 *hv, const u64 *hoff, 
         const u32 len = hlen[i];
         u64 b, e;
         segment<LIGHT>(a, u, b, e);
-        const f32 du = label_dist(a.lab[u]);
+        const f32 du = __uint_as_float(a.dl[u]);
         static_assert(kHubChunk % ((u64)kQ * kWave) == 0, "a hub chunk is whole rounds of visit_edges");
 #pragma unroll
         for (u64 j = 0; j < kHubChunk; j += (u64)kQ * kWave) {
@@ -339,7 +340,7 @@ __device__ void process_hubs(const SsspArgs &a, const u32 *hv, const u64 *hoff, 
     }
 }
 
-// Calls f(v, label distance bits) for every reached vertex that is not
+// Calls f(v, probe distance bits) for every reached vertex that is not
 // settled, with the whole wave converged: each lane reads one 64-vertex word
 // of reached & ~settled, a ballot picks the words with such vertices, and
 // the wave walks them one word at a time (lane = vertex of the word; lanes of
@@ -361,7 +362,7 @@ __device__ __forceinline__ void scan_open(const SsspArgs &a, F &&f) {
             todo &= todo - 1;
             const u64 open = shfl64(open_l, j < 0 ? 0 : j);
             const u64 v = (w0 + (u64)(j < 0 ? 0 : j)) * 64 + lane;
-            lbits = (j >= 0 && ((open >> lane) & 1)) ? (u32)(a.lab[v] >> 32) : kInfBits;
+            lbits = (j >= 0 && ((open >> lane) & 1)) ? a.dl[v] : kInfBits;
         };
         int j;
         u32 lbits;
@@ -440,6 +441,7 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
     if (gtid == 0) {
         const u32 r = trav_id(a.perm, a.root);
         st_relaxed(&a.lab[r], (u64)r);  // distance 0.0f, parent = root
+        st_relaxed(&a.dl[r], 0u);
         atomicOr((unsigned long long *)&a.reached[r >> 6], 1ull << (r & 63));
         for (int i = 0; i < 3; ++i) st_relaxed(&ctl->nmin[i], 0xFFFFFFFFu);
     }
@@ -580,7 +582,9 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
         for (int i = 0; i < 3; ++i) st_relaxed(&ctl->ticks[i], t_acc[i]);
         st_relaxed(&ctl->ticks[3], wall_clock64() - t_start);
     }
-    // export in original ids (traversal.cu:333-340): unreached -> -1.0f / -1
+    // export in original ids (traversal.cu:333-340): unreached -> -1.0f / -1.
+    // The probe is no longer read after the last barrier, so dist (its
+    // storage) can take the output.
     for (u64 v = gtid; v < a.n; v += gthreads) {
         const u64 l = a.lab[v];
         const bool reached = (u32)(l >> 32) < kInfBits;
@@ -604,11 +608,12 @@ __global__ __launch_bounds__(256) void sssp_split_kernel(const u64 *__restrict__
     }
 }
 
-__global__ __launch_bounds__(256) void sssp_clear_kernel(u64 *__restrict__ lab, u32 *__restrict__ stamp,
-                                                         u64 *__restrict__ settled, u64 *__restrict__ reached,
-                                                         u64 n) {
+__global__ __launch_bounds__(256) void sssp_clear_kernel(u64 *__restrict__ lab, u32 *__restrict__ dl,
+                                                         u32 *__restrict__ stamp, u64 *__restrict__ settled,
+                                                         u64 *__restrict__ reached, u64 n) {
     for (u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (u64)gridDim.x * blockDim.x) {
         lab[v] = kInfLabel;
+        dl[v] = kInfBits;
         stamp[v] = 0;
         if ((v & 63) == 0) {
             settled[v >> 6] = 0;
@@ -657,8 +662,9 @@ void oned_graph_sssp_clear(Worker_State *ws, Sssp_State *s) {
     (void)ws;
     Context &c = ctx();
     const u64 n = s->max_nlocalverts;
-    sssp_clear_kernel<<<grid_for(n, 256, (u64)c.num_cus * 32), 256, 0, c.stream>>>(s->dist_pred, s->stamp,
-                                                                                  s->settled, s->reached, n);
+    sssp_clear_kernel<<<grid_for(n, 256, (u64)c.num_cus * 32), 256, 0, c.stream>>>(s->dist_pred, (u32 *)s->dist,
+                                                                                  s->stamp, s->settled, s->reached,
+                                                                                  n);
     G500_HIP(hipGetLastError());
     G500_HIP(hipMemsetAsync(s->ctl, 0, sizeof(SsspCtl), c.stream));
     sync();
@@ -692,6 +698,7 @@ void oned_graph_sssp_run(Worker_State *ws, const Oned_Graph *g, Sssp_State *s, u
     a.iperm = tv.iperm;
     a.split = s->split;
     a.lab = s->dist_pred;
+    a.dl = (u32 *)s->dist;
     for (int b = 0; b < 2; ++b) {
         for (int t = 0; t < 3; ++t) a.fq[b][t] = s->frontier + (u64)(b * 3 + t) * n;
         a.fq_off[b] = s->hub_offsets + (u64)b * n;

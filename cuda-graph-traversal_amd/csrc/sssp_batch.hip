@@ -6,7 +6,13 @@
 // reaches the same FP32 fixed point.
 //
 // Labels are vertex-major: lab[v * 32 + r] = (distance bits << 32) | parent of
-// vertex v from root r, so the labels of a vertex share two 128-byte lines.
+// vertex v from root r, with a u32 probe dl[v * 32 + r] of the distance bits
+// alone (the state's dist array while the kernel runs), so the 32 probes of a
+// vertex fill one 128-byte line. A relaxation reads the probe, lowers it with
+// a 32-bit atomic min and, only when that strictly lowered it, commits
+// (distance, parent) to the label with a 64-bit atomic min; strict lowerings
+// of a probe are by distinct values, so the label ends as (final distance, a
+// parent that achieved it), as in the single-root kernel.
 // Buckets are [lo, hi) with hi = fl(lo + delta) (traversal.cu:693-694), the
 // same sequence for every root. Work items are (vertex, root mask) pairs: one
 // pass over a row serves every root of the mask, and relaxing an edge for
@@ -16,11 +22,19 @@
 //     the weight-sorted row) are relaxed for every root of M; a label lowered
 //     below hi ORs its root into the target's pending mask for the next step,
 //     and the first OR queues the target;
-//   dense step: a lane per vertex with open work reads its labels; roots with
-//     d in [lo, hi) form the vertex's heavy item (and become final), roots with
-//     d in [hi, fl(hi + delta)) are
-//     queued for the next bucket; after a barrier the heavy items relax their
-//     heavy edges (never into [lo, hi), as for one root);
+//   dense step: a lane per vertex with open work reads its probes; roots with
+//     d in [lo, hi) become final and form the vertex's medium item, roots with
+//     d in [hi, fl(hi + delta)) are queued for the next bucket; after a barrier
+//     the medium items relax their medium edges (delta <= w < W, never into
+//     [lo, hi), as for one root);
+//   long edges (w >= W, the row's suffix) are deferred: the dense step ORs the
+//     settled roots into the vertex's deferred mask and queues the vertex once.
+//     A relaxation from a root settled at d >= e (e: the floor of the first
+//     bucket since the last flush) lands at fl(d + w) >= fl(e + W), so the
+//     deferred queue is flushed (its long edges relaxed for every collected
+//     root) only before a bucket reaching past fl(e + W) starts, and before a
+//     min step. A vertex settles a handful of roots per window instead of one
+//     per bucket, and each flushed entry probes one line for all of them;
 //   when the next bucket is empty: a min step over all labels and a collect
 //     step queueing the bucket that holds the minimum.
 // Work is mapped by segment length: a lane per item (<= 32 entries), a wave
@@ -35,19 +49,33 @@ namespace g500 {
 constexpr int kBatchRoots = kSsspBatchRoots;  // label stride and largest batch
 constexpr u32 kBSmall = 32;
 constexpr u32 kBHub = 1024;
+constexpr u32 kBStepLog = 4096;
 
 // Error bits: 1 barrier timeout, 2 hub list overflow, 4 queue entry out of
 // range, 8 queue length above n, 16 bucket invariant violated.
 struct SsspBatchCtl {
     GridBarrier bar;
-    alignas(256) u32 flen[3][2];  // light queue lengths [slot][tier 0-1]
-    u64 fhub[3];                  // light queue hub counters [slot]
-    u32 hlen[3][2];               // heavy item list lengths [slot][tier 0-1]
-    u64 hhub[3];                  // heavy hub counters [slot]
-    u32 nmin[3];                  // smallest distance bits >= hi, per slot
-    u32 nnext[3];                 // labels that entered [hi, hi + delta), per slot (nonzero: bucket not empty)
-    alignas(64) u64 buckets, light_iters, improvements, scans;
-    u64 ticks[4];                 // constant-clock ticks of block 0: light, dense+heavy, min+collect, kernel
+    // Every counter that waves append to has a 256-byte line of its own, so
+    // that appends to different lists do not queue behind each other.
+    CounterLine<u32> flen[3][2];  // light queue lengths [slot][tier 0-1]
+    CounterLine<u64> fhub[3];     // light queue hub counters [slot]
+    CounterLine<u32> hlen[3][2];  // medium item list lengths [slot][tier 0-1]
+    CounterLine<u64> hhub[3];     // medium hub counters [slot]
+    CounterLine<u32> llen[2][2];  // deferred queue lengths [set][tier 0-1], set = flush parity
+    CounterLine<u64> lhub[2];     // deferred queue hub counters [set]
+    alignas(256) u32 nmin[3];     // smallest distance bits >= hi, per slot
+    u32 nnext[3];                 // 1: a label entered [hi, hi + delta) (the bucket is not empty), per slot
+    alignas(256) u64 buckets, light_iters, improvements, scans;
+    u64 flushes;
+    u64 ticks[6];                 // constant-clock ticks of block 0: light, dense scan, medium, flush,
+                                  // min+collect, kernel
+    u64 work[4][3];               // [light, medium, long][items, entries, (entry, root) pairs];
+                                  // [dense scans][open words walked, vertices reading their line, settling]
+    // per-phase log (block 0): phase (0 light, 1 dense scan, 2 medium, 3 flush,
+    // 4 min + collect), items processed, constant-clock ticks
+    u32 nlog;
+    u32 log_kind[kBStepLog], log_items[kBStepLog];
+    u64 log_ticks[kBStepLog];
     alignas(256) BlockTrace trace[kMaxTraceBlocks];
 };
 
@@ -57,10 +85,13 @@ struct SsspBatchArgs {
     const u32 *col;
     const f32 *w;
     const u32 *split;
+    const u32 *lsplit;  // [n] entries with w < W (long edges are the rest)
     const u32 *perm, *iperm;  // traversal <-> original ids (null: identity)
     u64 *lab;          // [n * kBatchRoots]
+    u32 *dl;           // [n * kBatchRoots] probes (aliases dist)
     u32 *rmask;        // [n] roots with a finite label of the vertex
     u32 *smask;        // [n] roots whose label of the vertex is final
+    u32 *vmin;         // [n] smallest probe bits of the vertex's open roots
     u64 *openw;        // [ceil(n/64)] vertex has a root in rmask & ~smask
     u32 *pend[2];      // [n] pending root masks, by step parity
     u32 *fq[2][3];     // light queues [buffer][tier]; tier 2 holds hub vertices
@@ -72,12 +103,18 @@ struct SsspBatchArgs {
     u64 *hq_off;       //   work offsets,
     u32 *hq_len;       //   segment lengths,
     u32 *hq_mask;      //   root masks
+    u32 *lpend;        // [n] deferred root masks
+    u32 *lq[3];        // deferred queue tiers; tier 2 holds hub vertices
+    u64 *lq_off;       //   hub work offsets,
+    u32 *lq_len;       //   segment lengths,
+    u32 *lq_done;      //   finished chunks
     SsspBatchCtl *ctl;
     f32 *dist;         // [nroots * n], root-major
     i64 *pred;         // [nroots * n]
     u64 roots[kBatchRoots];
     i32 nroots;
     f32 delta;
+    f32 wlong;         // W >= delta
 };
 
 __device__ __forceinline__ void bsync(SsspBatchCtl *ctl, u32 phase, u32 a, u32 b) {
@@ -100,13 +137,132 @@ struct BTarget {
 
 __device__ __forceinline__ int btier(u32 len) { return len <= kBSmall ? 0 : (len <= kBHub ? 1 : 2); }
 
-// Wave-converged: ORs `mask` into v's pending mask; the first OR of the step
-// queues v in the tier of its light segment.
-__device__ __forceinline__ void push_item(const SsspBatchArgs &a, u32 v, u32 mask, const BTarget &t) {
+// Wave-buffered appends. A wave stages what it appends to the light queue
+// tiers, the deferred queue tiers (u32 vertices) and the medium lists (u64
+// items) in LDS and writes them out in runs: when its buffer for a list is
+// full (one atomic for the run) and at the end of the phase, where the block
+// sums its waves' counts and makes one atomic per list. Counters then see one
+// atomic per block and phase instead of one per wave and relaxation round.
+constexpr u32 kQCap = 64;
+constexpr int kMaxBlockWaves = 512 / kWave;
+enum : int { kQLight0 = 0, kQLong0 = 2, kQ32 = 4, kQMed0 = 0, kQ64 = 2 };
+
+struct QLds {
+    u32 v32[kMaxBlockWaves][kQ32][kQCap];
+    u64 v64[kMaxBlockWaves][kQ64][kQCap];
+    u32 cnt[kMaxBlockWaves][kQ32 + kQ64];  // counts at the phase end, then the waves' offsets
+};
+
+struct WaveQ {
+    QLds *L;
+    int w;                 // wave of the block
+    u32 n[kQ32 + kQ64];    // staged entries per list (u32 lists first)
+    int buf, slot;         // light queue buffer and counter slot of the phase's pushes
+    int hs, ls;            // medium list slot and deferred queue set of a dense scan
+};
+
+// Destination array and counter of list l (u32 lists 0-3, u64 lists 4-5).
+__device__ __forceinline__ u32 *q_dst32(const SsspBatchArgs &a, const WaveQ &q, int l) {
+    // a select, not an index: a runtime index into the argument struct would
+    // copy the struct to scratch
+    if (l < kQLong0) return q.buf ? a.fq[1][l] : a.fq[0][l];
+    return a.lq[l - kQLong0];
+}
+__device__ __forceinline__ u32 *q_counter(const SsspBatchArgs &a, const WaveQ &q, int l) {
+    SsspBatchCtl *ctl = a.ctl;
+    if (l < kQLong0) return &ctl->flen[q.slot][l].v;
+    if (l < kQ32) return &ctl->llen[q.ls][l - kQLong0].v;
+    return &ctl->hlen[q.hs][l - kQ32].v;
+}
+
+// Writes the staged entries [0, cnt) of one list at dst[base...].
+template <typename T>
+__device__ __forceinline__ void q_copy(const volatile T *buf, u32 cnt, T *dst, u32 base, u64 cap) {
+    for (u32 i = lane_id(); i < cnt; i += kWave) {
+        const u64 o = (u64)base + i;
+        if (o < cap) dst[o] = buf[i];
+    }
+}
+
+template <int L>
+__device__ __forceinline__ void q_push32(const SsspBatchArgs &a, WaveQ &q, bool want, u32 val) {
+    const u64 m = ballot(want);
+    if (!m) return;
+    const u32 k = (u32)__popcll(m);
+    volatile u32 *buf = q.L->v32[q.w][L];
+    if (q.n[L] + k > kQCap) {  // spill the full buffer as one run
+        u32 base = 0;
+        if (lane_id() == 0) base = atomicAdd(q_counter(a, q, L), q.n[L]);
+        base = __shfl(base, 0);
+        q_copy(buf, q.n[L], q_dst32(a, q, L), base, a.n);
+        q.n[L] = 0;
+    }
+    if (want) buf[q.n[L] + mask_rank(m)] = val;
+    q.n[L] += k;
+}
+
+template <int L>
+__device__ __forceinline__ void q_push64(const SsspBatchArgs &a, WaveQ &q, bool want, u64 val) {
+    const u64 m = ballot(want);
+    if (!m) return;
+    const u32 k = (u32)__popcll(m);
+    volatile u64 *buf = q.L->v64[q.w][L];
+    if (q.n[kQ32 + L] + k > kQCap) {
+        u32 base = 0;
+        if (lane_id() == 0) base = atomicAdd(q_counter(a, q, kQ32 + L), q.n[kQ32 + L]);
+        base = __shfl(base, 0);
+        q_copy(buf, q.n[kQ32 + L], a.hq[L], base, a.n);
+        q.n[kQ32 + L] = 0;
+    }
+    if (want) buf[q.n[kQ32 + L] + mask_rank(m)] = val;
+    q.n[kQ32 + L] += k;
+}
+
+// End of a phase, called by every thread of the block: one atomic per list
+// with staged entries, then each wave writes its runs.
+__device__ __forceinline__ void q_flush(const SsspBatchArgs &a, WaveQ &q) {
+    constexpr int kL = kQ32 + kQ64;
+    volatile u32 *cnt = &q.L->cnt[0][0];
+    const int nw = (int)(blockDim.x / kWave);
+    if (lane_id() == 0) {
+#pragma unroll
+        for (int l = 0; l < kL; ++l) cnt[q.w * kL + l] = q.n[l];
+    }
+    __syncthreads();
+    if (threadIdx.x < (u32)kL) {
+        const int l = (int)threadIdx.x;
+        u32 tot = 0;
+        for (int i = 0; i < nw; ++i) {
+            const u32 c = cnt[i * kL + l];
+            cnt[i * kL + l] = tot;
+            tot += c;
+        }
+        u32 b = 0;
+        if (tot) b = atomicAdd(q_counter(a, q, l), tot);
+        for (int i = 0; i < nw; ++i) cnt[i * kL + l] += b;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < kQ32; ++l) {
+        if (q.n[l]) q_copy((volatile u32 *)q.L->v32[q.w][l], q.n[l], q_dst32(a, q, l), cnt[q.w * kL + l], a.n);
+        q.n[l] = 0;
+    }
+#pragma unroll
+    for (int l = 0; l < kQ64; ++l) {
+        if (q.n[kQ32 + l])
+            q_copy((volatile u64 *)q.L->v64[q.w][l], q.n[kQ32 + l], a.hq[l], cnt[q.w * kL + kQ32 + l], a.n);
+        q.n[kQ32 + l] = 0;
+    }
+    __syncthreads();  // the LDS buffers and counts are reused by the next phase
+}
+
+__device__ __forceinline__ void push_item(const SsspBatchArgs &a, u32 v, u32 mask, const BTarget &t, WaveQ &wq) {
     const u64 wm = ballot(mask != 0);
     if (!wm) return;
     SsspBatchCtl *ctl = a.ctl;
-    if (lane_id() == (u32)(__ffsll((unsigned long long)wm) - 1)) atomicAdd(&ctl->nnext[t.slot], (u32)__popcll(wm));
+    // the flag only tells that the next bucket is not empty: set it once
+    if (lane_id() == (u32)(__ffsll((unsigned long long)wm) - 1) && !ld_relaxed(&ctl->nnext[t.slot]))
+        st_relaxed(&ctl->nnext[t.slot], 1u);
     // a vertex without light entries has nothing to do in light steps
     bool q = false;
     u32 lw = 0;
@@ -115,13 +271,9 @@ __device__ __forceinline__ void push_item(const SsspBatchArgs &a, u32 v, u32 mas
         if (lw) q = atomicOr(&a.pend[t.buf][v], mask) == 0;
     }
     if (!ballot(q)) return;
-#pragma unroll
-    for (int tier = 0; tier < 2; ++tier) {
-        const bool qt = q && btier(lw) == tier;
-        const u32 s = wave_append(&ctl->flen[t.slot][tier], qt);
-        if (qt) a.fq[t.buf][tier][s] = v;
-    }
-    hub_append(&ctl->fhub[t.slot], a.fq[t.buf][2], a.fq_off[t.buf], a.fq_len[t.buf], q && btier(lw) == 2, v, lw,
+    q_push32<kQLight0>(a, wq, q && btier(lw) == 0, v);
+    q_push32<kQLight0 + 1>(a, wq, q && btier(lw) == 1, v);
+    hub_append(&ctl->fhub[t.slot].v, a.fq[t.buf][2], a.fq_off[t.buf], a.fq_len[t.buf], q && btier(lw) == 2, v, lw,
                a.n, &ctl->bar.error);
 }
 
@@ -130,10 +282,15 @@ __device__ __forceinline__ void push_item(const SsspBatchArgs &a, u32 v, u32 mas
 __device__ __forceinline__ u32 relax_pair(const SsspBatchArgs &a, u32 u, int r, u64 k, const BTarget &t, u32 &v,
                                           u64 &improved) {
     v = a.col[k];
-    const f32 cand = label_dist(a.lab[(u64)u * kBatchRoots + r]) + a.w[k];
-    bool first = false;
-    if (!relax_first(&a.lab[(u64)v * kBatchRoots + r], cand, u, first)) return 0;
-    if (first) {  // v reached from root r: it has open work again
+    const f32 cand = __uint_as_float(a.dl[(u64)u * kBatchRoots + r]) + a.w[k];
+    const u32 cb = __float_as_uint(cand);
+    u32 *p = &a.dl[(u64)v * kBatchRoots + r];
+    if (*p <= cb) return 0;  // a plain read may be stale, i.e. larger: it only lets an atomic through
+    const u32 prev = atomicMin(p, cb);
+    if (prev <= cb) return 0;
+    atomicMin((unsigned long long *)&a.lab[(u64)v * kBatchRoots + r], ((unsigned long long)cb << 32) | u);
+    atomicMin(&a.vmin[v], cb);
+    if (prev >= kInfBits) {  // v reached from root r: it has open work again
         atomicOr(&a.rmask[v], 1u << r);
         atomicOr((unsigned long long *)&a.openw[v >> 6], 1ull << (v & 63));
     }
@@ -142,17 +299,38 @@ __device__ __forceinline__ u32 relax_pair(const SsspBatchArgs &a, u32 u, int r, 
     return cand < t.below ? 1u << r : 0u;
 }
 
+// Work counts (items, entries, pairs) of one segment kind, kept per block in
+// LDS and summed into the control block at the end.
+__device__ __forceinline__ void work_add(u64 *wk, u64 items, u64 entries, u64 pairs) {
+    atomicAdd((unsigned long long *)&wk[0], (unsigned long long)items);
+    atomicAdd((unsigned long long *)&wk[1], (unsigned long long)entries);
+    atomicAdd((unsigned long long *)&wk[2], (unsigned long long)pairs);
+}
+
 // Index of the i-th set bit of M (M has more than i bits).
 __device__ __forceinline__ int nth_root(u32 M, u32 i) {
     for (; i; --i) M &= M - 1;
     return __ffs(M) - 1;
 }
 
-template <bool LIGHT>
+// Row segments (rows are sorted by weight): light w < delta, medium
+// delta <= w < W, long w >= W. Light and long items take their root mask
+// from a pending array (cleared when read); medium items carry it.
+enum : int { kSegLight = 0, kSegMedium = 1, kSegLong = 2 };
+
+template <int K>
 __device__ __forceinline__ void bsegment(const SsspBatchArgs &a, u32 u, u64 &b, u64 &e) {
-    const u64 r0 = a.rs[u], s = a.split[u];
-    b = LIGHT ? r0 : r0 + s;
-    e = LIGHT ? r0 + s : a.rs[u + 1];
+    const u64 r0 = a.rs[u];
+    if (K == kSegLight) {
+        b = r0;
+        e = r0 + a.split[u];
+    } else if (K == kSegMedium) {
+        b = r0 + a.split[u];
+        e = r0 + a.lsplit[u];
+    } else {
+        b = r0 + a.lsplit[u];
+        e = a.rs[u + 1];
+    }
 }
 
 // The work of an item (u, M) over a segment is its (entry, root) pairs,
@@ -161,21 +339,20 @@ __device__ __forceinline__ void bsegment(const SsspBatchArgs &a, u32 u, u64 &b, 
 // 128-byte line of the target.
 
 // Tier 0 (segments <= 32 entries): a wave takes 64 items and walks their
-// concatenated pair ranges 64 pairs at a time. Light items read their mask
-// from pend and clear it; heavy items carry it in the list entry.
-template <bool LIGHT>
-__device__ void group_tier(const SsspBatchArgs &a, int buf, const u32 *lq, const u64 *hq, u32 count,
-                           const BTarget &t, u64 &improved, volatile int *slot) {
+// concatenated pair ranges 64 pairs at a time. Light and long items read their
+// mask from pend and clear it; medium items carry it in the list entry.
+template <int K>
+__device__ __forceinline__ void group_tier(const SsspBatchArgs &a, u32 *pend, const u32 *lq, const u64 *hq, u32 count,
+                           const BTarget &t, u64 &improved, volatile int *slot, u64 *wk, WaveQ &wq) {
     const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
     const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
-    u32 *pend = LIGHT ? a.pend[buf] : nullptr;
     for (u64 i0 = wave_gid * kWave; i0 < count; i0 += nwaves * kWave) {
         const u64 i = i0 + lane;
         u32 u = 0, M = 0;
         u64 b = 0, e = 0;
         if (i < count) {
-            if (LIGHT) {
+            if (K != kSegMedium) {
                 u = lq[i];
                 if (u < a.n) {
                     M = pend[u];
@@ -192,11 +369,15 @@ __device__ void group_tier(const SsspBatchArgs &a, int buf, const u32 *lq, const
                     M = 0;
                 }
             }
-            if (M) bsegment<LIGHT>(a, u, b, e);
+            if (M) bsegment<K>(a, u, b, e);
         }
         const u32 pairs = (u32)(e - b) * (u32)__popc(M);
         const u32 incl = wave_inclusive_scan(pairs);
         const u32 off = incl - pairs, total = __shfl(incl, kWave - 1);
+        {
+            const u64 nitems = (u64)__popcll(ballot(M != 0)), nent = wave_sum((u64)(e - b));
+            if (lane == 0) work_add(wk, nitems, nent, total);
+        }
         for (u32 c = 0; c < total; c += kWave) {
             bool inter;
             int last;
@@ -211,7 +392,7 @@ __device__ void group_tier(const SsspBatchArgs &a, int buf, const u32 *lq, const
                 const u32 entry = q / k;
                 out = relax_pair(a, uj, nth_root(Mj, q - entry * k), bj + entry, t, v, improved);
             }
-            push_item(a, v, out, t);
+            push_item(a, v, out, t, wq);
         }
     }
 }
@@ -234,22 +415,22 @@ __device__ __forceinline__ PairLanes pair_lanes(u32 M) {
 }
 
 // Tier 1 (segments <= 1024 entries): a wave per item.
-template <bool LIGHT>
-__device__ void wave_tier(const SsspBatchArgs &a, int buf, const u32 *lq, const u64 *hq, u32 count, const BTarget &t,
-                          u64 &improved) {
+template <int K>
+__device__ __forceinline__ void wave_tier(const SsspBatchArgs &a, u32 *pend, const u32 *lq, const u64 *hq, u32 count,
+                          const BTarget &t, u64 &improved, u64 *wk, WaveQ &wq) {
     const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
     const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
     for (u64 i = wave_gid; i < count; i += nwaves) {
         u32 u, M;
-        if (LIGHT) {
+        if (K != kSegMedium) {
             u = lq[i];
             if (u >= a.n) {
                 if (lane == 0) atomicOr(&a.ctl->bar.error, 4u);
                 continue;
             }
-            M = __builtin_amdgcn_readfirstlane(a.pend[buf][u]);
-            if (lane == 0) a.pend[buf][u] = 0;
+            M = __builtin_amdgcn_readfirstlane(pend[u]);
+            if (lane == 0) pend[u] = 0;
         } else {
             const u64 it = hq[i];
             u = (u32)it;
@@ -261,22 +442,24 @@ __device__ void wave_tier(const SsspBatchArgs &a, int buf, const u32 *lq, const 
         }
         if (!M) continue;
         u64 b, e;
-        bsegment<LIGHT>(a, u, b, e);
+        bsegment<K>(a, u, b, e);
+        if (lane == 0) work_add(wk, 1, e - b, (e - b) * (u64)__popc(M));
         const PairLanes p = pair_lanes(M);
         for (u64 e0 = b; e0 < e; e0 += p.per) {
             const u64 k = e0 + p.eoff;
             u32 v = 0, out = 0;
             if (p.active && k < e) out = relax_pair(a, u, p.r, k, t, v, improved);
-            push_item(a, v, out, t);
+            push_item(a, v, out, t, wq);
         }
     }
 }
 
-// Hub tier: chunks of the flattened work space. Light hubs read their mask
-// from pend (cleared by the item's last chunk); heavy hubs from `aux`.
-template <bool LIGHT>
-__device__ void hub_tier(const SsspBatchArgs &a, const u32 *hv, const u64 *hoff, const u32 *hlen, u32 *aux,
-                         u32 *pend, u64 packed, const BTarget &t, u64 &improved) {
+// Hub tier: chunks of the flattened work space. Light and long hubs read their
+// mask from pend (cleared by the item's last chunk; `aux` counts finished
+// chunks); medium hubs from `aux`.
+template <int K>
+__device__ __forceinline__ void hub_tier(const SsspBatchArgs &a, const u32 *hv, const u64 *hoff, const u32 *hlen, u32 *aux,
+                         u32 *pend, u64 packed, const BTarget &t, u64 &improved, u64 *wk, WaveQ &wq) {
     const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
     const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
@@ -290,21 +473,23 @@ __device__ void hub_tier(const SsspBatchArgs &a, const u32 *hv, const u64 *hoff,
             if (lane == 0) atomicOr(&a.ctl->bar.error, 4u);
             continue;
         }
-        const u32 M = __builtin_amdgcn_readfirstlane(LIGHT ? pend[u] : aux[i]);
+        const u32 M = __builtin_amdgcn_readfirstlane(K != kSegMedium ? pend[u] : aux[i]);
         const u64 local = p0 - hoff[i];
         const u32 len = hlen[i];
         u64 b, e;
-        bsegment<LIGHT>(a, u, b, e);
+        bsegment<K>(a, u, b, e);
         if (M) {
+            const u64 inchunk = min((u64)kHubChunk, (u64)len - min((u64)len, local));
+            if (lane == 0) work_add(wk, local == 0 ? 1 : 0, inchunk, inchunk * (u64)__popc(M));
             const PairLanes p = pair_lanes(M);
             for (u32 j = 0; j < (u32)kHubChunk; j += p.per) {
                 const u64 off = local + j + p.eoff;
                 u32 v = 0, out = 0;
                 if (p.active && j + p.eoff < (u32)kHubChunk && off < len) out = relax_pair(a, u, p.r, b + off, t, v, improved);
-                push_item(a, v, out, t);
+                push_item(a, v, out, t, wq);
             }
         }
-        if (LIGHT && lane == 0) {
+        if (K != kSegMedium && lane == 0) {
             const u32 chunks = (len + (u32)kHubChunk - 1) / (u32)kHubChunk;
             if (atomicAdd(&aux[i], 1u) + 1 == chunks) {  // every chunk has read the mask
                 pend[u] = 0;
@@ -323,27 +508,54 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
     const u64 nwords = (a.n + 63) / 64;
     const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
     u64 improved = 0;
+    __shared__ u64 wks[4][3];  // work counts per segment kind and of the dense scans, this block
+    if (threadIdx.x < 12) wks[threadIdx.x / 3][threadIdx.x % 3] = 0;
+    __shared__ QLds qlds;
+    WaveQ wq;
+    wq.L = &qlds;
+    wq.w = (int)(threadIdx.x / kWave);
+#pragma unroll
+    for (int l = 0; l < kQ32 + kQ64; ++l) wq.n[l] = 0;
+    wq.buf = wq.slot = wq.hs = wq.ls = 0;
+    // the light queue tiers that a phase's pushes go to
+    auto light_dest = [&](const BTarget &t) {
+        wq.buf = t.buf;
+        wq.slot = t.slot;
+    };
+    __syncthreads();
     u32 x = 0;
     f32 lo = 0.0f, hi = next_bound(0.0f, a.delta);
+    // deferred long edges: flushes so far (selects the counter set), whether
+    // the queue holds work, and the floor of the first bucket since the flush
+    u32 nflush = 0;
+    bool lpending = false;
+    f32 epoch_lo = 0.0f;
 
     const bool timer = gtid == 0;
-    u64 t_prev = timer ? wall_clock64() : 0, t_start = t_prev, t_acc[3] = {0, 0, 0};
-    auto tick = [&](int phase) {
+    u64 t_prev = timer ? wall_clock64() : 0, t_start = t_prev, t_acc[5] = {0, 0, 0, 0, 0};
+    u32 nlog = 0;
+    auto tick = [&](int phase, u64 items = 0) {
         if (timer) {
             const u64 t = wall_clock64();
             t_acc[phase] += t - t_prev;
+            if (nlog < kBStepLog) {
+                st_relaxed(&ctl->log_kind[nlog], (u32)phase);
+                st_relaxed(&ctl->log_items[nlog], (u32)min(items, (u64)0xFFFFFFFFu));
+                st_relaxed(&ctl->log_ticks[nlog], t - t_prev);
+            }
+            ++nlog;
             t_prev = t;
         }
     };
     auto clear_ahead = [&](u32 step) {
         if (gtid == 0) {
             const int s = (int)((step + 2) % 3);
-            st_relaxed(&ctl->flen[s][0], 0u);
-            st_relaxed(&ctl->flen[s][1], 0u);
-            st_relaxed(&ctl->fhub[s], 0ull);
-            st_relaxed(&ctl->hlen[s][0], 0u);
-            st_relaxed(&ctl->hlen[s][1], 0u);
-            st_relaxed(&ctl->hhub[s], 0ull);
+            st_relaxed(&ctl->flen[s][0].v, 0u);
+            st_relaxed(&ctl->flen[s][1].v, 0u);
+            st_relaxed(&ctl->fhub[s].v, 0ull);
+            st_relaxed(&ctl->hlen[s][0].v, 0u);
+            st_relaxed(&ctl->hlen[s][1].v, 0u);
+            st_relaxed(&ctl->hhub[s].v, 0ull);
             st_relaxed(&ctl->nmin[s], 0xFFFFFFFFu);
             st_relaxed(&ctl->nnext[s], 0u);
         }
@@ -352,7 +564,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
         u64 total = 0;
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
-            u32 c = ld_relaxed(&ctl->flen[step % 3][t]);
+            u32 c = ld_relaxed(&ctl->flen[step % 3][t].v);
             if (c > a.n) {
                 if (gtid == 0) atomicOr(&ctl->bar.error, 8u);
                 c = 0;
@@ -360,7 +572,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
             cnt[t] = c;
             total += c;
         }
-        *hub = ld_relaxed(&ctl->fhub[step % 3]);
+        *hub = ld_relaxed(&ctl->fhub[step % 3].v);
         return total + hub_count(*hub);
     };
     auto target = [&](u32 step, f32 below, f32 floor) {
@@ -368,10 +580,14 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
     };
     // Word-parallel pass over the vertices with open work: each lane reads one
     // word of openw, the wave walks the words that have open vertices, one
-    // lane per vertex (its labels are adjacent). f(word, v, A, B, dmin)
-    // gets the roots whose open label is in [x, y) (A) and in [y, z) (B), and
-    // the smallest open distance bits >= y.
+    // lane per vertex. vmin[v], the smallest open probe of v (exact between
+    // phases), screens the vertex: only one with vmin < z reads its probe line,
+    // in the 16-byte pieces that hold a live root. f(word, v, A, B, dmin, act)
+    // gets the roots whose open probe is in [x, y) (A) and in [y, z) (B), the
+    // smallest open probe bits >= y (vmin itself for a screened-out open
+    // vertex, all ones for a closed one), and whether the line was read.
     auto scan_open = [&](f32 x, f32 y, f32 z, auto &&f) {
+        const u32 zb = __float_as_uint(z);
         for (u64 w0 = wave_gid * kWave; w0 < nwords; w0 += nwaves * kWave) {
             const u64 wl = w0 + lane;
             const u64 open_l = wl < nwords ? a.openw[wl] : 0ull;
@@ -382,18 +598,30 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                 const u64 word = w0 + (u64)j;
                 const u64 v = word * 64 + lane;
                 const bool isopen = (shfl64(open_l, j) >> lane) & 1;
-                const u32 live = isopen ? a.rmask[v] & ~a.smask[v] : 0u;
-                u32 A = 0, B = 0, dmin = 0xFFFFFFFFu;
-#pragma unroll 8
-                for (int r = 0; r < kBatchRoots; ++r) {
-                    if (!((live >> r) & 1)) continue;
-                    const u32 bits = (u32)(a.lab[v * kBatchRoots + r] >> 32);
-                    const f32 d = __uint_as_float(bits);
-                    if (d >= x && d < y) A |= 1u << r;
-                    if (d >= y && d < z) B |= 1u << r;
-                    if (d >= y) dmin = min(dmin, bits);
+                const u32 vm = isopen ? a.vmin[v] : 0xFFFFFFFFu;
+                const bool act = vm < zb;
+                const u32 live = act ? a.rmask[v] & ~a.smask[v] : 0u;
+                u32 A = 0, B = 0, dmin = act ? 0xFFFFFFFFu : vm;
+                if (live) {
+                    const uint4 *line = reinterpret_cast<const uint4 *>(a.dl + v * kBatchRoots);
+#pragma unroll 2
+                    for (int q = 0; q < kBatchRoots / 4; ++q) {
+                        const u32 lq = (live >> (4 * q)) & 0xFu;
+                        if (!lq) continue;
+                        const uint4 c = line[q];
+                        const u32 bits[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            if (!((lq >> i) & 1)) continue;
+                            const f32 d = __uint_as_float(bits[i]);
+                            const u32 bit = 1u << (4 * q + i);
+                            if (d >= x && d < y) A |= bit;
+                            if (d >= y && d < z) B |= bit;
+                            if (d >= y) dmin = min(dmin, bits[i]);
+                        }
+                    }
                 }
-                f(word, v, A, B, dmin);
+                f(word, v, A, B, dmin, act);
             }
         }
     };
@@ -401,16 +629,20 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
     if (gtid < (u64)a.nroots) {
         const u64 root = trav_id(a.perm, a.roots[gtid]);
         st_relaxed(&a.lab[root * kBatchRoots + gtid], root);  // distance 0.0f, parent = root
+        st_relaxed(&a.dl[root * kBatchRoots + gtid], 0u);
+        st_relaxed(&a.vmin[root], 0u);
         atomicOr(&a.rmask[root], 1u << gtid);
         atomicOr((unsigned long long *)&a.openw[root >> 6], 1ull << (root & 63));
     }
     if (gtid == 0)
         for (int i = 0; i < 3; ++i) st_relaxed(&ctl->nmin[i], 0xFFFFFFFFu);
     bsync(ctl, 1, 0, 0);
+    light_dest(target(0, 0.f, 0.f));
     if (gtid < kWave) {  // step 0 queues the roots (repeated roots share one item)
         const bool r = (i32)gtid < a.nroots;
-        push_item(a, r ? trav_id(a.perm, a.roots[gtid]) : 0u, r ? 1u << gtid : 0u, target(0, 0.f, 0.f));
+        push_item(a, r ? trav_id(a.perm, a.roots[gtid]) : 0u, r ? 1u << gtid : 0u, target(0, 0.f, 0.f), wq);
     }
+    q_flush(a, wq);
     bsync(ctl, 1, 1, 0);
     x = 1;
 
@@ -424,23 +656,36 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
             if (gtid == 0) atomicAdd((unsigned long long *)&ctl->light_iters, 1ull);
             const BTarget t = target(x, hi, lo);
             const int buf = (int)(x & 1);
-            group_tier<true>(a, buf, a.fq[buf][0], nullptr, cnt[0], t, improved, slot);
-            wave_tier<true>(a, buf, a.fq[buf][1], nullptr, cnt[1], t, improved);
-            hub_tier<true>(a, a.fq[buf][2], a.fq_off[buf], a.fq_len[buf], a.fq_done[buf], a.pend[buf], hub, t,
-                           improved);
+            light_dest(t);
+            group_tier<kSegLight>(a, a.pend[buf], a.fq[buf][0], nullptr, cnt[0], t, improved, slot, wks[0], wq);
+            wave_tier<kSegLight>(a, a.pend[buf], a.fq[buf][1], nullptr, cnt[1], t, improved, wks[0], wq);
+            hub_tier<kSegLight>(a, a.fq[buf][2], a.fq_off[buf], a.fq_len[buf], a.fq_done[buf], a.pend[buf], hub, t,
+                                improved, wks[0], wq);
+            q_flush(a, wq);
             bsync(ctl, 2, __float_as_uint(lo), x);
-            tick(0);
+            tick(0, total);
             ++x;
         }
         if (ld_relaxed(&ctl->bar.error)) break;
-        // dense step x: heavy items of [lo, hi), next bucket [hi, hi2) queued
+        // dense step x: medium items of [lo, hi), long edges deferred, next
+        // bucket [hi, hi2) queued
         if (gtid == 0) atomicAdd((unsigned long long *)&ctl->buckets, 1ull);
         const f32 hi2 = next_bound(hi, a.delta);
+        if (!lpending) epoch_lo = lo;
         {
             const BTarget t = target(x, hi2, hi);
             const int hs = (int)((x + 1) % 3);
-            scan_open(lo, hi, hi2, [&](u64 word, u64 v, u32 H, u32 N, u32 dmin) {
-                (void)dmin;
+            const int ls = (int)(nflush & 1);
+            light_dest(t);
+            wq.hs = hs;
+            wq.ls = ls;
+            scan_open(lo, hi, hi2, [&](u64 word, u64 v, u32 H, u32 N, u32 dmin, bool act) {
+                if (act) a.vmin[v] = dmin;  // the open roots left are those >= hi
+                {
+                    const u64 na = __popcll(ballot(act)), nh = __popcll(ballot(H != 0));
+                    if (lane == 0) work_add(wks[3], 1, na, nh);
+                }
+                if (!ballot(H | N)) return;
                 // the bucket's roots of v are final now; v closes when no reached root is open
                 bool closed = false;
                 if (H) {
@@ -450,28 +695,75 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                 }
                 const u64 cm = ballot(closed);
                 if (cm && lane == 0) atomicAnd((unsigned long long *)&a.openw[word], ~(unsigned long long)cm);
-                push_item(a, (u32)v, N, t);
+                push_item(a, (u32)v, N, t, wq);
                 u64 b = 0, e = 0;
-                if (H) bsegment<false>(a, (u32)v, b, e);
+                u32 llen = 0;
+                bool lq = false;
+                if (H) {
+                    bsegment<kSegMedium>(a, (u32)v, b, e);
+                    llen = (u32)(a.rs[v + 1] - e);
+                    if (llen) {  // the lane owns v in this pass: no atomic needed
+                        const u32 old = a.lpend[v];
+                        a.lpend[v] = old | H;
+                        lq = old == 0;
+                    }
+                }
+                q_push32<kQLong0>(a, wq, lq && btier(llen) == 0, (u32)v);
+                q_push32<kQLong0 + 1>(a, wq, lq && btier(llen) == 1, (u32)v);
+                hub_append(&ctl->lhub[ls].v, a.lq[2], a.lq_off, a.lq_len, lq && btier(llen) == 2, (u32)v, llen, a.n,
+                           &ctl->bar.error);
                 const u32 len = (u32)(e - b);
                 const bool item = H && len > 0;
-#pragma unroll
-                for (int tier = 0; tier < 2; ++tier) {
-                    const bool want = item && btier(len) == tier;
-                    const u32 sl = wave_append(&ctl->hlen[hs][tier], want);
-                    if (want) a.hq[tier][sl] = ((u64)H << 32) | (u32)v;
-                }
-                const u64 hslot = hub_append(&ctl->hhub[hs], a.hqh, a.hq_off, a.hq_len, item && btier(len) == 2,
+                q_push64<kQMed0>(a, wq, item && btier(len) == 0, ((u64)H << 32) | (u32)v);
+                q_push64<kQMed0 + 1>(a, wq, item && btier(len) == 1, ((u64)H << 32) | (u32)v);
+                const u64 hslot = hub_append(&ctl->hhub[hs].v, a.hqh, a.hq_off, a.hq_len, item && btier(len) == 2,
                                              (u32)v, len, a.n, &ctl->bar.error);
                 if (hslot != ~0ull) a.hq_mask[hslot] = H;
             });
+            q_flush(a, wq);
             bsync(ctl, 3, __float_as_uint(lo), x);
-            group_tier<false>(a, 0, nullptr, a.hq[0], min(ld_relaxed(&ctl->hlen[hs][0]), (u32)a.n), t, improved, slot);
-            wave_tier<false>(a, 0, nullptr, a.hq[1], min(ld_relaxed(&ctl->hlen[hs][1]), (u32)a.n), t, improved);
-            hub_tier<false>(a, a.hqh, a.hq_off, a.hq_len, a.hq_mask, nullptr, ld_relaxed(&ctl->hhub[hs]), t,
-                            improved);
-            bsync(ctl, 4, __float_as_uint(lo), x);
+            // The deferred queue is read between the two barriers of this
+            // step: the next appends to this set come after the second one.
+            u32 lc[2];
+            for (int tier = 0; tier < 2; ++tier) lc[tier] = min(ld_relaxed(&ctl->llen[ls][tier].v), (u32)a.n);
+            const u64 lh = ld_relaxed(&ctl->lhub[ls].v);
             tick(1);
+            group_tier<kSegMedium>(a, nullptr, nullptr, a.hq[0], min(ld_relaxed(&ctl->hlen[hs][0].v), (u32)a.n), t,
+                                   improved, slot, wks[1], wq);
+            wave_tier<kSegMedium>(a, nullptr, nullptr, a.hq[1], min(ld_relaxed(&ctl->hlen[hs][1].v), (u32)a.n), t,
+                                  improved, wks[1], wq);
+            hub_tier<kSegMedium>(a, a.hqh, a.hq_off, a.hq_len, a.hq_mask, nullptr, ld_relaxed(&ctl->hhub[hs].v), t,
+                                 improved, wks[1], wq);
+            q_flush(a, wq);
+            bsync(ctl, 4, __float_as_uint(lo), x);
+            tick(2, (u64)ld_relaxed(&ctl->hlen[hs][0].v) + ld_relaxed(&ctl->hlen[hs][1].v) +
+                        hub_count(ld_relaxed(&ctl->hhub[hs].v)));
+            // Flush the deferred queue when the next bucket reaches past
+            // fl(epoch_lo + W), where its relaxations may land, or when the
+            // next bucket looks empty (a min step must see every label).
+            lpending = lc[0] || lc[1] || hub_count(lh);
+            if (lpending) {
+                u32 qc[2];
+                u64 qh;
+                const bool next_empty =
+                    queue_size(x + 1, qc, &qh) == 0 && ld_relaxed(&ctl->nnext[(x + 1) % 3]) == 0;
+                if (hi2 > epoch_lo + a.wlong || next_empty) {
+                    if (gtid == 0) {  // the other set was last read by the previous flush
+                        st_relaxed(&ctl->llen[ls ^ 1][0].v, 0u);
+                        st_relaxed(&ctl->llen[ls ^ 1][1].v, 0u);
+                        st_relaxed(&ctl->lhub[ls ^ 1].v, 0ull);
+                        atomicAdd((unsigned long long *)&ctl->flushes, 1ull);
+                    }
+                    group_tier<kSegLong>(a, a.lpend, a.lq[0], nullptr, lc[0], t, improved, slot, wks[2], wq);
+                    wave_tier<kSegLong>(a, a.lpend, a.lq[1], nullptr, lc[1], t, improved, wks[2], wq);
+                    hub_tier<kSegLong>(a, a.lq[2], a.lq_off, a.lq_len, a.lq_done, a.lpend, lh, t, improved, wks[2], wq);
+                    q_flush(a, wq);
+                    bsync(ctl, 7, __float_as_uint(lo), x);
+                    tick(3, (u64)lc[0] + lc[1] + hub_count(lh));
+                    ++nflush;
+                    lpending = false;
+                }
+            }
             if (gtid == 0) atomicAdd((unsigned long long *)&ctl->scans, 1ull);
             ++x;
         }
@@ -490,12 +782,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
         clear_ahead(x);
         {
             u32 m = 0xFFFFFFFFu;
-            scan_open(hi, hi, hi, [&](u64 word, u64 v, u32 A, u32 B, u32 dmin) {
+            scan_open(hi, hi, hi, [&](u64 word, u64 v, u32 A, u32 B, u32 dmin, bool act) {
                 (void)word;
                 (void)v;
                 (void)A;
                 (void)B;
-                m = min(m, dmin);
+                (void)act;
+                if (dmin < kInfBits) m = min(m, dmin);
             });
             m = wave_min(m);
             if (lane == 0 && m != 0xFFFFFFFFu) atomicMin(&ctl->nmin[(x + 1) % 3], m);
@@ -505,7 +798,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
         const u32 nm = ld_relaxed(&ctl->nmin[x % 3]);
         if (gtid == 0) atomicAdd((unsigned long long *)&ctl->scans, 1ull);
         if (nm == 0xFFFFFFFFu) {
-            tick(2);
+            tick(4);
             break;
         }
         const f32 dmin = __uint_as_float(nm);
@@ -519,25 +812,35 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
         clear_ahead(x);
         {
             const BTarget t = target(x, hi, lo);
-            scan_open(lo, hi, hi, [&](u64 word, u64 v, u32 M, u32 B, u32 dmin) {
+            light_dest(t);
+            scan_open(lo, hi, hi, [&](u64 word, u64 v, u32 M, u32 B, u32 dmin, bool act) {
                 (void)word;
                 (void)B;
                 (void)dmin;
-                push_item(a, (u32)v, M, t);
+                (void)act;
+                push_item(a, (u32)v, M, t, wq);
             });
+            q_flush(a, wq);
         }
         bsync(ctl, 6, __float_as_uint(lo), x);
         if (gtid == 0) atomicAdd((unsigned long long *)&ctl->scans, 1ull);
-        tick(2);
+        tick(4);
         ++x;
     }
     improved = wave_sum(improved);
     if (lane == 0 && improved) atomicAdd((unsigned long long *)&ctl->improvements, (unsigned long long)improved);
+    __syncthreads();
+    if (threadIdx.x < 12 && wks[threadIdx.x / 3][threadIdx.x % 3])
+        atomicAdd((unsigned long long *)&ctl->work[threadIdx.x / 3][threadIdx.x % 3],
+                  (unsigned long long)wks[threadIdx.x / 3][threadIdx.x % 3]);
     if (timer) {
-        for (int i = 0; i < 3; ++i) st_relaxed(&ctl->ticks[i], t_acc[i]);
-        st_relaxed(&ctl->ticks[3], wall_clock64() - t_start);
+        st_relaxed(&ctl->nlog, nlog);
+        for (int i = 0; i < 5; ++i) st_relaxed(&ctl->ticks[i], t_acc[i]);
+        st_relaxed(&ctl->ticks[5], wall_clock64() - t_start);
     }
-    // export, root-major (traversal.cu:333-340): unreached -> -1.0f / -1
+    // export, root-major (traversal.cu:333-340): unreached -> -1.0f / -1. The
+    // probes are no longer read after the last barrier, so dist (their
+    // storage) can take the output.
     const u64 nlab = a.n * kBatchRoots;
     for (u64 tt = gtid; tt < nlab; tt += gthreads) {
         const u64 v = tt / kBatchRoots;
@@ -551,16 +854,24 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
     }
 }
 
-__global__ __launch_bounds__(256) void sssp_batch_clear_kernel(u64 *__restrict__ lab, u32 *__restrict__ pend,
-                                                               u32 *__restrict__ done, u32 *__restrict__ masks,
-                                                               u64 *__restrict__ openw, u64 n) {
+__global__ __launch_bounds__(256) void sssp_batch_clear_kernel(u64 *__restrict__ lab, u32 *__restrict__ dl,
+                                                               u32 *__restrict__ pend, u32 *__restrict__ done,
+                                                               u32 *__restrict__ masks, u32 *__restrict__ lpend,
+                                                               u32 *__restrict__ ldone, u64 *__restrict__ openw,
+                                                               u64 n) {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x; t < n * kBatchRoots; t += stride) {
         lab[t] = kInfLabel;
+        dl[t] = kInfBits;
         if (t < 2 * n) {
             pend[t] = 0;
             done[t] = 0;
             masks[t] = 0;
+        }
+        if (t < n) masks[2 * n + t] = 0xFFFFFFFFu;  // vmin
+        if (t < n) {
+            lpend[t] = 0;
+            ldone[t] = 0;
         }
         if (t < (n + 63) / 64) openw[t] = 0;
     }
@@ -616,6 +927,11 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
         fail("SSSP delta must be positive and finite (got %g)", (double)s->delta);
         return false;
     }
+    if (!(s->long_weight >= 0.0f) || !(s->long_weight < 3.0e38f)) {
+        fail("batched SSSP long_weight must be non-negative and finite (got %g)", (double)s->long_weight);
+        return false;
+    }
+    const f32 wlong = s->long_weight > s->delta ? s->long_weight : s->delta;
     const u64 n = g->nlocalverts;
     if (n > s->max_nlocalverts) {
         fail("batched SSSP state holds %llu vertices, graph has %llu", (unsigned long long)s->max_nlocalverts,
@@ -626,13 +942,16 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
     const hipStream_t st = library ? c.stream : stream;
     // the split depends on the graph too: a state reused across graphs recomputes it
     const TraversalView tv = traversal_view(g);
-    if (s->split_delta != s->delta || s->split_graph != (const void *)tv.rs) {
+    if (s->split_delta != s->delta || s->long_split_weight != wlong || s->split_graph != (const void *)tv.rs) {
         compute_split(tv.rs, tv.w, n, s->delta, s->split, st);
+        compute_split(tv.rs, tv.w, n, wlong, s->long_split, st);
         s->split_delta = s->delta;
+        s->long_split_weight = wlong;
         s->split_graph = (const void *)tv.rs;
     }
     sssp_batch_clear_kernel<<<grid_for(n * kBatchRoots, 256, (u64)c.num_cus * 64), 256, 0, st>>>(
-        s->labels, s->pending, s->hub_aux, s->root_masks, s->open_words, n);
+        s->labels, (u32 *)s->dist, s->pending, s->hub_aux, s->root_masks, s->long_work, s->long_work + 5 * n,
+        s->open_words, n);
     G500_HIP(hipGetLastError());
     G500_HIP(hipMemsetAsync(s->ctl, 0, sizeof(SsspBatchCtl), st));
     SsspBatchArgs a;
@@ -643,9 +962,17 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
     a.perm = tv.perm;
     a.iperm = tv.iperm;
     a.split = s->split;
+    a.lsplit = s->long_split;
+    a.lpend = s->long_work;
+    for (int t = 0; t < 3; ++t) a.lq[t] = s->long_work + (u64)(1 + t) * n;
+    a.lq_len = s->long_work + 4 * n;
+    a.lq_done = s->long_work + 5 * n;
+    a.lq_off = s->long_offsets;
     a.lab = s->labels;
+    a.dl = (u32 *)s->dist;
     a.rmask = s->root_masks;
     a.smask = s->root_masks + n;
+    a.vmin = s->root_masks + 2 * n;
     a.openw = s->open_words;
     for (int b = 0; b < 2; ++b) {
         a.pend[b] = s->pending + (u64)b * n;
@@ -666,6 +993,7 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
     for (i32 i = 0; i < nroots; ++i) a.roots[i] = roots[i];
     a.nroots = nroots;
     a.delta = s->delta;
+    a.wlong = wlong;
     const int block = ctx().sssp_block;
     if (grid <= 0) grid = cooperative_blocks((const void *)sssp_batch_kernel, block, 0);
     if (failed()) return false;
@@ -703,6 +1031,7 @@ Sssp_Batch_State oned_graph_sssp_batch_create(Worker_State *ws, const Oned_Graph
     const u64 n = g->nlocalverts;
     s.max_nlocalverts = n;
     s.delta = 0.1f;
+    s.long_weight = 0.2f;
     s.dist = (f32 *)arena_alloc(ws, kBatchRoots * n * sizeof(f32), "batched SSSP dist");
     s.pred = (i64 *)arena_alloc(ws, kBatchRoots * n * sizeof(i64), "batched SSSP pred");
     s.labels = (u64 *)arena_alloc(ws, kBatchRoots * n * sizeof(u64), "batched SSSP labels");
@@ -714,8 +1043,11 @@ Sssp_Batch_State oned_graph_sssp_batch_create(Worker_State *ws, const Oned_Graph
     s.hub_lengths = (u32 *)arena_alloc(ws, 3 * n * sizeof(u32), "batched SSSP hub lengths");
     s.hub_aux = (u32 *)arena_alloc(ws, 3 * n * sizeof(u32), "batched SSSP hub chunk counts / masks");
     s.split = (u32 *)arena_alloc(ws, n * sizeof(u32), "batched SSSP light split");
-    s.root_masks = (u32 *)arena_alloc(ws, 2 * n * sizeof(u32), "batched SSSP reached/settled root masks");
+    s.root_masks = (u32 *)arena_alloc(ws, 3 * n * sizeof(u32), "batched SSSP root masks and smallest open probes");
     s.open_words = (u64 *)arena_alloc(ws, ((n + 63) / 64) * sizeof(u64), "batched SSSP open-vertex bitmap");
+    s.long_split = (u32 *)arena_alloc(ws, n * sizeof(u32), "batched SSSP long split");
+    s.long_work = (u32 *)arena_alloc(ws, 6 * n * sizeof(u32), "batched SSSP deferred long-edge queue");
+    s.long_offsets = (u64 *)arena_alloc(ws, n * sizeof(u64), "batched SSSP deferred hub offsets");
     s.ctl = arena_alloc(ws, sizeof(SsspBatchCtl), "batched SSSP control");
     return s;
 }
@@ -739,8 +1071,35 @@ void g500_sssp_batch_phase_ms(const Sssp_Batch_State *s, f64 *ms) {
     static SsspBatchCtl h;
     g500_copy_to_host(&h, s->ctl, sizeof(h));
     const f64 khz = ctx().wall_clock_khz;
-    for (int i = 0; i < 4; ++i) ms[i] = (f64)h.ticks[i] / khz;
+    ms[0] = (f64)h.ticks[0] / khz;
+    ms[1] = (f64)(h.ticks[1] + h.ticks[2] + h.ticks[3]) / khz;
+    ms[2] = (f64)h.ticks[4] / khz;
+    ms[3] = (f64)h.ticks[5] / khz;
     ms[4] = (f64)h.scans;
+    ms[5] = (f64)h.flushes;
+    ms[6] = (f64)h.ticks[1] / khz;
+    ms[7] = (f64)h.ticks[2] / khz;
+    ms[8] = (f64)h.ticks[3] / khz;
+}
+
+i32 g500_sssp_batch_step_log(const Sssp_Batch_State *s, i32 *kind, u64 *items, f64 *ms, i32 max_steps) {
+    static SsspBatchCtl h;
+    g500_copy_to_host(&h, s->ctl, sizeof(h));
+    const f64 khz = ctx().wall_clock_khz;
+    i32 k = 0;
+    for (u32 i = 0; i < MIN(h.nlog, kBStepLog) && k < max_steps; ++i, ++k) {
+        if (kind) kind[k] = (i32)h.log_kind[i];
+        if (items) items[k] = h.log_items[i];
+        if (ms) ms[k] = (f64)h.log_ticks[i] / khz;
+    }
+    return k;
+}
+
+void g500_sssp_batch_work(const Sssp_Batch_State *s, u64 *out) {
+    static SsspBatchCtl h;
+    g500_copy_to_host(&h, s->ctl, sizeof(h));
+    for (int k = 0; k < 4; ++k)
+        for (int j = 0; j < 3; ++j) out[k * 3 + j] = h.work[k][j];
 }
 
 void g500_sssp_batch_run_stats(const Sssp_Batch_State *s, u64 *buckets, u64 *light_iterations, u64 *relaxations) {

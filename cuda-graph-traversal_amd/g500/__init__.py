@@ -64,8 +64,9 @@ class BfsBatchState(C.Structure):
 class SsspBatchState(C.Structure):
     _fields_ = [("dist", vp), ("pred", vp), ("labels", vp), ("pending", vp), ("frontier", vp), ("heavy", vp),
                 ("hub_vertices", vp), ("hub_offsets", vp), ("hub_lengths", vp), ("hub_aux", vp), ("split", vp),
-                ("root_masks", vp), ("open_words", vp), ("ctl", vp), ("split_graph", vp), ("delta", f32), ("split_delta", f32), ("nroots", i32),
-                ("reserved", i32),
+                ("root_masks", vp), ("open_words", vp), ("long_split", vp), ("long_work", vp), ("long_offsets", vp),
+                ("ctl", vp), ("split_graph", vp), ("delta", f32), ("split_delta", f32), ("long_weight", f32),
+                ("long_split_weight", f32), ("nroots", i32), ("reserved", i32),
                 ("max_nlocalverts", u64)]
 
 
@@ -127,6 +128,8 @@ _SIGS = {
     "oned_graph_sssp_batch_destroy": (None, [P(SsspBatchState)]),
     "g500_sssp_batch_run_stats": (None, [P(SsspBatchState), P(u64), P(u64), P(u64)]),
     "g500_sssp_batch_phase_ms": (None, [P(SsspBatchState), P(f64)]),
+    "g500_sssp_batch_work": (None, [P(SsspBatchState), P(u64)]),
+    "g500_sssp_batch_step_log": (i32, [P(SsspBatchState), P(i32), P(u64), P(f64), i32]),
     "g500_server_create": (vp, [P(WorkerState), i32, u64]),
     "g500_server_add_graph": (i32, [vp, P(OnedGraph)]),
     "g500_server_start": (i32, [vp, f32]),
@@ -363,14 +366,27 @@ class Session:
                 self.to_host(s.pred + 8 * n * i, n, np.int64))
 
     def sssp_batch_phase_ms(self, s):
-        out = (f64 * 5)()
+        out = (f64 * 9)()
         self.lib.g500_sssp_batch_phase_ms(C.byref(s), out)
         return list(out)
+
+    def sssp_batch_work(self, s):
+        out = (u64 * 12)()
+        self.lib.g500_sssp_batch_work(C.byref(s), out)
+        return {k: tuple(int(x) for x in out[3 * i:3 * i + 3])
+                for i, k in enumerate(("light", "medium", "long", "scan"))}
 
     def sssp_batch_stats(self, s):
         b, li, r = u64(), u64(), u64()
         self.lib.g500_sssp_batch_run_stats(C.byref(s), C.byref(b), C.byref(li), C.byref(r))
         return b.value, li.value, r.value
+
+    def sssp_batch_step_log(self, s, max_steps=4096):
+        kind = (i32 * max_steps)()
+        items = (u64 * max_steps)()
+        ms = (f64 * max_steps)()
+        k = self.lib.g500_sssp_batch_step_log(C.byref(s), kind, items, ms, max_steps)
+        return list(kind[:k]), list(items[:k]), list(ms[:k])
 
     def sssp_step_log(self, s, max_steps=1023):
         kind = (i32 * max_steps)()

@@ -74,14 +74,19 @@ i32 g500_bfs_batch_level_log(const Bfs_Batch_State *b, i32 *modes, u64 *records,
 
 /* ---- batched delta-stepping SSSP ---------------------------------------- */
 /* Up to 32 roots in one persistent kernel with one bucket sequence shared by
- * all roots; items are (vertex, root mask) pairs and the labels of a vertex
- * share two 128-byte lines. No counterpart in the reference (its runner solves
- * one root at a time, graph500_runner.cu:393-416); each root's distances equal
- * the single-root oned_graph_sssp_run's (the FP32 fixed point is unique), and
- * its parents satisfy the same certificate. */
+ * all roots; items are (vertex, root mask) pairs and the 32 distance probes of
+ * a vertex share one 128-byte line. Edges with w >= long_weight are relaxed in
+ * deferred batches: their relaxations land at least long_weight above the
+ * bucket that settled the source, so the kernel collects the roots a vertex
+ * settles over that window and relaxes its long edges once for all of them.
+ * No counterpart in the reference (its runner solves one root at a time,
+ * graph500_runner.cu:393-416); each root's distances equal the single-root
+ * oned_graph_sssp_run's (the FP32 fixed point is unique), and its parents
+ * satisfy the same certificate. */
 typedef struct Sssp_Batch_State Sssp_Batch_State;
 struct Sssp_Batch_State {
-    f32 *dist;          /* [32n] root-major: root index r, vertex v at r * n + v; -1.0f unreached */
+    f32 *dist;          /* [32n] root-major: root index r, vertex v at r * n + v; -1.0f unreached
+                           (vertex-major distance probes while a run is in flight) */
     i64 *pred;          /* [32n] root-major parents; -1 unreached; root -> root */
     u64 *labels;        /* [32n] vertex-major working labels (v * 32 + r) */
     u32 *pending;       /* [2n] pending root masks, two steps */
@@ -92,12 +97,19 @@ struct Sssp_Batch_State {
     u32 *hub_lengths;   /* [3n] hub segment lengths */
     u32 *hub_aux;       /* [3n] light: finished chunks per hub; heavy: root masks */
     u32 *split;         /* [n] light entries (w < split_delta) per row */
-    u32 *root_masks;    /* [2n] per vertex: roots with a finite label, roots whose label is final */
+    u32 *root_masks;    /* [3n] per vertex: roots with a finite label, roots whose label is final,
+                           smallest distance bits among the roots not yet final */
     u64 *open_words;    /* [ceil(n/64)] vertices with a reached root not yet final */
+    u32 *long_split;    /* [n] entries with w < max(long_weight, delta) per row */
+    u32 *long_work;     /* [6n] deferred long edges: pending root masks, queue tiers 0-2, hub lengths,
+                           hub chunk counts */
+    u64 *long_offsets;  /* [n] hub work offsets of the deferred queue */
     void *ctl;          /* device control block */
-    const void *split_graph; /* row offsets that `split` was computed for */
+    const void *split_graph; /* row offsets that `split` / `long_split` were computed for */
     f32 delta;          /* bucket width (default 0.1f) */
     f32 split_delta;
+    f32 long_weight;    /* deferral threshold (default 0.2f; <= delta: no deferral window) */
+    f32 long_split_weight;
     i32 nroots;         /* roots of the last run */
     i32 reserved;
     u64 max_nlocalverts;
@@ -111,9 +123,21 @@ void oned_graph_sssp_batch_destroy(Sssp_Batch_State *s);
  * (improvements summed over the roots). */
 void g500_sssp_batch_run_stats(const Sssp_Batch_State *s, u64 *buckets, u64 *light_iterations,
                                u64 *relaxations);
-/* Device milliseconds of the last batched run: light steps, dense + heavy
- * steps, min + collect steps, whole kernel; ms[4] = dense passes. */
-void g500_sssp_batch_phase_ms(const Sssp_Batch_State *s, f64 ms[5]);
+/* Device milliseconds of the last batched run: light steps, dense steps
+ * (scan + medium + flush), min + collect steps, whole kernel; ms[4] = dense
+ * passes, ms[5] = deferred-queue flushes; ms[6..8] = the dense steps' scan,
+ * medium and flush parts. */
+void g500_sssp_batch_phase_ms(const Sssp_Batch_State *s, f64 ms[9]);
+/* Work of the last batched run per row segment k (0 light, 1 medium, 2 long):
+ * out[3k + 0] items, out[3k + 1] row entries visited, out[3k + 2]
+ * (entry, root) relaxations; out[9..11] of the dense scans: open 64-vertex
+ * words walked, vertices that read their probe line, vertices settling a
+ * root. */
+void g500_sssp_batch_work(const Sssp_Batch_State *s, u64 out[12]);
+/* Per-phase log of the last batched run (first 4096 phases): kind (0 light
+ * step, 1 dense scan, 2 medium relaxations, 3 deferred flush, 4 min + collect
+ * steps), items processed and device milliseconds. Returns the entries. */
+i32 g500_sssp_batch_step_log(const Sssp_Batch_State *s, i32 *kind, u64 *items, f64 *ms, i32 max_steps);
 
 /* ---- serving loop -------------------------------------------------------- */
 /* Resident graphs, (graph, root, BFS | SSSP) requests and continuous batching.

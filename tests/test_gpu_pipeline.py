@@ -373,12 +373,14 @@ def test_sssp_batch_bitexact(arena, oracle, scale):
     s = arena.sssp_batch_create(g)
     roots = oracle.roots(ref)
     width = int(arena.get_param("sssp_batch_max"))
-    for delta in (0.1, 0.25, 0.02):
-        s.delta = delta
-        say("sssp batch", scale, delta)
+    # (delta, long_weight): default deferral window, W below delta (no window),
+    # narrow and wide windows, no deferral at all (W = 0)
+    for delta, wl in ((0.1, 0.2), (0.25, 0.2), (0.02, 0.2), (0.02, 0.05), (0.02, 0.0), (0.02, 0.7)):
+        s.delta, s.long_weight = delta, wl
+        say("sssp batch", scale, delta, wl)
         sssp_batch_check(arena, oracle, g, ref, s, roots[:width])
     # fewer roots, repeats, an isolated vertex as a root
-    s.delta = 0.1
+    s.delta, s.long_weight = 0.1, 0.2
     isolated = int(np.flatnonzero(np.diff(ref.rowstarts.astype(np.int64)) == 0)[0])
     some = np.array([roots[20], roots[20], isolated, roots[17]], np.uint64)
     say("sssp batch repeats", scale, some.tolist())
@@ -392,13 +394,20 @@ def test_sssp_batch_matches_plain(arena, oracle):
     sb = arena.sssp_batch_create(g)
     sp = arena.sssp_create(g)
     roots = arena.roots(g)[:16]
-    arena.sssp_batch(g, sb, roots)
-    for i, r in enumerate(roots):
+    plain = []
+    for r in roots:
         arena.sssp(g, sp, int(r))
-        d1, _ = arena.sssp_result(sp)
-        d, p = arena.sssp_batch_result(g, sb, i)
-        assert np.array_equal(d.view(np.uint32), d1.view(np.uint32)), i
-        assert oracle.sssp_check(ref, int(r), d, p) == 0
+        plain.append(arena.sssp_result(sp)[0])
+    for delta, wl in ((0.1, 0.2), (0.0125, 0.25)):
+        sb.delta, sb.long_weight = delta, wl
+        arena.sssp_batch(g, sb, roots)
+        flushes = arena.sssp_batch_phase_ms(sb)[5]
+        say("sssp batch S18", delta, wl, "flushes", flushes)
+        assert flushes > 0
+        for i, r in enumerate(roots):
+            d, p = arena.sssp_batch_result(g, sb, i)
+            assert np.array_equal(d.view(np.uint32), plain[i].view(np.uint32)), (delta, wl, i)
+            assert oracle.sssp_check(ref, int(r), d, p) == 0
 
 
 def test_sssp_batch_errors_return(arena):
@@ -410,6 +419,10 @@ def test_sssp_batch_errors_return(arena):
     for roots in ([], list(range(width + 1)), [1 << 10]):
         with pytest.raises(g500.LibraryError):
             arena.sssp_batch(g, s, np.array(roots, np.uint64))
+    for wl in (-1.0, float("nan"), float("inf")):
+        s.long_weight = wl
+        with pytest.raises(g500.LibraryError):
+            arena.sssp_batch(g, s, np.array([1], np.uint64))
 
 
 # ------------------------------------------------------------- knobs / tuner
