@@ -55,13 +55,14 @@ constexpr u32 kStepLog = 1024;
 // range, 8 queue length above n, 16 bucket invariant violated.
 struct SsspCtl {
     GridBarrier bar;
-    alignas(256) u32 flen[3][2];   // light queue lengths [slot][tier 0-1]
-    u64 fhub[3];                   // light queue hub counters [slot]
-    u32 hlen[3];                   // heavy wave-tier list length [slot]
-    u64 hhub[3];                   // heavy hub-tier counters [slot]
-    u32 nmin[3];                   // smallest distance bits >= hi, per slot
-    u32 nnext[3];                  // labels that entered [hi, hi + delta), per slot (nonzero: bucket not empty)
-    alignas(64) u64 buckets, light_iters, improvements, scans;
+    // every counter that waves append to has a 256-byte line of its own
+    CounterLine<u32> flen[3][2];   // light queue lengths [slot][tier 0-1]
+    CounterLine<u64> fhub[3];      // light queue hub counters [slot]
+    CounterLine<u32> hlen[3];      // heavy wave-tier list length [slot]
+    CounterLine<u64> hhub[3];      // heavy hub-tier counters [slot]
+    alignas(256) u32 nmin[3];      // smallest distance bits >= hi, per slot
+    u32 nnext[3];                  // 1: a label entered [hi, hi + delta) (the bucket is not empty), per slot
+    alignas(256) u64 buckets, light_iters, improvements, scans;
     u64 ticks[4];                  // constant-clock ticks of block 0: light, fused, min+collect, kernel
     // per-step log (block 0): kind (0 light, 1 fused, 2 min, 3 collect), queue
     // entries of tiers 0/1, packed hub counter, constant-clock ticks
@@ -121,13 +122,93 @@ struct Target {
     f32 floor;   // new distances below this would break the bucket invariant
 };
 
+// Wave-staged appends to the light queue tiers 0-1 and the heavy wave-tier
+// list: a wave buffers its entries in LDS and writes a run with one atomic
+// when the buffer is full; at the end of a phase the block sums its waves'
+// counts and makes one atomic per list. The counters then take one atomic per
+// block and phase instead of one per wave and relaxation round.
+constexpr u32 kSCap = 64;
+constexpr int kSLists = 3;  // light tier 0, light tier 1, heavy wave tier
+struct StageLds {
+    u32 v[1024 / kWave][kSLists][kSCap];
+    u32 cnt[1024 / kWave][kSLists];
+};
+struct Stage {
+    StageLds *L;
+    int w;            // wave of the block
+    u32 n[kSLists];   // staged entries per list
+    int buf, slot;    // light queue buffer and counter slot of the phase's pushes
+    int hs;           // heavy list slot of a fused step
+};
+
+__device__ __forceinline__ u32 *stage_dst(const SsspArgs &a, const Stage &q, int l) {
+    if (l == 2) return a.hq1;
+    return q.buf ? a.fq[1][l] : a.fq[0][l];  // a select: a runtime index would copy the arguments to scratch
+}
+__device__ __forceinline__ u32 *stage_counter(const SsspArgs &a, const Stage &q, int l) {
+    return l == 2 ? &a.ctl->hlen[q.hs].v : &a.ctl->flen[q.slot][l].v;
+}
+__device__ __forceinline__ void stage_copy(const volatile u32 *buf, u32 cnt, u32 *dst, u32 base, u64 cap) {
+    for (u32 i = lane_id(); i < cnt; i += kWave) {
+        const u64 o = (u64)base + i;
+        if (o < cap) dst[o] = buf[i];
+    }
+}
+template <int L>
+__device__ __forceinline__ void stage_push(const SsspArgs &a, Stage &q, bool want, u32 val) {
+    const u64 m = ballot(want);
+    if (!m) return;
+    const u32 k = (u32)__popcll(m);
+    volatile u32 *buf = q.L->v[q.w][L];
+    if (q.n[L] + k > kSCap) {
+        u32 base = 0;
+        if (lane_id() == 0) base = atomicAdd(stage_counter(a, q, L), q.n[L]);
+        base = __shfl(base, 0);
+        stage_copy(buf, q.n[L], stage_dst(a, q, L), base, a.n);
+        q.n[L] = 0;
+    }
+    if (want) buf[q.n[L] + mask_rank(m)] = val;
+    q.n[L] += k;
+}
+// End of a phase, called by every thread of the block.
+__device__ __forceinline__ void stage_flush(const SsspArgs &a, Stage &q) {
+    volatile u32 *cnt = &q.L->cnt[0][0];
+    const int nw = (int)(blockDim.x / kWave);
+    if (lane_id() == 0) {
+#pragma unroll
+        for (int l = 0; l < kSLists; ++l) cnt[q.w * kSLists + l] = q.n[l];
+    }
+    __syncthreads();
+    if (threadIdx.x < (u32)kSLists) {
+        const int l = (int)threadIdx.x;
+        u32 tot = 0;
+        for (int i = 0; i < nw; ++i) {
+            const u32 c = cnt[i * kSLists + l];
+            cnt[i * kSLists + l] = tot;
+            tot += c;
+        }
+        u32 b = 0;
+        if (tot) b = atomicAdd(stage_counter(a, q, l), tot);
+        for (int i = 0; i < nw; ++i) cnt[i * kSLists + l] += b;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < kSLists; ++l) {
+        if (q.n[l]) stage_copy((volatile u32 *)q.L->v[q.w][l], q.n[l], stage_dst(a, q, l), cnt[q.w * kSLists + l], a.n);
+        q.n[l] = 0;
+    }
+    __syncthreads();
+}
+
 // Wave-converged: queues v (lanes with `want`) into the light queue of `t`,
 // once per queue (stamp tag), in the work tier of its light segment.
-__device__ __forceinline__ void push_light(const SsspArgs &a, bool want, u32 v, const Target &t) {
+__device__ __forceinline__ void push_light(const SsspArgs &a, bool want, u32 v, const Target &t, Stage &sq) {
     const u64 wm = ballot(want);
     if (!wm) return;
     SsspCtl *ctl = a.ctl;
-    if (lane_id() == (u32)(__ffsll((unsigned long long)wm) - 1)) atomicAdd(&ctl->nnext[t.slot], (u32)__popcll(wm));
+    // the flag only tells that the next bucket is not empty: set it once
+    if (lane_id() == (u32)(__ffsll((unsigned long long)wm) - 1) && !ld_relaxed(&ctl->nnext[t.slot]))
+        st_relaxed(&ctl->nnext[t.slot], 1u);
     // a vertex without light entries has nothing to do in light steps
     bool q = false;
     u32 lw = 0;
@@ -135,13 +216,10 @@ __device__ __forceinline__ void push_light(const SsspArgs &a, bool want, u32 v, 
         lw = a.split[v];
         if (lw) q = atomicExch(&a.stamp[v], t.tag) != t.tag;
     }
-#pragma unroll
-    for (int tier = 0; tier < 2; ++tier) {
-        const bool qt = q && tier_of(lw) == tier;
-        const u32 s = wave_append(&ctl->flen[t.slot][tier], qt);
-        if (qt) a.fq[t.buf][tier][s] = v;
-    }
-    hub_append(&ctl->fhub[t.slot], a.fq[t.buf][2], a.fq_off[t.buf], a.fq_len[t.buf], q && tier_of(lw) == 2, v, lw,
+    if (!ballot(q)) return;
+    stage_push<0>(a, sq, q && tier_of(lw) == 0, v);
+    stage_push<1>(a, sq, q && tier_of(lw) == 1, v);
+    hub_append(&ctl->fhub[t.slot].v, a.fq[t.buf][2], a.fq_off[t.buf], a.fq_len[t.buf], q && tier_of(lw) == 2, v, lw,
                a.n, &ctl->bar.error);
 }
 
@@ -152,7 +230,7 @@ constexpr int kQ = 2;
 
 __device__ __forceinline__ void visit_edges(const SsspArgs &a, const Target &t, const bool (&act)[kQ],
                                             const u32 (&u)[kQ], const f32 (&du)[kQ], const u64 (&k)[kQ],
-                                            u64 &improved) {
+                                            u64 &improved, Stage &sq) {
     u32 v[kQ];
     f32 cand[kQ];
     bool live[kQ];
@@ -193,7 +271,7 @@ __device__ __forceinline__ void visit_edges(const SsspArgs &a, const Target &t, 
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {
         if (first[q]) atomicOr((unsigned long long *)&a.reached[v[q] >> 6], 1ull << (v[q] & 63));
-        push_light(a, push[q], v[q], t);
+        push_light(a, push[q], v[q], t, sq);
     }
 }
 
@@ -209,7 +287,7 @@ __device__ __forceinline__ void segment(const SsspArgs &a, u32 u, u64 &b, u64 &e
 // [b, b + len) of u's row, u at distance du), 64 entries per step, so short
 // and uneven segments keep every lane busy.
 __device__ __forceinline__ void walk_segments(const SsspArgs &a, const Target &t, u32 u, f32 du, u64 b, u32 len,
-                                              u64 &improved, volatile int *slot) {
+                                              u64 &improved, volatile int *slot, Stage &sq) {
     if (!ballot(len != 0)) return;
     const u32 incl = wave_inclusive_scan(len);
     const u32 off = incl - len, total = __shfl(incl, kWave - 1);
@@ -240,15 +318,15 @@ __device__ __forceinline__ void walk_segments(const SsspArgs &a, const Target &t
                 kk[q] = bj + (pos - offj);
             }
         }
-        visit_edges(a, t, act, uu, dd, kk, improved);
+        visit_edges(a, t, act, uu, dd, kk, improved, sq);
     }
 }
 
 // Vertices list[0, count) of work tier 0 (64 vertices per wave, segments
 // walked together) or 1 (a wave each).
 template <bool LIGHT>
-__device__ void process_tier(const SsspArgs &a, const u32 *list, u32 count, int tier, const Target &t,
-                             u64 &improved, volatile int *slot) {
+__device__ __forceinline__ void process_tier(const SsspArgs &a, const u32 *list, u32 count, int tier,
+                                             const Target &t, u64 &improved, volatile int *slot, Stage &sq) {
     const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
     const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
@@ -267,7 +345,7 @@ __device__ void process_tier(const SsspArgs &a, const u32 *list, u32 count, int 
                     atomicOr(&a.ctl->bar.error, 4u);
                 }
             }
-            walk_segments(a, t, u, du, b, (u32)(e - b), improved, slot);
+            walk_segments(a, t, u, du, b, (u32)(e - b), improved, slot, sq);
         }
     } else {
         for (u64 i = wave_gid; i < count; i += nwaves) {
@@ -291,7 +369,7 @@ __device__ void process_tier(const SsspArgs &a, const u32 *list, u32 count, int 
                     uu[q] = u;
                     dd[q] = du;
                 }
-                visit_edges(a, t, act, uu, dd, kk, improved);
+                visit_edges(a, t, act, uu, dd, kk, improved, sq);
             }
         }
     }
@@ -300,8 +378,8 @@ __device__ void process_tier(const SsspArgs &a, const u32 *list, u32 count, int 
 // Hub tier: the flattened work space [0, total) of hubs [0, count); each wave
 // takes kHubChunk-edge chunks, so a long row is spread over the whole grid.
 template <bool LIGHT>
-__device__ void process_hubs(const SsspArgs &a, const u32 *hv, const u64 *hoff, const u32 *hlen, u64 packed,
-                             const Target &t, u64 &improved) {
+__device__ __forceinline__ void process_hubs(const SsspArgs &a, const u32 *hv, const u64 *hoff, const u32 *hlen,
+                                             u64 packed, const Target &t, u64 &improved, Stage &sq) {
     const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
     const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
@@ -335,7 +413,7 @@ __device__ void process_hubs(const SsspArgs &a, const u32 *hv, const u64 *hoff, 
                 uu[q] = u;
                 dd[q] = du;
             }
-            visit_edges(a, t, act, uu, dd, kk, improved);
+            visit_edges(a, t, act, uu, dd, kk, improved, sq);
         }
     }
 }
@@ -378,12 +456,23 @@ __device__ __forceinline__ void scan_open(const SsspArgs &a, F &&f) {
     }
 }
 
-__global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void sssp_kernel(SsspArgs a) {
     SsspCtl *ctl = a.ctl;
     const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
     __shared__ int slots[1024 / kWave][kWave];
     volatile int *slot = slots[threadIdx.x / kWave];
+    __shared__ StageLds slds;
+    Stage sq;
+    sq.L = &slds;
+    sq.w = (int)(threadIdx.x / kWave);
+#pragma unroll
+    for (int l = 0; l < kSLists; ++l) sq.n[l] = 0;
+    sq.buf = sq.slot = sq.hs = 0;
+    auto stage_to = [&](const Target &t) {
+        sq.buf = t.buf;
+        sq.slot = t.slot;
+    };
     u64 improved = 0;
     u32 x = 0;                       // step
     f32 lo = 0.0f, hi = next_bound(0.0f, a.delta);
@@ -409,11 +498,11 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
     auto clear_ahead = [&](u32 step) {
         if (gtid == 0) {
             const int s = (int)((step + 2) % 3);
-            st_relaxed(&ctl->flen[s][0], 0u);
-            st_relaxed(&ctl->flen[s][1], 0u);
-            st_relaxed(&ctl->fhub[s], 0ull);
-            st_relaxed(&ctl->hlen[s], 0u);
-            st_relaxed(&ctl->hhub[s], 0ull);
+            st_relaxed(&ctl->flen[s][0].v, 0u);
+            st_relaxed(&ctl->flen[s][1].v, 0u);
+            st_relaxed(&ctl->fhub[s].v, 0ull);
+            st_relaxed(&ctl->hlen[s].v, 0u);
+            st_relaxed(&ctl->hhub[s].v, 0ull);
             st_relaxed(&ctl->nmin[s], 0xFFFFFFFFu);
             st_relaxed(&ctl->nnext[s], 0u);
         }
@@ -423,7 +512,7 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
         u64 total = 0;
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
-            u32 c = ld_relaxed(&ctl->flen[step % 3][t]);
+            u32 c = ld_relaxed(&ctl->flen[step % 3][t].v);
             if (c > a.n) {
                 if (gtid == 0) atomicOr(&ctl->bar.error, 8u);
                 c = 0;
@@ -431,7 +520,7 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
             cnt[t] = c;
             total += c;
         }
-        *hub = ld_relaxed(&ctl->fhub[step % 3]);
+        *hub = ld_relaxed(&ctl->fhub[step % 3].v);
         return total + hub_count(*hub);
     };
     auto target = [&](u32 step, f32 below, f32 floor) {
@@ -446,8 +535,10 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
         for (int i = 0; i < 3; ++i) st_relaxed(&ctl->nmin[i], 0xFFFFFFFFu);
     }
     sync_traced(ctl, kPhaseInit, 0, 0);
+    stage_to(target(0, 0.f, 0.f));
     if (gtid < kWave)  // step 0 queues the root
-        push_light(a, lane == 0, trav_id(a.perm, a.root), target(0, 0.f, 0.f));
+        push_light(a, lane == 0, trav_id(a.perm, a.root), target(0, 0.f, 0.f), sq);
+    stage_flush(a, sq);
     sync_traced(ctl, kPhaseInit, 1, 0);
     x = 1;
 
@@ -461,10 +552,12 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
             if (gtid == 0) atomicAdd((unsigned long long *)&ctl->light_iters, 1ull);
             log_step(0, cnt[0], cnt[1], hub);
             const Target t = target(x, hi, lo);
+            stage_to(t);
 #pragma unroll
             for (int tier = 0; tier < 2; ++tier)
-                process_tier<true>(a, a.fq[x & 1][tier], cnt[tier], tier, t, improved, slot);
-            process_hubs<true>(a, a.fq[x & 1][2], a.fq_off[x & 1], a.fq_len[x & 1], hub, t, improved);
+                process_tier<true>(a, a.fq[x & 1][tier], cnt[tier], tier, t, improved, slot, sq);
+            process_hubs<true>(a, a.fq[x & 1][2], a.fq_off[x & 1], a.fq_len[x & 1], hub, t, improved, sq);
+            stage_flush(a, sq);
             sync_traced(ctl, kPhaseLight, __float_as_uint(lo), x);
             tick(0);
             ++x;
@@ -478,6 +571,8 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
         {
             const Target t = target(x, hi2, hi);
             const int hs = (int)((x + 1) % 3);
+            stage_to(t);
+            sq.hs = hs;
             scan_open(a, [&](u64 v, u32 lbits) {
                 bool mine = false, next = false;
                 u64 b = 0, e = 0;
@@ -493,24 +588,25 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
                     }
                 }
                 if (!ballot(mine || next)) return;  // the word holds only later buckets
-                push_light(a, next, (u32)v, t);
+                push_light(a, next, (u32)v, t, sq);
                 // the bucket's vertices are final now: the wave owns the word of v
                 const u64 sm = ballot(mine);
                 if (sm && lane == 0) a.settled[v >> 6] |= sm;
                 const u32 len = (u32)(e - b);
                 const bool medium = mine && len > kSmall && len <= kHub;
-                const u32 s1 = wave_append(&ctl->hlen[hs], medium);
-                if (medium) a.hq1[s1] = (u32)v;
-                hub_append(&ctl->hhub[hs], a.hqh, a.hq_off, a.hq_len, mine && len > kHub, (u32)v, len, a.n,
+                stage_push<2>(a, sq, medium, (u32)v);
+                hub_append(&ctl->hhub[hs].v, a.hqh, a.hq_off, a.hq_len, mine && len > kHub, (u32)v, len, a.n,
                            &ctl->bar.error);
-                walk_segments(a, t, (u32)v, dv, b, mine && len <= kSmall ? len : 0, improved, slot);
+                walk_segments(a, t, (u32)v, dv, b, mine && len <= kSmall ? len : 0, improved, slot, sq);
             });
+            stage_flush(a, sq);
             sync_traced(ctl, kPhaseFused, __float_as_uint(lo), x);
-            const u32 nmed = ld_relaxed(&ctl->hlen[hs]);
-            const u64 hh = ld_relaxed(&ctl->hhub[hs]);
+            const u32 nmed = ld_relaxed(&ctl->hlen[hs].v);
+            const u64 hh = ld_relaxed(&ctl->hhub[hs].v);
             if (nmed || hub_count(hh)) {  // second phase of the same step: longer heavy rows
-                process_tier<false>(a, a.hq1, min(nmed, (u32)a.n), 1, t, improved, slot);
-                process_hubs<false>(a, a.hqh, a.hq_off, a.hq_len, hh, t, improved);
+                process_tier<false>(a, a.hq1, min(nmed, (u32)a.n), 1, t, improved, slot, sq);
+                process_hubs<false>(a, a.hqh, a.hq_off, a.hq_len, hh, t, improved, sq);
+                stage_flush(a, sq);
                 sync_traced(ctl, kPhaseHeavy, __float_as_uint(lo), x);
             }
             tick(1);
@@ -560,6 +656,7 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
         clear_ahead(x);
         {
             const Target t = target(x, hi, lo);
+            stage_to(t);
             scan_open(a, [&](u64 v, u32 lbits) {
                 bool want = false;
                 if (lbits < kInfBits) {
@@ -567,9 +664,10 @@ __global__ __launch_bounds__(512) void sssp_kernel(SsspArgs a) {
                     want = d >= lo && d < hi;
                 }
                 if (!ballot(want)) return;
-                push_light(a, want, (u32)v, t);
+                push_light(a, want, (u32)v, t, sq);
             });
         }
+        stage_flush(a, sq);
         sync_traced(ctl, kPhaseCollect, __float_as_uint(lo), x);
         if (gtid == 0) atomicAdd((unsigned long long *)&ctl->scans, 1ull);
         tick(2);
