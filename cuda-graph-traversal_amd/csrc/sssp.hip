@@ -234,20 +234,23 @@ __device__ __forceinline__ void visit_edges(const SsspArgs &a, const Target &t, 
     u32 v[kQ];
     f32 cand[kQ];
     bool live[kQ];
+    // Branch-free loads (an inactive lane reads entry 0, valid whenever a
+    // segment is walked, or vertex 0, and its result is ignored), so that the
+    // kQ chains are in flight together.
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {
         live[q] = act[q];
-        v[q] = live[q] ? a.col[k[q]] : 0u;
-        cand[q] = live[q] ? du[q] + a.w[k[q]] : 0.f;
+        const u64 kk = live[q] ? k[q] : 0;
+        v[q] = a.col[kk];
+        cand[q] = du[q] + a.w[kk];
     }
     // a settled vertex has d_v < floor <= cand: skip its label
 #pragma unroll
-    for (int q = 0; q < kQ; ++q)
-        if (live[q]) live[q] = !((a.settled[v[q] >> 6] >> (v[q] & 63)) & 1);
+    for (int q = 0; q < kQ; ++q) live[q] = live[q] && !((a.settled[v[q] >> 6] >> (v[q] & 63)) & 1);
     // a plain read may be stale, i.e. larger: it only lets an atomic through
     u32 old[kQ];
 #pragma unroll
-    for (int q = 0; q < kQ; ++q) old[q] = live[q] ? a.dl[v[q]] : 0u;
+    for (int q = 0; q < kQ; ++q) old[q] = a.dl[live[q] ? v[q] : 0u];
     bool push[kQ], first[kQ];
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {

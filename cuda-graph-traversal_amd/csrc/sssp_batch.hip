@@ -277,26 +277,84 @@ __device__ __forceinline__ void push_item(const SsspBatchArgs &a, u32 v, u32 mas
                a.n, &ctl->bar.error);
 }
 
-// One relaxation: entry k of u's row for root r. Returns the root's bit when
-// the target's label went below t.below, and the target in v.
-__device__ __forceinline__ u32 relax_pair(const SsspBatchArgs &a, u32 u, int r, u64 k, const BTarget &t, u32 &v,
-                                          u64 &improved) {
-    v = a.col[k];
-    const f32 cand = __uint_as_float(a.dl[(u64)u * kBatchRoots + r]) + a.w[k];
-    const u32 cb = __float_as_uint(cand);
-    u32 *p = &a.dl[(u64)v * kBatchRoots + r];
-    if (*p <= cb) return 0;  // a plain read may be stale, i.e. larger: it only lets an atomic through
-    const u32 prev = atomicMin(p, cb);
-    if (prev <= cb) return 0;
-    atomicMin((unsigned long long *)&a.lab[(u64)v * kBatchRoots + r], ((unsigned long long)cb << 32) | u);
-    atomicMin(&a.vmin[v], cb);
-    if (prev >= kInfBits) {  // v reached from root r: it has open work again
-        atomicOr(&a.rmask[v], 1u << r);
-        atomicOr((unsigned long long *)&a.openw[v >> 6], 1ull << (v & 63));
+// A relaxation in two parts, so that a lane can have several relaxations in
+// flight: relax_load reads the target and its probe for entry k (u at
+// distance du, root r); relax_commit_push lowers the probes of a lane's
+// rounds and queues the targets lowered below t.below.
+struct Probe {
+    u32 v, cb, cur;  // target, candidate distance bits, probe as read
+};
+// Relaxation rounds a lane has in flight in the wave and hub tiers.
+constexpr int kRounds = 4;
+
+// Branch-free, so that the loads of both relaxations of a lane are issued
+// before the first one is waited for: an inactive lane reads entry 0 (valid
+// whenever a segment is walked) and its result is ignored.
+__device__ __forceinline__ void relax_load(const SsspBatchArgs &a, f32 du, int r, u64 k, bool act, Probe &p) {
+    const u64 kk = act ? k : 0;
+    p.v = a.col[kk];
+    p.cb = __float_as_uint(du + a.w[kk]);
+    p.cur = a.dl[(u64)p.v * kBatchRoots + r];  // a plain read may be stale, i.e. larger: it only lets an atomic through
+}
+
+
+// Commits R loaded relaxations of a lane and pushes the lowered targets, one
+// phase at a time (probe atomics, label and mask updates, light-row counts,
+// pending-mask ORs, queue appends), so that the R atomics of a phase are in
+// flight together instead of one relaxation's chain after the other's.
+template <int R>
+__device__ __forceinline__ void relax_commit_push(const SsspBatchArgs &a, const u32 (&u)[R], const int (&r)[R],
+                                                  const Probe (&pr)[R], const bool (&act)[R], const BTarget &t,
+                                                  u64 &improved, WaveQ &wq) {
+    bool tried[R];
+    u32 prev[R], out[R];
+#pragma unroll
+    for (int h = 0; h < R; ++h) tried[h] = act[h] && pr[h].cur > pr[h].cb;
+#pragma unroll
+    for (int h = 0; h < R; ++h)
+        prev[h] = tried[h] ? atomicMin(&a.dl[(u64)pr[h].v * kBatchRoots + r[h]], pr[h].cb) : 0u;
+#pragma unroll
+    for (int h = 0; h < R; ++h) {
+        out[h] = 0;
+        const u32 v = pr[h].v, cb = pr[h].cb;
+        if (tried[h] && prev[h] > cb) {
+            const f32 cand = __uint_as_float(cb);
+            atomicMin((unsigned long long *)&a.lab[(u64)v * kBatchRoots + r[h]], ((unsigned long long)cb << 32) | u[h]);
+            atomicMin(&a.vmin[v], cb);
+            if (prev[h] >= kInfBits) {  // v reached from root r: it has open work again
+                atomicOr(&a.rmask[v], 1u << r[h]);
+                atomicOr((unsigned long long *)&a.openw[v >> 6], 1ull << (v & 63));
+            }
+            ++improved;
+            if (cand < t.floor) atomicOr(&a.ctl->bar.error, 16u);
+            out[h] = cand < t.below ? 1u << r[h] : 0u;
+        }
     }
-    ++improved;
-    if (cand < t.floor) atomicOr(&a.ctl->bar.error, 16u);
-    return cand < t.below ? 1u << r : 0u;
+    u64 any = 0;
+#pragma unroll
+    for (int h = 0; h < R; ++h) any |= ballot(out[h] != 0);
+    if (!any) return;
+    SsspBatchCtl *ctl = a.ctl;
+    // the flag only tells that the next bucket is not empty: set it once
+    if (lane_id() == (u32)(__ffsll((unsigned long long)any) - 1) && !ld_relaxed(&ctl->nnext[t.slot]))
+        st_relaxed(&ctl->nnext[t.slot], 1u);
+    // a vertex without light entries has nothing to do in light steps
+    u32 *pend = t.buf ? a.pend[1] : a.pend[0];
+    u32 lw[R], po[R];
+#pragma unroll
+    for (int h = 0; h < R; ++h) lw[h] = out[h] ? a.split[pr[h].v] : 0u;
+#pragma unroll
+    for (int h = 0; h < R; ++h) po[h] = (out[h] && lw[h]) ? atomicOr(&pend[pr[h].v], out[h]) : 1u;
+#pragma unroll
+    for (int h = 0; h < R; ++h) {
+        const bool q = out[h] && lw[h] && po[h] == 0;
+        if (!ballot(q)) continue;
+        const u32 v = pr[h].v;
+        q_push32<kQLight0>(a, wq, q && btier(lw[h]) == 0, v);
+        q_push32<kQLight0 + 1>(a, wq, q && btier(lw[h]) == 1, v);
+        hub_append(&ctl->fhub[t.slot].v, t.buf ? a.fq[1][2] : a.fq[0][2], t.buf ? a.fq_off[1] : a.fq_off[0],
+                   t.buf ? a.fq_len[1] : a.fq_len[0], q && btier(lw[h]) == 2, v, lw[h], a.n, &ctl->bar.error);
+    }
 }
 
 // Work counts (items, entries, pairs) of one segment kind, kept per block in
@@ -378,21 +436,39 @@ __device__ __forceinline__ void group_tier(const SsspBatchArgs &a, u32 *pend, co
             const u64 nitems = (u64)__popcll(ballot(M != 0)), nent = wave_sum((u64)(e - b));
             if (lane == 0) work_add(wk, nitems, nent, total);
         }
-        for (u32 c = 0; c < total; c += kWave) {
-            bool inter;
-            int last;
-            int j = chunk_owner(slot, off, pairs, c, inter, last);
-            if (j < 0) j = 0;
-            const u32 uj = __shfl(u, j), Mj = __shfl(M, j), offj = __shfl(off, j);
-            const u64 bj = shfl64(b, j);
-            const u32 pos = c + lane;
-            u32 v = 0, out = 0;
-            if (pos < total) {
-                const u32 k = (u32)__popc(Mj), q = pos - offj;
-                const u32 entry = q / k;
-                out = relax_pair(a, uj, nth_root(Mj, q - entry * k), bj + entry, t, v, improved);
+        for (u32 c = 0; c < total; c += 2 * kWave) {
+            u32 uu[2];
+            int rr[2];
+            bool act[2];
+            Probe pr[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const u32 ch = c + (u32)h * kWave;
+                uu[h] = 0;
+                rr[h] = 0;
+                act[h] = false;
+                u64 k = 0;
+                if (ch < total) {
+                    bool inter;
+                    int last;
+                    int j = chunk_owner(slot, off, pairs, ch, inter, last);
+                    if (j < 0) j = 0;
+                    const u32 uj = __shfl(u, j), Mj = __shfl(M, j), offj = __shfl(off, j);
+                    const u64 bj = shfl64(b, j);
+                    const u32 pos = ch + lane;
+                    if (pos < total) {
+                        const u32 kk = (u32)__popc(Mj), q = pos - offj;
+                        const u32 entry = q / kk;
+                        uu[h] = uj;
+                        rr[h] = nth_root(Mj, q - entry * kk);
+                        k = bj + entry;
+                        act[h] = true;
+                    }
+                }
+                const f32 du = __uint_as_float(a.dl[(u64)uu[h] * kBatchRoots + rr[h]]);
+                relax_load(a, du, rr[h], k, act[h], pr[h]);
             }
-            push_item(a, v, out, t, wq);
+            relax_commit_push<2>(a, uu, rr, pr, act, t, improved, wq);
         }
     }
 }
@@ -445,11 +521,24 @@ __device__ __forceinline__ void wave_tier(const SsspBatchArgs &a, u32 *pend, con
         bsegment<K>(a, u, b, e);
         if (lane == 0) work_add(wk, 1, e - b, (e - b) * (u64)__popc(M));
         const PairLanes p = pair_lanes(M);
-        for (u64 e0 = b; e0 < e; e0 += p.per) {
-            const u64 k = e0 + p.eoff;
-            u32 v = 0, out = 0;
-            if (p.active && k < e) out = relax_pair(a, u, p.r, k, t, v, improved);
-            push_item(a, v, out, t, wq);
+        const f32 du = p.active ? __uint_as_float(a.dl[(u64)u * kBatchRoots + p.r]) : 0.f;
+        u32 uk[kRounds];
+        int rk[kRounds];
+#pragma unroll
+        for (int h = 0; h < kRounds; ++h) {
+            uk[h] = u;
+            rk[h] = p.r;
+        }
+        for (u64 e0 = b; e0 < e; e0 += kRounds * p.per) {
+            bool act[kRounds];
+            Probe pr[kRounds];
+#pragma unroll
+            for (int h = 0; h < kRounds; ++h) {
+                const u64 k = e0 + (u64)h * p.per + p.eoff;
+                act[h] = p.active && k < e;
+                relax_load(a, du, p.r, k, act[h], pr[h]);
+            }
+            relax_commit_push<kRounds>(a, uk, rk, pr, act, t, improved, wq);
         }
     }
 }
@@ -482,11 +571,25 @@ __device__ __forceinline__ void hub_tier(const SsspBatchArgs &a, const u32 *hv, 
             const u64 inchunk = min((u64)kHubChunk, (u64)len - min((u64)len, local));
             if (lane == 0) work_add(wk, local == 0 ? 1 : 0, inchunk, inchunk * (u64)__popc(M));
             const PairLanes p = pair_lanes(M);
-            for (u32 j = 0; j < (u32)kHubChunk; j += p.per) {
-                const u64 off = local + j + p.eoff;
-                u32 v = 0, out = 0;
-                if (p.active && j + p.eoff < (u32)kHubChunk && off < len) out = relax_pair(a, u, p.r, b + off, t, v, improved);
-                push_item(a, v, out, t, wq);
+            const f32 du = p.active ? __uint_as_float(a.dl[(u64)u * kBatchRoots + p.r]) : 0.f;
+            u32 uk[kRounds];
+            int rk[kRounds];
+#pragma unroll
+            for (int h = 0; h < kRounds; ++h) {
+                uk[h] = u;
+                rk[h] = p.r;
+            }
+            for (u32 j = 0; j < (u32)kHubChunk; j += kRounds * p.per) {
+                bool act[kRounds];
+                Probe pr[kRounds];
+#pragma unroll
+                for (int h = 0; h < kRounds; ++h) {
+                    const u32 jj = j + (u32)h * p.per + p.eoff;
+                    const u64 off = local + jj;
+                    act[h] = p.active && jj < (u32)kHubChunk && off < len;
+                    relax_load(a, du, p.r, b + off, act[h], pr[h]);
+                }
+                relax_commit_push<kRounds>(a, uk, rk, pr, act, t, improved, wq);
             }
         }
         if (K != kSegMedium && lane == 0) {

@@ -464,7 +464,8 @@ def test_autotuner_applies_candidates(arena, oracle):
         out = autotune.tune(arena, g, batch=True)
         p = out["params"]
         assert (p["bfs_alpha"], p["bfs_beta"]) in autotune.BFS_SWITCH and p["bfs_block"] in autotune.BLOCKS
-        assert p["sssp_delta"] in autotune.SSSP_DELTA and p["batch_sssp_delta"] in autotune.BATCH_SSSP_DELTA
+        assert p["sssp_delta"] in autotune.SSSP_DELTA
+        assert (p["batch_sssp_delta"], p["batch_sssp_long_weight"]) in autotune.BATCH_SSSP
         assert p["batch_alpha"] in autotune.BATCH_ALPHA and p["batch_heavy_degree"] in autotune.BATCH_HEAVY
         for k in keys:
             assert arena.get_param(k) == p[k]
@@ -472,6 +473,7 @@ def test_autotuner_applies_candidates(arena, oracle):
         arena.bfs_state = arena.bfs_create(g)
         bfs_check(arena, oracle, g, ref, int(roots[0]))
         sb = arena.sssp_batch_create(g, p["batch_sssp_delta"])
+        sb.long_weight = p["batch_sssp_long_weight"]
         sssp_batch_check(arena, oracle, g, ref, sb, roots[:16])
     finally:
         for k, v in old.items():
