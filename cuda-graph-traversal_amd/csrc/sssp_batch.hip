@@ -89,10 +89,9 @@ struct SsspBatchArgs {
     const u32 *perm, *iperm;  // traversal <-> original ids (null: identity)
     u64 *lab;          // [n * kBatchRoots]
     u32 *dl;           // [n * kBatchRoots] probes (aliases dist)
-    u32 *rmask;        // [n] roots with a finite label of the vertex
     u32 *smask;        // [n] roots whose label of the vertex is final
     u32 *vmin;         // [n] smallest probe bits of the vertex's open roots
-    u64 *openw;        // [ceil(n/64)] vertex has a root in rmask & ~smask
+    u64 *openw;        // [ceil(n/64)] vertex has a reached root that is not final
     u32 *pend[2];      // [n] pending root masks, by step parity
     u32 *fq[2][3];     // light queues [buffer][tier]; tier 2 holds hub vertices
     u64 *fq_off[2];    // light hub tier: work offsets
@@ -284,7 +283,7 @@ __device__ __forceinline__ void push_item(const SsspBatchArgs &a, u32 v, u32 mas
 struct Probe {
     u32 v, cb, cur;  // target, candidate distance bits, probe as read
 };
-// Relaxation rounds a lane has in flight in the wave and hub tiers.
+// Relaxation rounds a lane has in flight in the relaxation tiers.
 constexpr int kRounds = 4;
 
 // Branch-free, so that the loads of both relaxations of a lane are issued
@@ -321,9 +320,11 @@ __device__ __forceinline__ void relax_commit_push(const SsspBatchArgs &a, const 
             const f32 cand = __uint_as_float(cb);
             atomicMin((unsigned long long *)&a.lab[(u64)v * kBatchRoots + r[h]], ((unsigned long long)cb << 32) | u[h]);
             atomicMin(&a.vmin[v], cb);
-            if (prev[h] >= kInfBits) {  // v reached from root r: it has open work again
-                atomicOr(&a.rmask[v], 1u << r[h]);
-                atomicOr((unsigned long long *)&a.openw[v >> 6], 1ull << (v & 63));
+            if (prev[h] >= kInfBits) {  // v reached from root r: it has open work (again)
+                // bits are cleared only by dense scans, never during a relaxation phase, so a
+                // set bit read here is current; a clear one may race a concurrent OR, which is harmless
+                const u64 bit = 1ull << (v & 63);
+                if (!(a.openw[v >> 6] & bit)) atomicOr((unsigned long long *)&a.openw[v >> 6], bit);
             }
             ++improved;
             if (cand < t.floor) atomicOr(&a.ctl->bar.error, 16u);
@@ -436,13 +437,13 @@ __device__ __forceinline__ void group_tier(const SsspBatchArgs &a, u32 *pend, co
             const u64 nitems = (u64)__popcll(ballot(M != 0)), nent = wave_sum((u64)(e - b));
             if (lane == 0) work_add(wk, nitems, nent, total);
         }
-        for (u32 c = 0; c < total; c += 2 * kWave) {
-            u32 uu[2];
-            int rr[2];
-            bool act[2];
-            Probe pr[2];
+        for (u32 c = 0; c < total; c += kRounds * kWave) {
+            u32 uu[kRounds];
+            int rr[kRounds];
+            bool act[kRounds];
+            Probe pr[kRounds];
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            for (int h = 0; h < kRounds; ++h) {
                 const u32 ch = c + (u32)h * kWave;
                 uu[h] = 0;
                 rr[h] = 0;
@@ -468,7 +469,7 @@ __device__ __forceinline__ void group_tier(const SsspBatchArgs &a, u32 *pend, co
                 const f32 du = __uint_as_float(a.dl[(u64)uu[h] * kBatchRoots + rr[h]]);
                 relax_load(a, du, rr[h], k, act[h], pr[h]);
             }
-            relax_commit_push<2>(a, uu, rr, pr, act, t, improved, wq);
+            relax_commit_push<kRounds>(a, uu, rr, pr, act, t, improved, wq);
         }
     }
 }
@@ -685,10 +686,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
     // word of openw, the wave walks the words that have open vertices, one
     // lane per vertex. vmin[v], the smallest open probe of v (exact between
     // phases), screens the vertex: only one with vmin < z reads its probe line,
-    // in the 16-byte pieces that hold a live root. f(word, v, A, B, dmin, act)
-    // gets the roots whose open probe is in [x, y) (A) and in [y, z) (B), the
-    // smallest open probe bits >= y (vmin itself for a screened-out open
-    // vertex, all ones for a closed one), and whether the line was read.
+    // in the 16-byte pieces that hold a root not yet final. f(word, v, A, B,
+    // dmin, act, R) gets the roots whose open probe is in [x, y) (A) and in
+    // [y, z) (B), the smallest open probe bits >= y (vmin itself for a
+    // screened-out open vertex, all ones for a closed one), whether the line
+    // was read, and the reached roots that are not final (R, read lines only).
     auto scan_open = [&](f32 x, f32 y, f32 z, auto &&f) {
         const u32 zb = __float_as_uint(z);
         for (u64 w0 = wave_gid * kWave; w0 < nwords; w0 += nwaves * kWave) {
@@ -703,28 +705,29 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                 const bool isopen = (shfl64(open_l, j) >> lane) & 1;
                 const u32 vm = isopen ? a.vmin[v] : 0xFFFFFFFFu;
                 const bool act = vm < zb;
-                const u32 live = act ? a.rmask[v] & ~a.smask[v] : 0u;
-                u32 A = 0, B = 0, dmin = act ? 0xFFFFFFFFu : vm;
-                if (live) {
+                const u32 open = act ? ~a.smask[v] : 0u;
+                u32 A = 0, B = 0, R = 0, dmin = act ? 0xFFFFFFFFu : vm;
+                if (open) {
                     const uint4 *line = reinterpret_cast<const uint4 *>(a.dl + v * kBatchRoots);
 #pragma unroll 2
                     for (int q = 0; q < kBatchRoots / 4; ++q) {
-                        const u32 lq = (live >> (4 * q)) & 0xFu;
+                        const u32 lq = (open >> (4 * q)) & 0xFu;
                         if (!lq) continue;
                         const uint4 c = line[q];
                         const u32 bits[4] = {c.x, c.y, c.z, c.w};
 #pragma unroll
                         for (int i = 0; i < 4; ++i) {
-                            if (!((lq >> i) & 1)) continue;
+                            if (!((lq >> i) & 1) || bits[i] >= kInfBits) continue;  // final or unreached
                             const f32 d = __uint_as_float(bits[i]);
                             const u32 bit = 1u << (4 * q + i);
+                            R |= bit;
                             if (d >= x && d < y) A |= bit;
                             if (d >= y && d < z) B |= bit;
                             if (d >= y) dmin = min(dmin, bits[i]);
                         }
                     }
                 }
-                f(word, v, A, B, dmin, act);
+                f(word, v, A, B, dmin, act, R);
             }
         }
     };
@@ -734,7 +737,6 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
         st_relaxed(&a.lab[root * kBatchRoots + gtid], root);  // distance 0.0f, parent = root
         st_relaxed(&a.dl[root * kBatchRoots + gtid], 0u);
         st_relaxed(&a.vmin[root], 0u);
-        atomicOr(&a.rmask[root], 1u << gtid);
         atomicOr((unsigned long long *)&a.openw[root >> 6], 1ull << (root & 63));
     }
     if (gtid == 0)
@@ -782,7 +784,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
             light_dest(t);
             wq.hs = hs;
             wq.ls = ls;
-            scan_open(lo, hi, hi2, [&](u64 word, u64 v, u32 H, u32 N, u32 dmin, bool act) {
+            scan_open(lo, hi, hi2, [&](u64 word, u64 v, u32 H, u32 N, u32 dmin, bool act, u32 R) {
                 if (act) a.vmin[v] = dmin;  // the open roots left are those >= hi
                 {
                     const u64 na = __popcll(ballot(act)), nh = __popcll(ballot(H != 0));
@@ -792,9 +794,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                 // the bucket's roots of v are final now; v closes when no reached root is open
                 bool closed = false;
                 if (H) {
-                    const u32 sm = a.smask[v] | H;
-                    a.smask[v] = sm;
-                    closed = (a.rmask[v] & ~sm) == 0;
+                    a.smask[v] |= H;
+                    closed = (R & ~H) == 0;
                 }
                 const u64 cm = ballot(closed);
                 if (cm && lane == 0) atomicAnd((unsigned long long *)&a.openw[word], ~(unsigned long long)cm);
@@ -885,12 +886,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
         clear_ahead(x);
         {
             u32 m = 0xFFFFFFFFu;
-            scan_open(hi, hi, hi, [&](u64 word, u64 v, u32 A, u32 B, u32 dmin, bool act) {
+            scan_open(hi, hi, hi, [&](u64 word, u64 v, u32 A, u32 B, u32 dmin, bool act, u32 R) {
                 (void)word;
                 (void)v;
                 (void)A;
                 (void)B;
                 (void)act;
+                (void)R;
                 if (dmin < kInfBits) m = min(m, dmin);
             });
             m = wave_min(m);
@@ -916,11 +918,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
         {
             const BTarget t = target(x, hi, lo);
             light_dest(t);
-            scan_open(lo, hi, hi, [&](u64 word, u64 v, u32 M, u32 B, u32 dmin, bool act) {
+            scan_open(lo, hi, hi, [&](u64 word, u64 v, u32 M, u32 B, u32 dmin, bool act, u32 R) {
                 (void)word;
                 (void)B;
                 (void)dmin;
                 (void)act;
+                (void)R;
                 push_item(a, (u32)v, M, t, wq);
             });
             q_flush(a, wq);
@@ -969,9 +972,8 @@ __global__ __launch_bounds__(256) void sssp_batch_clear_kernel(u64 *__restrict__
         if (t < 2 * n) {
             pend[t] = 0;
             done[t] = 0;
-            masks[t] = 0;
+            masks[t] = t < n ? 0u : 0xFFFFFFFFu;  // final-root masks, then vmin
         }
-        if (t < n) masks[2 * n + t] = 0xFFFFFFFFu;  // vmin
         if (t < n) {
             lpend[t] = 0;
             ldone[t] = 0;
@@ -1073,9 +1075,8 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
     a.lq_off = s->long_offsets;
     a.lab = s->labels;
     a.dl = (u32 *)s->dist;
-    a.rmask = s->root_masks;
-    a.smask = s->root_masks + n;
-    a.vmin = s->root_masks + 2 * n;
+    a.smask = s->root_masks;
+    a.vmin = s->root_masks + n;
     a.openw = s->open_words;
     for (int b = 0; b < 2; ++b) {
         a.pend[b] = s->pending + (u64)b * n;
@@ -1146,7 +1147,7 @@ Sssp_Batch_State oned_graph_sssp_batch_create(Worker_State *ws, const Oned_Graph
     s.hub_lengths = (u32 *)arena_alloc(ws, 3 * n * sizeof(u32), "batched SSSP hub lengths");
     s.hub_aux = (u32 *)arena_alloc(ws, 3 * n * sizeof(u32), "batched SSSP hub chunk counts / masks");
     s.split = (u32 *)arena_alloc(ws, n * sizeof(u32), "batched SSSP light split");
-    s.root_masks = (u32 *)arena_alloc(ws, 3 * n * sizeof(u32), "batched SSSP root masks and smallest open probes");
+    s.root_masks = (u32 *)arena_alloc(ws, 2 * n * sizeof(u32), "batched SSSP final-root masks and smallest open probes");
     s.open_words = (u64 *)arena_alloc(ws, ((n + 63) / 64) * sizeof(u64), "batched SSSP open-vertex bitmap");
     s.long_split = (u32 *)arena_alloc(ws, n * sizeof(u32), "batched SSSP long split");
     s.long_work = (u32 *)arena_alloc(ws, 6 * n * sizeof(u32), "batched SSSP deferred long-edge queue");

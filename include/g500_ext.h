@@ -97,8 +97,8 @@ struct Sssp_Batch_State {
     u32 *hub_lengths;   /* [3n] hub segment lengths */
     u32 *hub_aux;       /* [3n] light: finished chunks per hub; heavy: root masks */
     u32 *split;         /* [n] light entries (w < split_delta) per row */
-    u32 *root_masks;    /* [3n] per vertex: roots with a finite label, roots whose label is final,
-                           smallest distance bits among the roots not yet final */
+    u32 *root_masks;    /* [2n] per vertex: roots whose label is final, then the smallest distance
+                           bits among the roots not yet final */
     u64 *open_words;    /* [ceil(n/64)] vertices with a reached root not yet final */
     u32 *long_split;    /* [n] entries with w < max(long_weight, delta) per row */
     u32 *long_work;     /* [6n] deferred long edges: pending root masks, queue tiers 0-2, hub lengths,
