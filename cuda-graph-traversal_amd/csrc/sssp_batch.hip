@@ -283,8 +283,10 @@ __device__ __forceinline__ void push_item(const SsspBatchArgs &a, u32 v, u32 mas
 struct Probe {
     u32 v, cb, cur;  // target, candidate distance bits, probe as read
 };
-// Relaxation rounds a lane has in flight in the relaxation tiers.
+// Relaxation rounds a lane has in flight: group tier, and the wave and hub
+// tiers (one item's row, so a round costs fewer registers there).
 constexpr int kRounds = 4;
+constexpr int kWideRounds = 8;
 
 // Branch-free, so that the loads of both relaxations of a lane are issued
 // before the first one is waited for: an inactive lane reads entry 0 (valid
@@ -523,23 +525,23 @@ __device__ __forceinline__ void wave_tier(const SsspBatchArgs &a, u32 *pend, con
         if (lane == 0) work_add(wk, 1, e - b, (e - b) * (u64)__popc(M));
         const PairLanes p = pair_lanes(M);
         const f32 du = p.active ? __uint_as_float(a.dl[(u64)u * kBatchRoots + p.r]) : 0.f;
-        u32 uk[kRounds];
-        int rk[kRounds];
+        u32 uk[kWideRounds];
+        int rk[kWideRounds];
 #pragma unroll
-        for (int h = 0; h < kRounds; ++h) {
+        for (int h = 0; h < kWideRounds; ++h) {
             uk[h] = u;
             rk[h] = p.r;
         }
-        for (u64 e0 = b; e0 < e; e0 += kRounds * p.per) {
-            bool act[kRounds];
-            Probe pr[kRounds];
+        for (u64 e0 = b; e0 < e; e0 += kWideRounds * p.per) {
+            bool act[kWideRounds];
+            Probe pr[kWideRounds];
 #pragma unroll
-            for (int h = 0; h < kRounds; ++h) {
+            for (int h = 0; h < kWideRounds; ++h) {
                 const u64 k = e0 + (u64)h * p.per + p.eoff;
                 act[h] = p.active && k < e;
                 relax_load(a, du, p.r, k, act[h], pr[h]);
             }
-            relax_commit_push<kRounds>(a, uk, rk, pr, act, t, improved, wq);
+            relax_commit_push<kWideRounds>(a, uk, rk, pr, act, t, improved, wq);
         }
     }
 }
@@ -573,24 +575,24 @@ __device__ __forceinline__ void hub_tier(const SsspBatchArgs &a, const u32 *hv, 
             if (lane == 0) work_add(wk, local == 0 ? 1 : 0, inchunk, inchunk * (u64)__popc(M));
             const PairLanes p = pair_lanes(M);
             const f32 du = p.active ? __uint_as_float(a.dl[(u64)u * kBatchRoots + p.r]) : 0.f;
-            u32 uk[kRounds];
-            int rk[kRounds];
+            u32 uk[kWideRounds];
+            int rk[kWideRounds];
 #pragma unroll
-            for (int h = 0; h < kRounds; ++h) {
+            for (int h = 0; h < kWideRounds; ++h) {
                 uk[h] = u;
                 rk[h] = p.r;
             }
-            for (u32 j = 0; j < (u32)kHubChunk; j += kRounds * p.per) {
-                bool act[kRounds];
-                Probe pr[kRounds];
+            for (u32 j = 0; j < (u32)kHubChunk; j += kWideRounds * p.per) {
+                bool act[kWideRounds];
+                Probe pr[kWideRounds];
 #pragma unroll
-                for (int h = 0; h < kRounds; ++h) {
+                for (int h = 0; h < kWideRounds; ++h) {
                     const u32 jj = j + (u32)h * p.per + p.eoff;
                     const u64 off = local + jj;
                     act[h] = p.active && jj < (u32)kHubChunk && off < len;
                     relax_load(a, du, p.r, b + off, act[h], pr[h]);
                 }
-                relax_commit_push<kRounds>(a, uk, rk, pr, act, t, improved, wq);
+                relax_commit_push<kWideRounds>(a, uk, rk, pr, act, t, improved, wq);
             }
         }
         if (K != kSegMedium && lane == 0) {
