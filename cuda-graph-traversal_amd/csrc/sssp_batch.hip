@@ -136,6 +136,13 @@ struct BTarget {
 
 __device__ __forceinline__ int btier(u32 len) { return len <= kBSmall ? 0 : (len <= kBHub ? 1 : 2); }
 
+// Wave indices as wave-uniform (scalar) values: the compiler cannot tell on
+// its own that threadIdx.x / 64 is the same for every lane of a wave.
+__device__ __forceinline__ int wave_in_block() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave)); }
+__device__ __forceinline__ u64 wave_index() {
+    return (u64)blockIdx.x * (blockDim.x / kWave) + (u64)wave_in_block();
+}
+
 // Wave-buffered appends. A wave stages what it appends to the light queue
 // tiers, the deferred queue tiers (u32 vertices) and the medium lists (u64
 // items) in LDS and writes them out in runs: when its buffer for a list is
@@ -314,6 +321,7 @@ __device__ __forceinline__ void relax_commit_push(const SsspBatchArgs &a, const 
 #pragma unroll
     for (int h = 0; h < R; ++h)
         prev[h] = tried[h] ? atomicMin(&a.dl[(u64)pr[h].v * kBatchRoots + r[h]], pr[h].cb) : 0u;
+    u32 nimp = 0;  // lowered probes of this lane; summed per wave below (a wave-uniform counter)
 #pragma unroll
     for (int h = 0; h < R; ++h) {
         out[h] = 0;
@@ -328,11 +336,12 @@ __device__ __forceinline__ void relax_commit_push(const SsspBatchArgs &a, const 
                 const u64 bit = 1ull << (v & 63);
                 if (!(a.openw[v >> 6] & bit)) atomicOr((unsigned long long *)&a.openw[v >> 6], bit);
             }
-            ++improved;
+            ++nimp;
             if (cand < t.floor) atomicOr(&a.ctl->bar.error, 16u);
             out[h] = cand < t.below ? 1u << r[h] : 0u;
         }
     }
+    if (ballot(nimp != 0)) improved += wave_sum((u64)nimp);
     u64 any = 0;
 #pragma unroll
     for (int h = 0; h < R; ++h) any |= ballot(out[h] != 0);
@@ -407,7 +416,7 @@ __device__ __forceinline__ void group_tier(const SsspBatchArgs &a, u32 *pend, co
                            const BTarget &t, u64 &improved, volatile int *slot, u64 *wk, WaveQ &wq) {
     const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
-    const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
+    const u64 wave_gid = wave_index(), nwaves = gthreads / kWave;
     for (u64 i0 = wave_gid * kWave; i0 < count; i0 += nwaves * kWave) {
         const u64 i = i0 + lane;
         u32 u = 0, M = 0;
@@ -499,7 +508,7 @@ __device__ __forceinline__ void wave_tier(const SsspBatchArgs &a, u32 *pend, con
                           const BTarget &t, u64 &improved, u64 *wk, WaveQ &wq) {
     const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
-    const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
+    const u64 wave_gid = wave_index(), nwaves = gthreads / kWave;
     for (u64 i = wave_gid; i < count; i += nwaves) {
         u32 u, M;
         if (K != kSegMedium) {
@@ -554,7 +563,7 @@ __device__ __forceinline__ void hub_tier(const SsspBatchArgs &a, const u32 *hv, 
                          u32 *pend, u64 packed, const BTarget &t, u64 &improved, u64 *wk, WaveQ &wq) {
     const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
-    const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
+    const u64 wave_gid = wave_index(), nwaves = gthreads / kWave;
     const u64 count = hub_count(packed), total = hub_total(packed);
     u64 i = 0;
     for (u64 c = wave_gid; c * kHubChunk < total; c += nwaves) {
@@ -610,16 +619,16 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
     const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
     __shared__ int slots[512 / kWave][kWave];
-    volatile int *slot = slots[threadIdx.x / kWave];
+    volatile int *slot = slots[wave_in_block()];
     const u64 nwords = (a.n + 63) / 64;
-    const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
+    const u64 wave_gid = wave_index(), nwaves = gthreads / kWave;
     u64 improved = 0;
     __shared__ u64 wks[4][3];  // work counts per segment kind and of the dense scans, this block
     if (threadIdx.x < 12) wks[threadIdx.x / 3][threadIdx.x % 3] = 0;
     __shared__ QLds qlds;
     WaveQ wq;
     wq.L = &qlds;
-    wq.w = (int)(threadIdx.x / kWave);
+    wq.w = wave_in_block();
 #pragma unroll
     for (int l = 0; l < kQ32 + kQ64; ++l) wq.n[l] = 0;
     wq.buf = wq.slot = wq.hs = wq.ls = 0;
@@ -638,19 +647,26 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
     f32 epoch_lo = 0.0f;
 
     const bool timer = gtid == 0;
-    u64 t_prev = timer ? wall_clock64() : 0, t_start = t_prev, t_acc[5] = {0, 0, 0, 0, 0};
-    u32 nlog = 0;
+    // timer state of thread 0 lives in LDS: [0] previous tick, [1] start,
+    // [2..6] per-phase sums, [7] log entries (registers held by every lane
+    // for one thread's bookkeeping would cost the relaxation loops)
+    __shared__ u64 tst[8];
+    if (timer) {
+        tst[0] = tst[1] = wall_clock64();
+        for (int i = 2; i < 8; ++i) tst[i] = 0;
+    }
     auto tick = [&](int phase, u64 items = 0) {
         if (timer) {
-            const u64 t = wall_clock64();
-            t_acc[phase] += t - t_prev;
+            const u64 t = wall_clock64(), d = t - tst[0];
+            const u32 nlog = (u32)tst[7];
+            tst[2 + phase] += d;
             if (nlog < kBStepLog) {
                 st_relaxed(&ctl->log_kind[nlog], (u32)phase);
                 st_relaxed(&ctl->log_items[nlog], (u32)min(items, (u64)0xFFFFFFFFu));
-                st_relaxed(&ctl->log_ticks[nlog], t - t_prev);
+                st_relaxed(&ctl->log_ticks[nlog], d);
             }
-            ++nlog;
-            t_prev = t;
+            tst[7] = nlog + 1;
+            tst[0] = t;
         }
     };
     auto clear_ahead = [&](u32 step) {
@@ -935,16 +951,15 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
         tick(4);
         ++x;
     }
-    improved = wave_sum(improved);
     if (lane == 0 && improved) atomicAdd((unsigned long long *)&ctl->improvements, (unsigned long long)improved);
     __syncthreads();
     if (threadIdx.x < 12 && wks[threadIdx.x / 3][threadIdx.x % 3])
         atomicAdd((unsigned long long *)&ctl->work[threadIdx.x / 3][threadIdx.x % 3],
                   (unsigned long long)wks[threadIdx.x / 3][threadIdx.x % 3]);
     if (timer) {
-        st_relaxed(&ctl->nlog, nlog);
-        for (int i = 0; i < 5; ++i) st_relaxed(&ctl->ticks[i], t_acc[i]);
-        st_relaxed(&ctl->ticks[5], wall_clock64() - t_start);
+        st_relaxed(&ctl->nlog, (u32)tst[7]);
+        for (int i = 0; i < 5; ++i) st_relaxed(&ctl->ticks[i], tst[2 + i]);
+        st_relaxed(&ctl->ticks[5], wall_clock64() - tst[1]);
     }
     // export, root-major (traversal.cu:333-340): unreached -> -1.0f / -1. The
     // probes are no longer read after the last barrier, so dist (their
