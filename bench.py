@@ -78,8 +78,10 @@ def main():
     ap.add_argument("--scale", type=int, default=26)
     ap.add_argument("--edgefactor", type=int, default=16)
     ap.add_argument("--delta", type=float, default=0.1, help="SSSP delta when not tuned")
-    ap.add_argument("--long-weight", type=float, default=0.2,
-                    help="batched SSSP deferral threshold (edges with w >= it are relaxed in batches) when not tuned")
+    ap.add_argument("--long-weight", type=float, default=0.25,
+                    help="batched SSSP long-edge threshold (deferred over windows this wide) when not tuned")
+    ap.add_argument("--mid-weight", type=float, default=0.075,
+                    help="batched SSSP mid-edge threshold (deferred over windows this wide) when not tuned")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (knobs at their defaults, delta from --delta)")
     ap.add_argument("--arena-gib", type=int, default=200)
@@ -120,7 +122,7 @@ def main():
         sess.release(m)
     tuned = None
     delta_plain = delta_batch = args.delta
-    long_weight = args.long_weight
+    long_weight, mid_weight = args.long_weight, args.mid_weight
     if not args.no_tune:
         from g500 import autotune
         t0 = time.perf_counter()
@@ -128,6 +130,7 @@ def main():
         delta_plain = tuned["sssp_delta"]
         delta_batch = tuned.get("batch_sssp_delta", delta_plain)
         long_weight = tuned.get("batch_sssp_long_weight", long_weight)
+        mid_weight = tuned.get("batch_sssp_mid_weight", mid_weight)
         log(f"[bench] autotuner ({time.perf_counter() - t0:.1f} s, untimed): {tuned}")
     roots = sess.roots(g)
     nroots = len(roots)
@@ -178,7 +181,7 @@ def main():
     else:
         bb = sess.batch_create(g)
         sb = sess.sssp_batch_create(g, delta_batch)
-        sb.long_weight = long_weight
+        sb.long_weight, sb.mid_weight = long_weight, mid_weight
         rsets = [list(range(k, min(k + BFS_GROUP, nroots))) for k in range(0, nroots, BFS_GROUP)]
 
         def batched_step(grp):
@@ -268,6 +271,7 @@ def main():
         "config": {"workload": workload, "scale": args.scale, "edgefactor": args.edgefactor,
                    "delta": delta_batch if args.mode == "batched" else delta_plain,
                    "plain_delta": delta_plain, "long_weight": long_weight if args.mode == "batched" else None,
+                   "mid_weight": mid_weight if args.mode == "batched" else None,
                    "roots": nroots, "mode": args.mode, "tuned": tuned},
         "bfs_gteps": round(me.sum() / tb.sum() / 1e9, 4),
         "sssp_gteps": round(me.sum() / ts.sum() / 1e9, 4),

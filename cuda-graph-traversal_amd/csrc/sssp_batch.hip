@@ -25,16 +25,18 @@
 //   dense step: a lane per vertex with open work reads its probes; roots with
 //     d in [lo, hi) become final and form the vertex's medium item, roots with
 //     d in [hi, fl(hi + delta)) are queued for the next bucket; after a barrier
-//     the medium items relax their medium edges (delta <= w < W, never into
+//     the medium items relax their medium edges (delta <= w < W_0, never into
 //     [lo, hi), as for one root);
-//   long edges (w >= W, the row's suffix) are deferred: the dense step ORs the
-//     settled roots into the vertex's deferred mask and queues the vertex once.
-//     A relaxation from a root settled at d >= e (e: the floor of the first
-//     bucket since the last flush) lands at fl(d + w) >= fl(e + W), so the
-//     deferred queue is flushed (its long edges relaxed for every collected
-//     root) only before a bucket reaching past fl(e + W) starts, and before a
-//     min step. A vertex settles a handful of roots per window instead of one
-//     per bucket, and each flushed entry probes one line for all of them;
+//   mid edges (W_0 <= w < W_1) and long edges (w >= W_1, the row's suffix)
+//     are deferred, each class c with its own queue: the dense step ORs the
+//     settled roots into the vertex's class-c mask and queues the vertex once.
+//     A class-c relaxation from a root settled at d >= e_c (e_c: the floor of
+//     the first bucket since the class's last flush) lands at
+//     fl(d + w) >= fl(e_c + W_c), so a class is flushed (its edges relaxed for
+//     every collected root) only before a bucket reaching past fl(e_c + W_c)
+//     starts, and before a min step. A vertex settles a handful of roots per
+//     window instead of one per bucket, and each flushed entry probes one line
+//     for all of them;
 //   when the next bucket is empty: a min step over all labels and a collect
 //     step queueing the bucket that holds the minimum.
 // Work is mapped by segment length: a lane per item (<= 32 entries), a wave
@@ -61,16 +63,17 @@ struct SsspBatchCtl {
     CounterLine<u64> fhub[3];     // light queue hub counters [slot]
     CounterLine<u32> hlen[3][2];  // medium item list lengths [slot][tier 0-1]
     CounterLine<u64> hhub[3];     // medium hub counters [slot]
-    CounterLine<u32> llen[2][2];  // deferred queue lengths [set][tier 0-1], set = flush parity
-    CounterLine<u64> lhub[2];     // deferred queue hub counters [set]
+    CounterLine<u32> llen[2][2][2];  // deferred queue lengths [class][set][tier 0-1], set = the class's flush parity
+    CounterLine<u64> lhub[2][2];     // deferred queue hub counters [class][set]
     alignas(256) u32 nmin[3];     // smallest distance bits >= hi, per slot
     u32 nnext[3];                 // 1: a label entered [hi, hi + delta) (the bucket is not empty), per slot
     alignas(256) u64 buckets, light_iters, improvements, scans;
     u64 flushes;
     u64 ticks[6];                 // constant-clock ticks of block 0: light, dense scan, medium, flush,
                                   // min+collect, kernel
-    u64 work[4][3];               // [light, medium, long][items, entries, (entry, root) pairs];
-                                  // [dense scans][open words walked, vertices reading their line, settling]
+    u64 work[5][3];               // [light, medium, long][items, entries, (entry, root) pairs];
+                                  // [dense scans][open words walked, vertices reading their line, settling];
+                                  // [mid][items, entries, pairs]
     // per-phase log (block 0): phase (0 light, 1 dense scan, 2 medium, 3 flush,
     // 4 min + collect), items processed, constant-clock ticks
     u32 nlog;
@@ -85,7 +88,7 @@ struct SsspBatchArgs {
     const u32 *col;
     const f32 *w;
     const u32 *split;
-    const u32 *lsplit;  // [n] entries with w < W (long edges are the rest)
+    const u32 *dsplit[2];  // [n] entries with w < W_c: deferred class c is [dsplit[c], dsplit[c+1]), class 1 to the row end
     const u32 *perm, *iperm;  // traversal <-> original ids (null: identity)
     u64 *lab;          // [n * kBatchRoots]
     u32 *dl;           // [n * kBatchRoots] probes (aliases dist)
@@ -102,18 +105,18 @@ struct SsspBatchArgs {
     u64 *hq_off;       //   work offsets,
     u32 *hq_len;       //   segment lengths,
     u32 *hq_mask;      //   root masks
-    u32 *lpend;        // [n] deferred root masks
-    u32 *lq[3];        // deferred queue tiers; tier 2 holds hub vertices
-    u64 *lq_off;       //   hub work offsets,
-    u32 *lq_len;       //   segment lengths,
-    u32 *lq_done;      //   finished chunks
+    u32 *dpend[2];     // [n] deferred root masks per class
+    u32 *dq[2][3];     // deferred queue tiers per class; tier 2 holds hub vertices
+    u64 *dq_off[2];    //   hub work offsets,
+    u32 *dq_len[2];    //   segment lengths,
+    u32 *dq_done[2];   //   finished chunks
     SsspBatchCtl *ctl;
     f32 *dist;         // [nroots * n], root-major
     i64 *pred;         // [nroots * n]
     u64 roots[kBatchRoots];
     i32 nroots;
     f32 delta;
-    f32 wlong;         // W >= delta
+    f32 dw[2];         // class floor weights W_0 (mid) <= W_1 (long), both >= delta
 };
 
 __device__ __forceinline__ void bsync(SsspBatchCtl *ctl, u32 phase, u32 a, u32 b) {
@@ -151,7 +154,7 @@ __device__ __forceinline__ u64 wave_index() {
 // atomic per block and phase instead of one per wave and relaxation round.
 constexpr u32 kQCap = 64;
 constexpr int kMaxBlockWaves = 512 / kWave;
-enum : int { kQLight0 = 0, kQLong0 = 2, kQ32 = 4, kQMed0 = 0, kQ64 = 2 };
+enum : int { kQLight0 = 0, kQDef0 = 2, kQ32 = 6, kQMed0 = 0, kQ64 = 2 };  // u32: light 0-1, class 0 2-3, class 1 4-5
 
 struct QLds {
     u32 v32[kMaxBlockWaves][kQ32][kQCap];
@@ -164,20 +167,23 @@ struct WaveQ {
     int w;                 // wave of the block
     u32 n[kQ32 + kQ64];    // staged entries per list (u32 lists first)
     int buf, slot;         // light queue buffer and counter slot of the phase's pushes
-    int hs, ls;            // medium list slot and deferred queue set of a dense scan
+    int hs;                // medium list slot of a dense scan
+    int ls[2];             // deferred queue set per class of a dense scan
 };
 
-// Destination array and counter of list l (u32 lists 0-3, u64 lists 4-5).
+// Destination array and counter of list l (u32 lists 0-5, u64 lists 6-7).
 __device__ __forceinline__ u32 *q_dst32(const SsspBatchArgs &a, const WaveQ &q, int l) {
-    // a select, not an index: a runtime index into the argument struct would
+    // selects, not indexes: a runtime index into the argument struct would
     // copy the struct to scratch
-    if (l < kQLong0) return q.buf ? a.fq[1][l] : a.fq[0][l];
-    return a.lq[l - kQLong0];
+    if (l < kQDef0) return q.buf ? a.fq[1][l] : a.fq[0][l];
+    if (l < kQDef0 + 2) return l == kQDef0 ? a.dq[0][0] : a.dq[0][1];
+    return l == kQDef0 + 2 ? a.dq[1][0] : a.dq[1][1];
 }
 __device__ __forceinline__ u32 *q_counter(const SsspBatchArgs &a, const WaveQ &q, int l) {
     SsspBatchCtl *ctl = a.ctl;
-    if (l < kQLong0) return &ctl->flen[q.slot][l].v;
-    if (l < kQ32) return &ctl->llen[q.ls][l - kQLong0].v;
+    if (l < kQDef0) return &ctl->flen[q.slot][l].v;
+    if (l < kQDef0 + 2) return &ctl->llen[0][q.ls[0]][l - kQDef0].v;
+    if (l < kQ32) return &ctl->llen[1][q.ls[1]][l - kQDef0 - 2].v;
     return &ctl->hlen[q.hs][l - kQ32].v;
 }
 
@@ -384,9 +390,10 @@ __device__ __forceinline__ int nth_root(u32 M, u32 i) {
 }
 
 // Row segments (rows are sorted by weight): light w < delta, medium
-// delta <= w < W, long w >= W. Light and long items take their root mask
-// from a pending array (cleared when read); medium items carry it.
-enum : int { kSegLight = 0, kSegMedium = 1, kSegLong = 2 };
+// delta <= w < W_0, mid W_0 <= w < W_1, long w >= W_1. Light, mid and long
+// items take their root mask from a pending array (cleared when read); medium
+// items carry it.
+enum : int { kSegLight = 0, kSegMedium = 1, kSegLong = 2, kSegMid = 3 };
 
 template <int K>
 __device__ __forceinline__ void bsegment(const SsspBatchArgs &a, u32 u, u64 &b, u64 &e) {
@@ -396,9 +403,12 @@ __device__ __forceinline__ void bsegment(const SsspBatchArgs &a, u32 u, u64 &b, 
         e = r0 + a.split[u];
     } else if (K == kSegMedium) {
         b = r0 + a.split[u];
-        e = r0 + a.lsplit[u];
+        e = r0 + a.dsplit[0][u];
+    } else if (K == kSegMid) {
+        b = r0 + a.dsplit[0][u];
+        e = r0 + a.dsplit[1][u];
     } else {
-        b = r0 + a.lsplit[u];
+        b = r0 + a.dsplit[1][u];
         e = a.rs[u + 1];
     }
 }
@@ -623,15 +633,15 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
     const u64 nwords = (a.n + 63) / 64;
     const u64 wave_gid = wave_index(), nwaves = gthreads / kWave;
     u64 improved = 0;
-    __shared__ u64 wks[4][3];  // work counts per segment kind and of the dense scans, this block
-    if (threadIdx.x < 12) wks[threadIdx.x / 3][threadIdx.x % 3] = 0;
+    __shared__ u64 wks[5][3];  // work counts per segment kind and of the dense scans, this block
+    if (threadIdx.x < 15) wks[threadIdx.x / 3][threadIdx.x % 3] = 0;
     __shared__ QLds qlds;
     WaveQ wq;
     wq.L = &qlds;
     wq.w = wave_in_block();
 #pragma unroll
     for (int l = 0; l < kQ32 + kQ64; ++l) wq.n[l] = 0;
-    wq.buf = wq.slot = wq.hs = wq.ls = 0;
+    wq.buf = wq.slot = wq.hs = wq.ls[0] = wq.ls[1] = 0;
     // the light queue tiers that a phase's pushes go to
     auto light_dest = [&](const BTarget &t) {
         wq.buf = t.buf;
@@ -640,11 +650,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
     __syncthreads();
     u32 x = 0;
     f32 lo = 0.0f, hi = next_bound(0.0f, a.delta);
-    // deferred long edges: flushes so far (selects the counter set), whether
-    // the queue holds work, and the floor of the first bucket since the flush
-    u32 nflush = 0;
-    bool lpending = false;
-    f32 epoch_lo = 0.0f;
+    // per deferred class (0 mid, 1 long): flushes so far (selects the counter
+    // set), whether the queue holds work, and the floor of the first bucket
+    // since the class's last flush
+    u32 nflush0 = 0, nflush1 = 0;
+    bool lpending0 = false, lpending1 = false;
+    f32 epoch0 = 0.0f, epoch1 = 0.0f;
 
     const bool timer = gtid == 0;
     // timer state of thread 0 lives in LDS: [0] previous tick, [1] start,
@@ -790,18 +801,35 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
             ++x;
         }
         if (ld_relaxed(&ctl->bar.error)) break;
-        // dense step x: medium items of [lo, hi), long edges deferred, next
-        // bucket [hi, hi2) queued
+        // dense step x: medium items of [lo, hi), mid and long edges deferred,
+        // next bucket [hi, hi2) queued
         if (gtid == 0) atomicAdd((unsigned long long *)&ctl->buckets, 1ull);
         const f32 hi2 = next_bound(hi, a.delta);
-        if (!lpending) epoch_lo = lo;
+        if (!lpending0) epoch0 = lo;
+        if (!lpending1) epoch1 = lo;
         {
             const BTarget t = target(x, hi2, hi);
             const int hs = (int)((x + 1) % 3);
-            const int ls = (int)(nflush & 1);
+            const int ls0 = (int)(nflush0 & 1), ls1 = (int)(nflush1 & 1);
             light_dest(t);
             wq.hs = hs;
-            wq.ls = ls;
+            wq.ls[0] = ls0;
+            wq.ls[1] = ls1;
+            // defers class c's segment of v (the lane owns v in this pass: no
+            // atomic needed): ORs H into its pending mask, queues v once
+            auto defer = [&](auto cls, u32 v, u32 H, u32 len) {
+                constexpr int c = decltype(cls)::value;
+                bool q = false;
+                if (H && len) {
+                    const u32 old = a.dpend[c][v];
+                    a.dpend[c][v] = old | H;
+                    q = old == 0;
+                }
+                q_push32<kQDef0 + 2 * c>(a, wq, q && btier(len) == 0, v);
+                q_push32<kQDef0 + 2 * c + 1>(a, wq, q && btier(len) == 1, v);
+                hub_append(&ctl->lhub[c][c ? ls1 : ls0].v, a.dq[c][2], a.dq_off[c], a.dq_len[c], q && btier(len) == 2,
+                           v, len, a.n, &ctl->bar.error);
+            };
             scan_open(lo, hi, hi2, [&](u64 word, u64 v, u32 H, u32 N, u32 dmin, bool act, u32 R) {
                 if (act) a.vmin[v] = dmin;  // the open roots left are those >= hi
                 {
@@ -819,21 +847,15 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                 if (cm && lane == 0) atomicAnd((unsigned long long *)&a.openw[word], ~(unsigned long long)cm);
                 push_item(a, (u32)v, N, t, wq);
                 u64 b = 0, e = 0;
-                u32 llen = 0;
-                bool lq = false;
+                u32 mlen = 0, llen = 0;
                 if (H) {
                     bsegment<kSegMedium>(a, (u32)v, b, e);
-                    llen = (u32)(a.rs[v + 1] - e);
-                    if (llen) {  // the lane owns v in this pass: no atomic needed
-                        const u32 old = a.lpend[v];
-                        a.lpend[v] = old | H;
-                        lq = old == 0;
-                    }
+                    const u64 e1 = a.rs[v] + a.dsplit[1][v];
+                    mlen = (u32)(e1 - e);
+                    llen = (u32)(a.rs[v + 1] - e1);
                 }
-                q_push32<kQLong0>(a, wq, lq && btier(llen) == 0, (u32)v);
-                q_push32<kQLong0 + 1>(a, wq, lq && btier(llen) == 1, (u32)v);
-                hub_append(&ctl->lhub[ls].v, a.lq[2], a.lq_off, a.lq_len, lq && btier(llen) == 2, (u32)v, llen, a.n,
-                           &ctl->bar.error);
+                defer(std::integral_constant<int, 0>(), (u32)v, H, mlen);
+                defer(std::integral_constant<int, 1>(), (u32)v, H, llen);
                 const u32 len = (u32)(e - b);
                 const bool item = H && len > 0;
                 q_push64<kQMed0>(a, wq, item && btier(len) == 0, ((u64)H << 32) | (u32)v);
@@ -844,11 +866,14 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
             });
             q_flush(a, wq);
             bsync(ctl, 3, __float_as_uint(lo), x);
-            // The deferred queue is read between the two barriers of this
-            // step: the next appends to this set come after the second one.
-            u32 lc[2];
-            for (int tier = 0; tier < 2; ++tier) lc[tier] = min(ld_relaxed(&ctl->llen[ls][tier].v), (u32)a.n);
-            const u64 lh = ld_relaxed(&ctl->lhub[ls].v);
+            // The deferred queues are read between the two barriers of this
+            // step: the next appends to these sets come after the second one.
+            u32 lc0[2], lc1[2];
+            for (int tier = 0; tier < 2; ++tier) {
+                lc0[tier] = min(ld_relaxed(&ctl->llen[0][ls0][tier].v), (u32)a.n);
+                lc1[tier] = min(ld_relaxed(&ctl->llen[1][ls1][tier].v), (u32)a.n);
+            }
+            const u64 lh0 = ld_relaxed(&ctl->lhub[0][ls0].v), lh1 = ld_relaxed(&ctl->lhub[1][ls1].v);
             tick(1);
             group_tier<kSegMedium>(a, nullptr, nullptr, a.hq[0], min(ld_relaxed(&ctl->hlen[hs][0].v), (u32)a.n), t,
                                    improved, slot, wks[1], wq);
@@ -860,30 +885,56 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
             bsync(ctl, 4, __float_as_uint(lo), x);
             tick(2, (u64)ld_relaxed(&ctl->hlen[hs][0].v) + ld_relaxed(&ctl->hlen[hs][1].v) +
                         hub_count(ld_relaxed(&ctl->hhub[hs].v)));
-            // Flush the deferred queue when the next bucket reaches past
-            // fl(epoch_lo + W), where its relaxations may land, or when the
+            // Flush a deferred class c when the next bucket reaches past
+            // fl(epoch_c + W_c), where its relaxations may land, or when the
             // next bucket looks empty (a min step must see every label).
-            lpending = lc[0] || lc[1] || hub_count(lh);
-            if (lpending) {
+            lpending0 = lc0[0] || lc0[1] || hub_count(lh0);
+            lpending1 = lc1[0] || lc1[1] || hub_count(lh1);
+            if (lpending0 || lpending1) {
                 u32 qc[2];
                 u64 qh;
                 const bool next_empty =
                     queue_size(x + 1, qc, &qh) == 0 && ld_relaxed(&ctl->nnext[(x + 1) % 3]) == 0;
-                if (hi2 > epoch_lo + a.wlong || next_empty) {
-                    if (gtid == 0) {  // the other set was last read by the previous flush
-                        st_relaxed(&ctl->llen[ls ^ 1][0].v, 0u);
-                        st_relaxed(&ctl->llen[ls ^ 1][1].v, 0u);
-                        st_relaxed(&ctl->lhub[ls ^ 1].v, 0ull);
+                const bool f0 = lpending0 && (hi2 > epoch0 + a.dw[0] || next_empty);
+                const bool f1 = lpending1 && (hi2 > epoch1 + a.dw[1] || next_empty);
+                if (f0 || f1) {
+                    if (gtid == 0) {  // a class's other set was last read by its previous flush
+                        if (f0) {
+                            st_relaxed(&ctl->llen[0][ls0 ^ 1][0].v, 0u);
+                            st_relaxed(&ctl->llen[0][ls0 ^ 1][1].v, 0u);
+                            st_relaxed(&ctl->lhub[0][ls0 ^ 1].v, 0ull);
+                        }
+                        if (f1) {
+                            st_relaxed(&ctl->llen[1][ls1 ^ 1][0].v, 0u);
+                            st_relaxed(&ctl->llen[1][ls1 ^ 1][1].v, 0u);
+                            st_relaxed(&ctl->lhub[1][ls1 ^ 1].v, 0ull);
+                        }
                         atomicAdd((unsigned long long *)&ctl->flushes, 1ull);
                     }
-                    group_tier<kSegLong>(a, a.lpend, a.lq[0], nullptr, lc[0], t, improved, slot, wks[2], wq);
-                    wave_tier<kSegLong>(a, a.lpend, a.lq[1], nullptr, lc[1], t, improved, wks[2], wq);
-                    hub_tier<kSegLong>(a, a.lq[2], a.lq_off, a.lq_len, a.lq_done, a.lpend, lh, t, improved, wks[2], wq);
+                    if (f0) {
+                        group_tier<kSegMid>(a, a.dpend[0], a.dq[0][0], nullptr, lc0[0], t, improved, slot, wks[4], wq);
+                        wave_tier<kSegMid>(a, a.dpend[0], a.dq[0][1], nullptr, lc0[1], t, improved, wks[4], wq);
+                        hub_tier<kSegMid>(a, a.dq[0][2], a.dq_off[0], a.dq_len[0], a.dq_done[0], a.dpend[0], lh0, t,
+                                          improved, wks[4], wq);
+                    }
+                    if (f1) {
+                        group_tier<kSegLong>(a, a.dpend[1], a.dq[1][0], nullptr, lc1[0], t, improved, slot, wks[2], wq);
+                        wave_tier<kSegLong>(a, a.dpend[1], a.dq[1][1], nullptr, lc1[1], t, improved, wks[2], wq);
+                        hub_tier<kSegLong>(a, a.dq[1][2], a.dq_off[1], a.dq_len[1], a.dq_done[1], a.dpend[1], lh1, t,
+                                           improved, wks[2], wq);
+                    }
                     q_flush(a, wq);
                     bsync(ctl, 7, __float_as_uint(lo), x);
-                    tick(3, (u64)lc[0] + lc[1] + hub_count(lh));
-                    ++nflush;
-                    lpending = false;
+                    tick(3, (f0 ? (u64)lc0[0] + lc0[1] + hub_count(lh0) : 0) +
+                                (f1 ? (u64)lc1[0] + lc1[1] + hub_count(lh1) : 0));
+                    if (f0) {
+                        ++nflush0;
+                        lpending0 = false;
+                    }
+                    if (f1) {
+                        ++nflush1;
+                        lpending1 = false;
+                    }
                 }
             }
             if (gtid == 0) atomicAdd((unsigned long long *)&ctl->scans, 1ull);
@@ -953,7 +1004,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
     }
     if (lane == 0 && improved) atomicAdd((unsigned long long *)&ctl->improvements, (unsigned long long)improved);
     __syncthreads();
-    if (threadIdx.x < 12 && wks[threadIdx.x / 3][threadIdx.x % 3])
+    if (threadIdx.x < 15 && wks[threadIdx.x / 3][threadIdx.x % 3])
         atomicAdd((unsigned long long *)&ctl->work[threadIdx.x / 3][threadIdx.x % 3],
                   (unsigned long long)wks[threadIdx.x / 3][threadIdx.x % 3]);
     if (timer) {
@@ -979,8 +1030,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
 
 __global__ __launch_bounds__(256) void sssp_batch_clear_kernel(u64 *__restrict__ lab, u32 *__restrict__ dl,
                                                                u32 *__restrict__ pend, u32 *__restrict__ done,
-                                                               u32 *__restrict__ masks, u32 *__restrict__ lpend,
-                                                               u32 *__restrict__ ldone, u64 *__restrict__ openw,
+                                                               u32 *__restrict__ masks, u32 *__restrict__ mwork,
+                                                               u32 *__restrict__ lwork, u64 *__restrict__ openw,
                                                                u64 n) {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x; t < n * kBatchRoots; t += stride) {
@@ -991,9 +1042,11 @@ __global__ __launch_bounds__(256) void sssp_batch_clear_kernel(u64 *__restrict__
             done[t] = 0;
             masks[t] = t < n ? 0u : 0xFFFFFFFFu;  // final-root masks, then vmin
         }
-        if (t < n) {
-            lpend[t] = 0;
-            ldone[t] = 0;
+        if (t < n) {  // deferred pending masks and hub chunk counts of both classes
+            mwork[t] = 0;
+            mwork[5 * n + t] = 0;
+            lwork[t] = 0;
+            lwork[5 * n + t] = 0;
         }
         if (t < (n + 63) / 64) openw[t] = 0;
     }
@@ -1053,7 +1106,13 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
         fail("batched SSSP long_weight must be non-negative and finite (got %g)", (double)s->long_weight);
         return false;
     }
+    if (!(s->mid_weight >= 0.0f) || !(s->mid_weight < 3.0e38f)) {
+        fail("batched SSSP mid_weight must be non-negative and finite (got %g)", (double)s->mid_weight);
+        return false;
+    }
+    // class floors: delta <= W_0 (mid) <= W_1 (long)
     const f32 wlong = s->long_weight > s->delta ? s->long_weight : s->delta;
+    const f32 wmid = s->mid_weight < s->delta ? s->delta : (s->mid_weight > wlong ? wlong : s->mid_weight);
     const u64 n = g->nlocalverts;
     if (n > s->max_nlocalverts) {
         fail("batched SSSP state holds %llu vertices, graph has %llu", (unsigned long long)s->max_nlocalverts,
@@ -1064,15 +1123,18 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
     const hipStream_t st = library ? c.stream : stream;
     // the split depends on the graph too: a state reused across graphs recomputes it
     const TraversalView tv = traversal_view(g);
-    if (s->split_delta != s->delta || s->long_split_weight != wlong || s->split_graph != (const void *)tv.rs) {
+    if (s->split_delta != s->delta || s->long_split_weight != wlong || s->mid_split_weight != wmid ||
+        s->split_graph != (const void *)tv.rs) {
         compute_split(tv.rs, tv.w, n, s->delta, s->split, st);
+        compute_split(tv.rs, tv.w, n, wmid, s->mid_split, st);
         compute_split(tv.rs, tv.w, n, wlong, s->long_split, st);
         s->split_delta = s->delta;
+        s->mid_split_weight = wmid;
         s->long_split_weight = wlong;
         s->split_graph = (const void *)tv.rs;
     }
     sssp_batch_clear_kernel<<<grid_for(n * kBatchRoots, 256, (u64)c.num_cus * 64), 256, 0, st>>>(
-        s->labels, (u32 *)s->dist, s->pending, s->hub_aux, s->root_masks, s->long_work, s->long_work + 5 * n,
+        s->labels, (u32 *)s->dist, s->pending, s->hub_aux, s->root_masks, s->mid_work, s->long_work,
         s->open_words, n);
     G500_HIP(hipGetLastError());
     G500_HIP(hipMemsetAsync(s->ctl, 0, sizeof(SsspBatchCtl), st));
@@ -1084,12 +1146,19 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
     a.perm = tv.perm;
     a.iperm = tv.iperm;
     a.split = s->split;
-    a.lsplit = s->long_split;
-    a.lpend = s->long_work;
-    for (int t = 0; t < 3; ++t) a.lq[t] = s->long_work + (u64)(1 + t) * n;
-    a.lq_len = s->long_work + 4 * n;
-    a.lq_done = s->long_work + 5 * n;
-    a.lq_off = s->long_offsets;
+    // deferred classes: 0 mid, 1 long; work arrays [6n]: pending masks, queue
+    // tiers 0-2, hub lengths, hub chunk counts
+    a.dsplit[0] = s->mid_split;
+    a.dsplit[1] = s->long_split;
+    u32 *dwork[2] = {s->mid_work, s->long_work};
+    u64 *doff[2] = {s->mid_offsets, s->long_offsets};
+    for (int d = 0; d < 2; ++d) {
+        a.dpend[d] = dwork[d];
+        for (int t = 0; t < 3; ++t) a.dq[d][t] = dwork[d] + (u64)(1 + t) * n;
+        a.dq_len[d] = dwork[d] + 4 * n;
+        a.dq_done[d] = dwork[d] + 5 * n;
+        a.dq_off[d] = doff[d];
+    }
     a.lab = s->labels;
     a.dl = (u32 *)s->dist;
     a.smask = s->root_masks;
@@ -1114,7 +1183,8 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
     for (i32 i = 0; i < nroots; ++i) a.roots[i] = roots[i];
     a.nroots = nroots;
     a.delta = s->delta;
-    a.wlong = wlong;
+    a.dw[0] = wmid;
+    a.dw[1] = wlong;
     const int block = ctx().sssp_block;
     if (grid <= 0) grid = cooperative_blocks((const void *)sssp_batch_kernel, block, 0);
     if (failed()) return false;
@@ -1152,7 +1222,8 @@ Sssp_Batch_State oned_graph_sssp_batch_create(Worker_State *ws, const Oned_Graph
     const u64 n = g->nlocalverts;
     s.max_nlocalverts = n;
     s.delta = 0.1f;
-    s.long_weight = 0.2f;
+    s.long_weight = 0.25f;
+    s.mid_weight = 0.075f;
     s.dist = (f32 *)arena_alloc(ws, kBatchRoots * n * sizeof(f32), "batched SSSP dist");
     s.pred = (i64 *)arena_alloc(ws, kBatchRoots * n * sizeof(i64), "batched SSSP pred");
     s.labels = (u64 *)arena_alloc(ws, kBatchRoots * n * sizeof(u64), "batched SSSP labels");
@@ -1166,6 +1237,9 @@ Sssp_Batch_State oned_graph_sssp_batch_create(Worker_State *ws, const Oned_Graph
     s.split = (u32 *)arena_alloc(ws, n * sizeof(u32), "batched SSSP light split");
     s.root_masks = (u32 *)arena_alloc(ws, 2 * n * sizeof(u32), "batched SSSP final-root masks and smallest open probes");
     s.open_words = (u64 *)arena_alloc(ws, ((n + 63) / 64) * sizeof(u64), "batched SSSP open-vertex bitmap");
+    s.mid_split = (u32 *)arena_alloc(ws, n * sizeof(u32), "batched SSSP mid split");
+    s.mid_work = (u32 *)arena_alloc(ws, 6 * n * sizeof(u32), "batched SSSP deferred mid-edge queue");
+    s.mid_offsets = (u64 *)arena_alloc(ws, n * sizeof(u64), "batched SSSP deferred mid hub offsets");
     s.long_split = (u32 *)arena_alloc(ws, n * sizeof(u32), "batched SSSP long split");
     s.long_work = (u32 *)arena_alloc(ws, 6 * n * sizeof(u32), "batched SSSP deferred long-edge queue");
     s.long_offsets = (u64 *)arena_alloc(ws, n * sizeof(u64), "batched SSSP deferred hub offsets");
@@ -1219,7 +1293,7 @@ i32 g500_sssp_batch_step_log(const Sssp_Batch_State *s, i32 *kind, u64 *items, f
 void g500_sssp_batch_work(const Sssp_Batch_State *s, u64 *out) {
     static SsspBatchCtl h;
     g500_copy_to_host(&h, s->ctl, sizeof(h));
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < 5; ++k)
         for (int j = 0; j < 3; ++j) out[k * 3 + j] = h.work[k][j];
 }
 

@@ -75,10 +75,12 @@ i32 g500_bfs_batch_level_log(const Bfs_Batch_State *b, i32 *modes, u64 *records,
 /* ---- batched delta-stepping SSSP ---------------------------------------- */
 /* Up to 32 roots in one persistent kernel with one bucket sequence shared by
  * all roots; items are (vertex, root mask) pairs and the 32 distance probes of
- * a vertex share one 128-byte line. Edges with w >= long_weight are relaxed in
- * deferred batches: their relaxations land at least long_weight above the
- * bucket that settled the source, so the kernel collects the roots a vertex
- * settles over that window and relaxes its long edges once for all of them.
+ * a vertex share one 128-byte line. Edges with w >= mid_weight are relaxed in
+ * deferred batches, in two classes: mid (mid_weight <= w < long_weight) and
+ * long (w >= long_weight). A class-c relaxation lands at least W_c above the
+ * bucket that settled its source, so the kernel collects the roots a vertex
+ * settles over a W_c-wide window and relaxes the class's edges of the vertex
+ * once for all of them.
  * No counterpart in the reference (its runner solves one root at a time,
  * graph500_runner.cu:393-416); each root's distances equal the single-root
  * oned_graph_sssp_run's (the FP32 fixed point is unique), and its parents
@@ -100,7 +102,11 @@ struct Sssp_Batch_State {
     u32 *root_masks;    /* [2n] per vertex: roots whose label is final, then the smallest distance
                            bits among the roots not yet final */
     u64 *open_words;    /* [ceil(n/64)] vertices with a reached root not yet final */
-    u32 *long_split;    /* [n] entries with w < max(long_weight, delta) per row */
+    u32 *mid_split;     /* [n] entries with w < W_mid per row (W_mid = mid_weight clamped to [delta, W_long]) */
+    u32 *mid_work;      /* [6n] deferred mid edges: pending root masks, queue tiers 0-2, hub lengths,
+                           hub chunk counts */
+    u64 *mid_offsets;   /* [n] hub work offsets of the deferred mid queue */
+    u32 *long_split;    /* [n] entries with w < W_long per row (W_long = max(long_weight, delta)) */
     u32 *long_work;     /* [6n] deferred long edges: pending root masks, queue tiers 0-2, hub lengths,
                            hub chunk counts */
     u64 *long_offsets;  /* [n] hub work offsets of the deferred queue */
@@ -108,8 +114,10 @@ struct Sssp_Batch_State {
     const void *split_graph; /* row offsets that `split` / `long_split` were computed for */
     f32 delta;          /* bucket width (default 0.1f) */
     f32 split_delta;
-    f32 long_weight;    /* deferral threshold (default 0.2f; <= delta: no deferral window) */
+    f32 long_weight;    /* long-edge threshold (default 0.25f; <= delta: no deferral window) */
     f32 long_split_weight;
+    f32 mid_weight;     /* mid-edge threshold (default 0.075f; <= delta: no mid class) */
+    f32 mid_split_weight;
     i32 nroots;         /* roots of the last run */
     i32 reserved;
     u64 max_nlocalverts;
@@ -128,12 +136,12 @@ void g500_sssp_batch_run_stats(const Sssp_Batch_State *s, u64 *buckets, u64 *lig
  * passes, ms[5] = deferred-queue flushes; ms[6..8] = the dense steps' scan,
  * medium and flush parts. */
 void g500_sssp_batch_phase_ms(const Sssp_Batch_State *s, f64 ms[9]);
-/* Work of the last batched run per row segment k (0 light, 1 medium, 2 long):
- * out[3k + 0] items, out[3k + 1] row entries visited, out[3k + 2]
+/* Work of the last batched run per row segment k (0 light, 1 medium, 2 long,
+ * 4 mid): out[3k + 0] items, out[3k + 1] row entries visited, out[3k + 2]
  * (entry, root) relaxations; out[9..11] of the dense scans: open 64-vertex
  * words walked, vertices that read their probe line, vertices settling a
  * root. */
-void g500_sssp_batch_work(const Sssp_Batch_State *s, u64 out[12]);
+void g500_sssp_batch_work(const Sssp_Batch_State *s, u64 out[15]);
 /* Per-phase log of the last batched run (first 4096 phases): kind (0 light
  * step, 1 dense scan, 2 medium relaxations, 3 deferred flush, 4 min + collect
  * steps), items processed and device milliseconds. Returns the entries. */

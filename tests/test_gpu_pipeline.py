@@ -373,14 +373,16 @@ def test_sssp_batch_bitexact(arena, oracle, scale):
     s = arena.sssp_batch_create(g)
     roots = oracle.roots(ref)
     width = int(arena.get_param("sssp_batch_max"))
-    # (delta, long_weight): default deferral window, W below delta (no window),
-    # narrow and wide windows, no deferral at all (W = 0)
-    for delta, wl in ((0.1, 0.2), (0.25, 0.2), (0.02, 0.2), (0.02, 0.05), (0.02, 0.0), (0.02, 0.7)):
-        s.delta, s.long_weight = delta, wl
-        say("sssp batch", scale, delta, wl)
+    # (delta, mid_weight, long_weight): both deferral classes, mid below delta
+    # (no mid class), both below delta (no window), narrow and wide windows,
+    # no deferral at all, mid above long (clamped: no mid class)
+    for delta, wm, wl in ((0.02, 0.05, 0.2), (0.1, 0.05, 0.2), (0.25, 0.05, 0.2), (0.02, 0.0, 0.05),
+                          (0.02, 0.0, 0.0), (0.02, 0.1, 0.7), (0.02, 0.5, 0.2)):
+        s.delta, s.mid_weight, s.long_weight = delta, wm, wl
+        say("sssp batch", scale, delta, wm, wl)
         sssp_batch_check(arena, oracle, g, ref, s, roots[:width])
     # fewer roots, repeats, an isolated vertex as a root
-    s.delta, s.long_weight = 0.1, 0.2
+    s.delta, s.mid_weight, s.long_weight = 0.1, 0.05, 0.2
     isolated = int(np.flatnonzero(np.diff(ref.rowstarts.astype(np.int64)) == 0)[0])
     some = np.array([roots[20], roots[20], isolated, roots[17]], np.uint64)
     say("sssp batch repeats", scale, some.tolist())
@@ -398,15 +400,15 @@ def test_sssp_batch_matches_plain(arena, oracle):
     for r in roots:
         arena.sssp(g, sp, int(r))
         plain.append(arena.sssp_result(sp)[0])
-    for delta, wl in ((0.1, 0.2), (0.0125, 0.25)):
-        sb.delta, sb.long_weight = delta, wl
+    for delta, wm, wl in ((0.1, 0.05, 0.2), (0.0125, 0.05, 0.25)):
+        sb.delta, sb.mid_weight, sb.long_weight = delta, wm, wl
         arena.sssp_batch(g, sb, roots)
         flushes = arena.sssp_batch_phase_ms(sb)[5]
-        say("sssp batch S18", delta, wl, "flushes", flushes)
+        say("sssp batch S18", delta, wm, wl, "flushes", flushes)
         assert flushes > 0
         for i, r in enumerate(roots):
             d, p = arena.sssp_batch_result(g, sb, i)
-            assert np.array_equal(d.view(np.uint32), plain[i].view(np.uint32)), (delta, wl, i)
+            assert np.array_equal(d.view(np.uint32), plain[i].view(np.uint32)), (delta, wm, wl, i)
             assert oracle.sssp_check(ref, int(r), d, p) == 0
 
 
@@ -421,6 +423,11 @@ def test_sssp_batch_errors_return(arena):
             arena.sssp_batch(g, s, np.array(roots, np.uint64))
     for wl in (-1.0, float("nan"), float("inf")):
         s.long_weight = wl
+        with pytest.raises(g500.LibraryError):
+            arena.sssp_batch(g, s, np.array([1], np.uint64))
+    s.long_weight = 0.2
+    for wm in (-1.0, float("nan"), float("inf")):
+        s.mid_weight = wm
         with pytest.raises(g500.LibraryError):
             arena.sssp_batch(g, s, np.array([1], np.uint64))
 
@@ -465,7 +472,7 @@ def test_autotuner_applies_candidates(arena, oracle):
         p = out["params"]
         assert (p["bfs_alpha"], p["bfs_beta"]) in autotune.BFS_SWITCH and p["bfs_block"] in autotune.BLOCKS
         assert p["sssp_delta"] in autotune.SSSP_DELTA
-        assert (p["batch_sssp_delta"], p["batch_sssp_long_weight"]) in autotune.BATCH_SSSP
+        assert (p["batch_sssp_delta"], p["batch_sssp_mid_weight"], p["batch_sssp_long_weight"]) in autotune.BATCH_SSSP
         assert p["batch_alpha"] in autotune.BATCH_ALPHA and p["batch_heavy_degree"] in autotune.BATCH_HEAVY
         for k in keys:
             assert arena.get_param(k) == p[k]
@@ -473,7 +480,7 @@ def test_autotuner_applies_candidates(arena, oracle):
         arena.bfs_state = arena.bfs_create(g)
         bfs_check(arena, oracle, g, ref, int(roots[0]))
         sb = arena.sssp_batch_create(g, p["batch_sssp_delta"])
-        sb.long_weight = p["batch_sssp_long_weight"]
+        sb.mid_weight, sb.long_weight = p["batch_sssp_mid_weight"], p["batch_sssp_long_weight"]
         sssp_batch_check(arena, oracle, g, ref, sb, roots[:16])
     finally:
         for k, v in old.items():
