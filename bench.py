@@ -90,6 +90,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=512,
                     help="requests of the serving leg (0: skip); resident graphs S16-S24")
     ap.add_argument("--serve-rate", type=float, default=1000.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-sweep", default="300,5000",
+                    help="further offered rates (comma-separated, empty for none) reported in serving_sweep")
     ap.add_argument("--calibrate-mib", type=int, default=0,
                     help="also run the device copy kernel on this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -290,10 +292,22 @@ def main():
         from g500 import serving
         log("[bench] serving leg: loading 8 resident graphs")
         res = serving.load_graphs(sess)
+        m = sess.mark()
         out["serving"] = serving.poisson_run(sess, res, args.serve_rate, args.serve_requests)
+        sess.release(m)
         log(f"[bench] serving: {out['serving']}")
+        sweep = []
+        for rate in [float(x) for x in args.serve_sweep.split(",") if x.strip()]:
+            m = sess.mark()  # each run's server states are released before the next
+            r = serving.poisson_run(sess, res, rate, args.serve_requests)
+            sess.release(m)
+            sweep.append({k: r[k] for k in ("rate_offered", "completed", "p50_ms", "p99_ms", "req_per_s", "gteps",
+                                            "mean_batch")})
+            log(f"[bench] serving at {rate:g} req/s: {sweep[-1]}")
+        out["serving_sweep"] = sweep
     else:
         out["serving"] = None
+        out["serving_sweep"] = []
     if not args.no_cpu:
         nthreads = min(16, len(os.sched_getaffinity(0)))
         log(f"[bench] CPU baseline sample ({nthreads} threads)")
