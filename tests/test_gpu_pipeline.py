@@ -412,6 +412,29 @@ def test_sssp_batch_matches_plain(arena, oracle):
             assert oracle.sssp_check(ref, int(r), d, p) == 0
 
 
+def test_sssp_batch_diagnostics(arena, oracle):
+    """Work counters and the phase log of a batched run are consistent."""
+    e, w, ref = oracle_graph(14)
+    g = arena.csr(arena.upload_tuples(e, w), 1 << 14)
+    s = arena.sssp_batch_create(g, 0.02)
+    s.mid_weight, s.long_weight = 0.05, 0.2
+    roots = arena.roots(g)[:int(arena.get_param("sssp_batch_max"))]
+    arena.sssp_batch(g, s, roots)
+    wk = arena.sssp_batch_work(s)
+    for seg in ("light", "medium", "mid", "long"):
+        items, entries, pairs = wk[seg]
+        assert entries >= items and pairs >= entries, (seg, wk[seg])
+    assert wk["medium"][1] > 0 and wk["mid"][1] > 0 and wk["long"][1] > 0
+    kind, items, ms = arena.sssp_batch_step_log(s)
+    ph = arena.sssp_batch_phase_ms(s)
+    b, li, imp = arena.sssp_batch_stats(s)
+    assert len(kind) > 0 and set(kind) <= {0, 1, 2, 3, 4}
+    assert kind.count(1) == b and kind.count(2) == b  # a dense scan and a medium phase per bucket
+    assert kind.count(3) == int(ph[5])               # one log entry per flush phase
+    assert kind.count(0) == li and imp > 0
+    assert abs(sum(ms) - (ph[0] + ph[1] + ph[2])) < 0.05 * ph[3] + 0.1
+
+
 def test_sssp_batch_errors_return(arena):
     import g500
     e, w, ref = oracle_graph(10)
