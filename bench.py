@@ -82,6 +82,8 @@ def main():
                     help="batched SSSP long-edge threshold (deferred over windows this wide) when not tuned")
     ap.add_argument("--mid-weight", type=float, default=0.075,
                     help="batched SSSP mid-edge threshold (deferred over windows this wide) when not tuned")
+    ap.add_argument("--far-weight", type=float, default=0.5,
+                    help="batched SSSP far-edge threshold (deferred over windows this wide) when not tuned")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (knobs at their defaults, delta from --delta)")
     ap.add_argument("--arena-gib", type=int, default=200)
@@ -124,7 +126,7 @@ def main():
         sess.release(m)
     tuned = None
     delta_plain = delta_batch = args.delta
-    long_weight, mid_weight = args.long_weight, args.mid_weight
+    long_weight, mid_weight, far_weight = args.long_weight, args.mid_weight, args.far_weight
     if not args.no_tune:
         from g500 import autotune
         t0 = time.perf_counter()
@@ -133,6 +135,7 @@ def main():
         delta_batch = tuned.get("batch_sssp_delta", delta_plain)
         long_weight = tuned.get("batch_sssp_long_weight", long_weight)
         mid_weight = tuned.get("batch_sssp_mid_weight", mid_weight)
+        far_weight = tuned.get("batch_sssp_far_weight", far_weight)
         log(f"[bench] autotuner ({time.perf_counter() - t0:.1f} s, untimed): {tuned}")
     roots = sess.roots(g)
     nroots = len(roots)
@@ -183,7 +186,7 @@ def main():
     else:
         bb = sess.batch_create(g)
         sb = sess.sssp_batch_create(g, delta_batch)
-        sb.long_weight, sb.mid_weight = long_weight, mid_weight
+        sb.long_weight, sb.mid_weight, sb.far_weight = long_weight, mid_weight, far_weight
         rsets = [list(range(k, min(k + BFS_GROUP, nroots))) for k in range(0, nroots, BFS_GROUP)]
 
         def batched_step(grp):
@@ -274,6 +277,7 @@ def main():
                    "delta": delta_batch if args.mode == "batched" else delta_plain,
                    "plain_delta": delta_plain, "long_weight": long_weight if args.mode == "batched" else None,
                    "mid_weight": mid_weight if args.mode == "batched" else None,
+                   "far_weight": far_weight if args.mode == "batched" else None,
                    "roots": nroots, "mode": args.mode, "tuned": tuned},
         "bfs_gteps": round(me.sum() / tb.sum() / 1e9, 4),
         "sssp_gteps": round(me.sum() / ts.sum() / 1e9, 4),

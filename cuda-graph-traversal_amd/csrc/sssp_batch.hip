@@ -52,6 +52,7 @@ constexpr int kBatchRoots = kSsspBatchRoots;  // label stride and largest batch
 constexpr u32 kBSmall = 32;
 constexpr u32 kBHub = 1024;
 constexpr u32 kBStepLog = 4096;
+constexpr int kDefer = 3;  // deferred edge classes: 0 mid, 1 long, 2 far
 
 // Error bits: 1 barrier timeout, 2 hub list overflow, 4 queue entry out of
 // range, 8 queue length above n, 16 bucket invariant violated.
@@ -63,17 +64,17 @@ struct SsspBatchCtl {
     CounterLine<u64> fhub[3];     // light queue hub counters [slot]
     CounterLine<u32> hlen[3][2];  // medium item list lengths [slot][tier 0-1]
     CounterLine<u64> hhub[3];     // medium hub counters [slot]
-    CounterLine<u32> llen[2][2][2];  // deferred queue lengths [class][set][tier 0-1], set = the class's flush parity
-    CounterLine<u64> lhub[2][2];     // deferred queue hub counters [class][set]
+    CounterLine<u32> llen[kDefer][2][2];  // deferred queue lengths [class][set][tier 0-1], set = the class's flush parity
+    CounterLine<u64> lhub[kDefer][2];     // deferred queue hub counters [class][set]
     alignas(256) u32 nmin[3];     // smallest distance bits >= hi, per slot
     u32 nnext[3];                 // 1: a label entered [hi, hi + delta) (the bucket is not empty), per slot
     alignas(256) u64 buckets, light_iters, improvements, scans;
     u64 flushes;
     u64 ticks[6];                 // constant-clock ticks of block 0: light, dense scan, medium, flush,
                                   // min+collect, kernel
-    u64 work[5][3];               // [light, medium, long][items, entries, (entry, root) pairs];
+    u64 work[3 + kDefer][3];      // [light, medium][items, entries, (entry, root) pairs];
                                   // [dense scans][open words walked, vertices reading their line, settling];
-                                  // [mid][items, entries, pairs]
+                                  // [3 + deferred class][items, entries, pairs]
     // per-phase log (block 0): phase (0 light, 1 dense scan, 2 medium, 3 flush,
     // 4 min + collect), items processed, constant-clock ticks
     u32 nlog;
@@ -88,7 +89,7 @@ struct SsspBatchArgs {
     const u32 *col;
     const f32 *w;
     const u32 *split;
-    const u32 *dsplit[2];  // [n] entries with w < W_c: deferred class c is [dsplit[c], dsplit[c+1]), class 1 to the row end
+    const u32 *dsplit[kDefer];  // [n] entries with w < W_c: class c is [dsplit[c], dsplit[c+1]), the last to the row end
     const u32 *perm, *iperm;  // traversal <-> original ids (null: identity)
     u64 *lab;          // [n * kBatchRoots]
     u32 *dl;           // [n * kBatchRoots] probes (aliases dist)
@@ -105,18 +106,18 @@ struct SsspBatchArgs {
     u64 *hq_off;       //   work offsets,
     u32 *hq_len;       //   segment lengths,
     u32 *hq_mask;      //   root masks
-    u32 *dpend[2];     // [n] deferred root masks per class
-    u32 *dq[2][3];     // deferred queue tiers per class; tier 2 holds hub vertices
-    u64 *dq_off[2];    //   hub work offsets,
-    u32 *dq_len[2];    //   segment lengths,
-    u32 *dq_done[2];   //   finished chunks
+    u32 *dpend[kDefer];     // [n] deferred root masks per class
+    u32 *dq[kDefer][3];     // deferred queue tiers per class; tier 2 holds hub vertices
+    u64 *dq_off[kDefer];    //   hub work offsets,
+    u32 *dq_len[kDefer];    //   segment lengths,
+    u32 *dq_done[kDefer];   //   finished chunks
     SsspBatchCtl *ctl;
     f32 *dist;         // [nroots * n], root-major
     i64 *pred;         // [nroots * n]
     u64 roots[kBatchRoots];
     i32 nroots;
     f32 delta;
-    f32 dw[2];         // class floor weights W_0 (mid) <= W_1 (long), both >= delta
+    f32 dw[kDefer];    // class floor weights delta <= W_0 (mid) <= W_1 (long) <= W_2 (far)
 };
 
 __device__ __forceinline__ void bsync(SsspBatchCtl *ctl, u32 phase, u32 a, u32 b) {
@@ -154,7 +155,7 @@ __device__ __forceinline__ u64 wave_index() {
 // atomic per block and phase instead of one per wave and relaxation round.
 constexpr u32 kQCap = 64;
 constexpr int kMaxBlockWaves = 512 / kWave;
-enum : int { kQLight0 = 0, kQDef0 = 2, kQ32 = 6, kQMed0 = 0, kQ64 = 2 };  // u32: light 0-1, class 0 2-3, class 1 4-5
+enum : int { kQLight0 = 0, kQDef0 = 2, kQ32 = 2 + 2 * kDefer, kQMed0 = 0, kQ64 = 2 };  // u32: light 0-1, class c 2+2c..3+2c
 
 struct QLds {
     u32 v32[kMaxBlockWaves][kQ32][kQCap];
@@ -168,23 +169,31 @@ struct WaveQ {
     u32 n[kQ32 + kQ64];    // staged entries per list (u32 lists first)
     int buf, slot;         // light queue buffer and counter slot of the phase's pushes
     int hs;                // medium list slot of a dense scan
-    int ls[2];             // deferred queue set per class of a dense scan
+    int ls[kDefer];        // deferred queue set per class of a dense scan
 };
 
-// Destination array and counter of list l (u32 lists 0-5, u64 lists 6-7).
+// Destination array and counter of list l (u32 lists 0..kQ32-1, u64 lists
+// after). Selects over constant indexes, not runtime ones: a runtime index into
+// the argument struct or a register array would move it to scratch.
 __device__ __forceinline__ u32 *q_dst32(const SsspBatchArgs &a, const WaveQ &q, int l) {
-    // selects, not indexes: a runtime index into the argument struct would
-    // copy the struct to scratch
-    if (l < kQDef0) return q.buf ? a.fq[1][l] : a.fq[0][l];
-    if (l < kQDef0 + 2) return l == kQDef0 ? a.dq[0][0] : a.dq[0][1];
-    return l == kQDef0 + 2 ? a.dq[1][0] : a.dq[1][1];
+    if (l < kQDef0) return q.buf ? (l ? a.fq[1][1] : a.fq[1][0]) : (l ? a.fq[0][1] : a.fq[0][0]);
+    u32 *p = nullptr;
+#pragma unroll
+    for (int c = 0; c < kDefer; ++c) {
+        if (l == kQDef0 + 2 * c) p = a.dq[c][0];
+        if (l == kQDef0 + 2 * c + 1) p = a.dq[c][1];
+    }
+    return p;
 }
 __device__ __forceinline__ u32 *q_counter(const SsspBatchArgs &a, const WaveQ &q, int l) {
     SsspBatchCtl *ctl = a.ctl;
     if (l < kQDef0) return &ctl->flen[q.slot][l].v;
-    if (l < kQDef0 + 2) return &ctl->llen[0][q.ls[0]][l - kQDef0].v;
-    if (l < kQ32) return &ctl->llen[1][q.ls[1]][l - kQDef0 - 2].v;
-    return &ctl->hlen[q.hs][l - kQ32].v;
+    if (l >= kQ32) return &ctl->hlen[q.hs][l - kQ32].v;
+    u32 *p = nullptr;
+#pragma unroll
+    for (int c = 0; c < kDefer; ++c)
+        if ((l - kQDef0) >> 1 == c) p = &ctl->llen[c][q.ls[c]][(l - kQDef0) & 1].v;
+    return p;
 }
 
 // Writes the staged entries [0, cnt) of one list at dst[base...].
@@ -280,12 +289,13 @@ __device__ __forceinline__ void push_item(const SsspBatchArgs &a, u32 v, u32 mas
     u32 lw = 0;
     if (mask) {
         lw = a.split[v];
-        if (lw) q = atomicOr(&a.pend[t.buf][v], mask) == 0;
+        if (lw) q = atomicOr(&(t.buf ? a.pend[1] : a.pend[0])[v], mask) == 0;
     }
     if (!ballot(q)) return;
     q_push32<kQLight0>(a, wq, q && btier(lw) == 0, v);
     q_push32<kQLight0 + 1>(a, wq, q && btier(lw) == 1, v);
-    hub_append(&ctl->fhub[t.slot].v, a.fq[t.buf][2], a.fq_off[t.buf], a.fq_len[t.buf], q && btier(lw) == 2, v, lw,
+    hub_append(&ctl->fhub[t.slot].v, t.buf ? a.fq[1][2] : a.fq[0][2], t.buf ? a.fq_off[1] : a.fq_off[0],
+               t.buf ? a.fq_len[1] : a.fq_len[0], q && btier(lw) == 2, v, lw,
                a.n, &ctl->bar.error);
 }
 
@@ -390,10 +400,10 @@ __device__ __forceinline__ int nth_root(u32 M, u32 i) {
 }
 
 // Row segments (rows are sorted by weight): light w < delta, medium
-// delta <= w < W_0, mid W_0 <= w < W_1, long w >= W_1. Light, mid and long
-// items take their root mask from a pending array (cleared when read); medium
-// items carry it.
-enum : int { kSegLight = 0, kSegMedium = 1, kSegLong = 2, kSegMid = 3 };
+// delta <= w < W_0, and deferred class c W_c <= w < W_{c+1} (the last class
+// to the row end): segment kSegDef + c. Light and deferred items take their
+// root mask from a pending array (cleared when read); medium items carry it.
+enum : int { kSegLight = 0, kSegMedium = 1, kSegDef = 2 };
 
 template <int K>
 __device__ __forceinline__ void bsegment(const SsspBatchArgs &a, u32 u, u64 &b, u64 &e) {
@@ -404,12 +414,10 @@ __device__ __forceinline__ void bsegment(const SsspBatchArgs &a, u32 u, u64 &b, 
     } else if (K == kSegMedium) {
         b = r0 + a.split[u];
         e = r0 + a.dsplit[0][u];
-    } else if (K == kSegMid) {
-        b = r0 + a.dsplit[0][u];
-        e = r0 + a.dsplit[1][u];
     } else {
-        b = r0 + a.dsplit[1][u];
-        e = a.rs[u + 1];
+        constexpr int c = K - kSegDef;
+        b = r0 + a.dsplit[c][u];
+        e = c + 1 < kDefer ? r0 + a.dsplit[c + 1 < kDefer ? c + 1 : c][u] : a.rs[u + 1];
     }
 }
 
@@ -633,15 +641,18 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
     const u64 nwords = (a.n + 63) / 64;
     const u64 wave_gid = wave_index(), nwaves = gthreads / kWave;
     u64 improved = 0;
-    __shared__ u64 wks[5][3];  // work counts per segment kind and of the dense scans, this block
-    if (threadIdx.x < 15) wks[threadIdx.x / 3][threadIdx.x % 3] = 0;
+    // work counts of this block: [light, medium, dense scans, deferred class c at 3 + c][3]
+    __shared__ u64 wks[3 + kDefer][3];
+    if (threadIdx.x < 3 * (3 + kDefer)) wks[threadIdx.x / 3][threadIdx.x % 3] = 0;
     __shared__ QLds qlds;
     WaveQ wq;
     wq.L = &qlds;
     wq.w = wave_in_block();
 #pragma unroll
     for (int l = 0; l < kQ32 + kQ64; ++l) wq.n[l] = 0;
-    wq.buf = wq.slot = wq.hs = wq.ls[0] = wq.ls[1] = 0;
+    wq.buf = wq.slot = wq.hs = 0;
+#pragma unroll
+    for (int c = 0; c < kDefer; ++c) wq.ls[c] = 0;
     // the light queue tiers that a phase's pushes go to
     auto light_dest = [&](const BTarget &t) {
         wq.buf = t.buf;
@@ -650,12 +661,18 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
     __syncthreads();
     u32 x = 0;
     f32 lo = 0.0f, hi = next_bound(0.0f, a.delta);
-    // per deferred class (0 mid, 1 long): flushes so far (selects the counter
-    // set), whether the queue holds work, and the floor of the first bucket
-    // since the class's last flush
-    u32 nflush0 = 0, nflush1 = 0;
-    bool lpending0 = false, lpending1 = false;
-    f32 epoch0 = 0.0f, epoch1 = 0.0f;
+    // per deferred class: flushes so far (selects the counter set), whether the
+    // queue holds work, and the floor of the first bucket since the class's
+    // last flush (indexed with constants only, so they stay in registers)
+    u32 nflush[kDefer];
+    bool lpending[kDefer];
+    f32 epoch[kDefer];
+#pragma unroll
+    for (int c = 0; c < kDefer; ++c) {
+        nflush[c] = 0;
+        lpending[c] = false;
+        epoch[c] = 0.0f;
+    }
 
     const bool timer = gtid == 0;
     // timer state of thread 0 lives in LDS: [0] previous tick, [1] start,
@@ -761,8 +778,14 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
         }
     };
 
+    // this thread's root (a select chain: a runtime index into the argument
+    // struct would copy it to scratch)
+    u64 my_root = 0;
+#pragma unroll
+    for (int i = 0; i < kBatchRoots; ++i)
+        if (gtid == (u64)i) my_root = a.roots[i];
     if (gtid < (u64)a.nroots) {
-        const u64 root = trav_id(a.perm, a.roots[gtid]);
+        const u64 root = trav_id(a.perm, my_root);
         st_relaxed(&a.lab[root * kBatchRoots + gtid], root);  // distance 0.0f, parent = root
         st_relaxed(&a.dl[root * kBatchRoots + gtid], 0u);
         st_relaxed(&a.vmin[root], 0u);
@@ -774,7 +797,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
     light_dest(target(0, 0.f, 0.f));
     if (gtid < kWave) {  // step 0 queues the roots (repeated roots share one item)
         const bool r = (i32)gtid < a.nroots;
-        push_item(a, r ? trav_id(a.perm, a.roots[gtid]) : 0u, r ? 1u << gtid : 0u, target(0, 0.f, 0.f), wq);
+        push_item(a, r ? trav_id(a.perm, my_root) : 0u, r ? 1u << gtid : 0u, target(0, 0.f, 0.f), wq);
     }
     q_flush(a, wq);
     bsync(ctl, 1, 1, 0);
@@ -791,9 +814,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
             const BTarget t = target(x, hi, lo);
             const int buf = (int)(x & 1);
             light_dest(t);
-            group_tier<kSegLight>(a, a.pend[buf], a.fq[buf][0], nullptr, cnt[0], t, improved, slot, wks[0], wq);
-            wave_tier<kSegLight>(a, a.pend[buf], a.fq[buf][1], nullptr, cnt[1], t, improved, wks[0], wq);
-            hub_tier<kSegLight>(a, a.fq[buf][2], a.fq_off[buf], a.fq_len[buf], a.fq_done[buf], a.pend[buf], hub, t,
+            // selects, not runtime indexes into the argument struct (which would copy it to scratch)
+            u32 *pb = buf ? a.pend[1] : a.pend[0];
+            group_tier<kSegLight>(a, pb, buf ? a.fq[1][0] : a.fq[0][0], nullptr, cnt[0], t, improved, slot, wks[0], wq);
+            wave_tier<kSegLight>(a, pb, buf ? a.fq[1][1] : a.fq[0][1], nullptr, cnt[1], t, improved, wks[0], wq);
+            hub_tier<kSegLight>(a, buf ? a.fq[1][2] : a.fq[0][2], buf ? a.fq_off[1] : a.fq_off[0],
+                                buf ? a.fq_len[1] : a.fq_len[0], buf ? a.fq_done[1] : a.fq_done[0], pb, hub, t,
                                 improved, wks[0], wq);
             q_flush(a, wq);
             bsync(ctl, 2, __float_as_uint(lo), x);
@@ -805,16 +831,16 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
         // next bucket [hi, hi2) queued
         if (gtid == 0) atomicAdd((unsigned long long *)&ctl->buckets, 1ull);
         const f32 hi2 = next_bound(hi, a.delta);
-        if (!lpending0) epoch0 = lo;
-        if (!lpending1) epoch1 = lo;
+#pragma unroll
+        for (int c = 0; c < kDefer; ++c)
+            if (!lpending[c]) epoch[c] = lo;
         {
             const BTarget t = target(x, hi2, hi);
             const int hs = (int)((x + 1) % 3);
-            const int ls0 = (int)(nflush0 & 1), ls1 = (int)(nflush1 & 1);
             light_dest(t);
             wq.hs = hs;
-            wq.ls[0] = ls0;
-            wq.ls[1] = ls1;
+#pragma unroll
+            for (int c = 0; c < kDefer; ++c) wq.ls[c] = (int)(nflush[c] & 1);
             // defers class c's segment of v (the lane owns v in this pass: no
             // atomic needed): ORs H into its pending mask, queues v once
             auto defer = [&](auto cls, u32 v, u32 H, u32 len) {
@@ -827,14 +853,14 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                 }
                 q_push32<kQDef0 + 2 * c>(a, wq, q && btier(len) == 0, v);
                 q_push32<kQDef0 + 2 * c + 1>(a, wq, q && btier(len) == 1, v);
-                hub_append(&ctl->lhub[c][c ? ls1 : ls0].v, a.dq[c][2], a.dq_off[c], a.dq_len[c], q && btier(len) == 2,
+                hub_append(&ctl->lhub[c][wq.ls[c]].v, a.dq[c][2], a.dq_off[c], a.dq_len[c], q && btier(len) == 2,
                            v, len, a.n, &ctl->bar.error);
             };
             scan_open(lo, hi, hi2, [&](u64 word, u64 v, u32 H, u32 N, u32 dmin, bool act, u32 R) {
                 if (act) a.vmin[v] = dmin;  // the open roots left are those >= hi
                 {
                     const u64 na = __popcll(ballot(act)), nh = __popcll(ballot(H != 0));
-                    if (lane == 0) work_add(wks[3], 1, na, nh);
+                    if (lane == 0) work_add(wks[2], 1, na, nh);
                 }
                 if (!ballot(H | N)) return;
                 // the bucket's roots of v are final now; v closes when no reached root is open
@@ -847,15 +873,21 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                 if (cm && lane == 0) atomicAnd((unsigned long long *)&a.openw[word], ~(unsigned long long)cm);
                 push_item(a, (u32)v, N, t, wq);
                 u64 b = 0, e = 0;
-                u32 mlen = 0, llen = 0;
+                u64 cut[kDefer + 1];  // class c is [cut[c], cut[c + 1])
                 if (H) {
                     bsegment<kSegMedium>(a, (u32)v, b, e);
-                    const u64 e1 = a.rs[v] + a.dsplit[1][v];
-                    mlen = (u32)(e1 - e);
-                    llen = (u32)(a.rs[v + 1] - e1);
+                    const u64 r0 = a.rs[v];
+#pragma unroll
+                    for (int c = 0; c < kDefer; ++c) cut[c] = r0 + a.dsplit[c][v];
+                    cut[kDefer] = a.rs[v + 1];
+                } else {
+#pragma unroll
+                    for (int c = 0; c <= kDefer; ++c) cut[c] = 0;
                 }
-                defer(std::integral_constant<int, 0>(), (u32)v, H, mlen);
-                defer(std::integral_constant<int, 1>(), (u32)v, H, llen);
+                defer(std::integral_constant<int, 0>(), (u32)v, H, (u32)(cut[1] - cut[0]));
+                defer(std::integral_constant<int, 1>(), (u32)v, H, (u32)(cut[2] - cut[1]));
+                defer(std::integral_constant<int, 2>(), (u32)v, H, (u32)(cut[3] - cut[2]));
+                static_assert(kDefer == 3, "one defer call per class");
                 const u32 len = (u32)(e - b);
                 const bool item = H && len > 0;
                 q_push64<kQMed0>(a, wq, item && btier(len) == 0, ((u64)H << 32) | (u32)v);
@@ -868,12 +900,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
             bsync(ctl, 3, __float_as_uint(lo), x);
             // The deferred queues are read between the two barriers of this
             // step: the next appends to these sets come after the second one.
-            u32 lc0[2], lc1[2];
-            for (int tier = 0; tier < 2; ++tier) {
-                lc0[tier] = min(ld_relaxed(&ctl->llen[0][ls0][tier].v), (u32)a.n);
-                lc1[tier] = min(ld_relaxed(&ctl->llen[1][ls1][tier].v), (u32)a.n);
+            u32 lc[kDefer][2];
+            u64 lh[kDefer];
+#pragma unroll
+            for (int c = 0; c < kDefer; ++c) {
+                for (int tier = 0; tier < 2; ++tier) lc[c][tier] = min(ld_relaxed(&ctl->llen[c][wq.ls[c]][tier].v), (u32)a.n);
+                lh[c] = ld_relaxed(&ctl->lhub[c][wq.ls[c]].v);
             }
-            const u64 lh0 = ld_relaxed(&ctl->lhub[0][ls0].v), lh1 = ld_relaxed(&ctl->lhub[1][ls1].v);
             tick(1);
             group_tier<kSegMedium>(a, nullptr, nullptr, a.hq[0], min(ld_relaxed(&ctl->hlen[hs][0].v), (u32)a.n), t,
                                    improved, slot, wks[1], wq);
@@ -888,53 +921,59 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
             // Flush a deferred class c when the next bucket reaches past
             // fl(epoch_c + W_c), where its relaxations may land, or when the
             // next bucket looks empty (a min step must see every label).
-            lpending0 = lc0[0] || lc0[1] || hub_count(lh0);
-            lpending1 = lc1[0] || lc1[1] || hub_count(lh1);
-            if (lpending0 || lpending1) {
+            bool anyp = false;
+#pragma unroll
+            for (int c = 0; c < kDefer; ++c) {
+                lpending[c] = lc[c][0] || lc[c][1] || hub_count(lh[c]);
+                anyp = anyp || lpending[c];
+            }
+            if (anyp) {
                 u32 qc[2];
                 u64 qh;
                 const bool next_empty =
                     queue_size(x + 1, qc, &qh) == 0 && ld_relaxed(&ctl->nnext[(x + 1) % 3]) == 0;
-                const bool f0 = lpending0 && (hi2 > epoch0 + a.dw[0] || next_empty);
-                const bool f1 = lpending1 && (hi2 > epoch1 + a.dw[1] || next_empty);
-                if (f0 || f1) {
+                bool fl[kDefer], anyf = false;
+                u64 items = 0;
+#pragma unroll
+                for (int c = 0; c < kDefer; ++c) {
+                    fl[c] = lpending[c] && (hi2 > epoch[c] + a.dw[c] || next_empty);
+                    anyf = anyf || fl[c];
+                    if (fl[c]) items += (u64)lc[c][0] + lc[c][1] + hub_count(lh[c]);
+                }
+                if (anyf) {
                     if (gtid == 0) {  // a class's other set was last read by its previous flush
-                        if (f0) {
-                            st_relaxed(&ctl->llen[0][ls0 ^ 1][0].v, 0u);
-                            st_relaxed(&ctl->llen[0][ls0 ^ 1][1].v, 0u);
-                            st_relaxed(&ctl->lhub[0][ls0 ^ 1].v, 0ull);
-                        }
-                        if (f1) {
-                            st_relaxed(&ctl->llen[1][ls1 ^ 1][0].v, 0u);
-                            st_relaxed(&ctl->llen[1][ls1 ^ 1][1].v, 0u);
-                            st_relaxed(&ctl->lhub[1][ls1 ^ 1].v, 0ull);
-                        }
+#pragma unroll
+                        for (int c = 0; c < kDefer; ++c)
+                            if (fl[c]) {
+                                const int o = wq.ls[c] ^ 1;
+                                st_relaxed(&ctl->llen[c][o][0].v, 0u);
+                                st_relaxed(&ctl->llen[c][o][1].v, 0u);
+                                st_relaxed(&ctl->lhub[c][o].v, 0ull);
+                            }
                         atomicAdd((unsigned long long *)&ctl->flushes, 1ull);
                     }
-                    if (f0) {
-                        group_tier<kSegMid>(a, a.dpend[0], a.dq[0][0], nullptr, lc0[0], t, improved, slot, wks[4], wq);
-                        wave_tier<kSegMid>(a, a.dpend[0], a.dq[0][1], nullptr, lc0[1], t, improved, wks[4], wq);
-                        hub_tier<kSegMid>(a, a.dq[0][2], a.dq_off[0], a.dq_len[0], a.dq_done[0], a.dpend[0], lh0, t,
-                                          improved, wks[4], wq);
-                    }
-                    if (f1) {
-                        group_tier<kSegLong>(a, a.dpend[1], a.dq[1][0], nullptr, lc1[0], t, improved, slot, wks[2], wq);
-                        wave_tier<kSegLong>(a, a.dpend[1], a.dq[1][1], nullptr, lc1[1], t, improved, wks[2], wq);
-                        hub_tier<kSegLong>(a, a.dq[1][2], a.dq_off[1], a.dq_len[1], a.dq_done[1], a.dpend[1], lh1, t,
-                                           improved, wks[2], wq);
-                    }
+                    auto flush_class = [&](auto cls) {
+                        constexpr int c = decltype(cls)::value;
+                        if (!fl[c]) return;
+                        group_tier<kSegDef + c>(a, a.dpend[c], a.dq[c][0], nullptr, lc[c][0], t, improved, slot,
+                                                wks[3 + c], wq);
+                        wave_tier<kSegDef + c>(a, a.dpend[c], a.dq[c][1], nullptr, lc[c][1], t, improved, wks[3 + c],
+                                               wq);
+                        hub_tier<kSegDef + c>(a, a.dq[c][2], a.dq_off[c], a.dq_len[c], a.dq_done[c], a.dpend[c], lh[c],
+                                              t, improved, wks[3 + c], wq);
+                    };
+                    flush_class(std::integral_constant<int, 0>());
+                    flush_class(std::integral_constant<int, 1>());
+                    flush_class(std::integral_constant<int, 2>());
                     q_flush(a, wq);
                     bsync(ctl, 7, __float_as_uint(lo), x);
-                    tick(3, (f0 ? (u64)lc0[0] + lc0[1] + hub_count(lh0) : 0) +
-                                (f1 ? (u64)lc1[0] + lc1[1] + hub_count(lh1) : 0));
-                    if (f0) {
-                        ++nflush0;
-                        lpending0 = false;
-                    }
-                    if (f1) {
-                        ++nflush1;
-                        lpending1 = false;
-                    }
+                    tick(3, items);
+#pragma unroll
+                    for (int c = 0; c < kDefer; ++c)
+                        if (fl[c]) {
+                            ++nflush[c];
+                            lpending[c] = false;
+                        }
                 }
             }
             if (gtid == 0) atomicAdd((unsigned long long *)&ctl->scans, 1ull);
@@ -1004,7 +1043,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
     }
     if (lane == 0 && improved) atomicAdd((unsigned long long *)&ctl->improvements, (unsigned long long)improved);
     __syncthreads();
-    if (threadIdx.x < 15 && wks[threadIdx.x / 3][threadIdx.x % 3])
+    if (threadIdx.x < 3 * (3 + kDefer) && wks[threadIdx.x / 3][threadIdx.x % 3])
         atomicAdd((unsigned long long *)&ctl->work[threadIdx.x / 3][threadIdx.x % 3],
                   (unsigned long long)wks[threadIdx.x / 3][threadIdx.x % 3]);
     if (timer) {
@@ -1031,8 +1070,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
 __global__ __launch_bounds__(256) void sssp_batch_clear_kernel(u64 *__restrict__ lab, u32 *__restrict__ dl,
                                                                u32 *__restrict__ pend, u32 *__restrict__ done,
                                                                u32 *__restrict__ masks, u32 *__restrict__ mwork,
-                                                               u32 *__restrict__ lwork, u64 *__restrict__ openw,
-                                                               u64 n) {
+                                                               u32 *__restrict__ lwork, u32 *__restrict__ fwork,
+                                                               u64 *__restrict__ openw, u64 n) {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x; t < n * kBatchRoots; t += stride) {
         lab[t] = kInfLabel;
@@ -1042,11 +1081,13 @@ __global__ __launch_bounds__(256) void sssp_batch_clear_kernel(u64 *__restrict__
             done[t] = 0;
             masks[t] = t < n ? 0u : 0xFFFFFFFFu;  // final-root masks, then vmin
         }
-        if (t < n) {  // deferred pending masks and hub chunk counts of both classes
+        if (t < n) {  // deferred pending masks and hub chunk counts of every class
             mwork[t] = 0;
             mwork[5 * n + t] = 0;
             lwork[t] = 0;
             lwork[5 * n + t] = 0;
+            fwork[t] = 0;
+            fwork[5 * n + t] = 0;
         }
         if (t < (n + 63) / 64) openw[t] = 0;
     }
@@ -1110,9 +1151,16 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
         fail("batched SSSP mid_weight must be non-negative and finite (got %g)", (double)s->mid_weight);
         return false;
     }
-    // class floors: delta <= W_0 (mid) <= W_1 (long)
-    const f32 wlong = s->long_weight > s->delta ? s->long_weight : s->delta;
-    const f32 wmid = s->mid_weight < s->delta ? s->delta : (s->mid_weight > wlong ? wlong : s->mid_weight);
+    if (!(s->far_weight >= 0.0f) || !(s->far_weight < 3.0e38f)) {
+        fail("batched SSSP far_weight must be non-negative and finite (got %g)", (double)s->far_weight);
+        return false;
+    }
+    // class floors: delta <= W_0 (mid) <= W_1 (long) <= W_2 (far); a class whose
+    // floor is clamped onto the next one is empty
+    auto clampw = [](f32 w, f32 lo, f32 hi) { return w < lo ? lo : (w > hi ? hi : w); };
+    const f32 wfar = s->far_weight > s->delta ? s->far_weight : s->delta;
+    const f32 wlong = clampw(s->long_weight, s->delta, wfar);
+    const f32 wmid = clampw(s->mid_weight, s->delta, wlong);
     const u64 n = g->nlocalverts;
     if (n > s->max_nlocalverts) {
         fail("batched SSSP state holds %llu vertices, graph has %llu", (unsigned long long)s->max_nlocalverts,
@@ -1124,17 +1172,19 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
     // the split depends on the graph too: a state reused across graphs recomputes it
     const TraversalView tv = traversal_view(g);
     if (s->split_delta != s->delta || s->long_split_weight != wlong || s->mid_split_weight != wmid ||
-        s->split_graph != (const void *)tv.rs) {
+        s->far_split_weight != wfar || s->split_graph != (const void *)tv.rs) {
         compute_split(tv.rs, tv.w, n, s->delta, s->split, st);
         compute_split(tv.rs, tv.w, n, wmid, s->mid_split, st);
         compute_split(tv.rs, tv.w, n, wlong, s->long_split, st);
+        compute_split(tv.rs, tv.w, n, wfar, s->far_split, st);
         s->split_delta = s->delta;
         s->mid_split_weight = wmid;
         s->long_split_weight = wlong;
+        s->far_split_weight = wfar;
         s->split_graph = (const void *)tv.rs;
     }
     sssp_batch_clear_kernel<<<grid_for(n * kBatchRoots, 256, (u64)c.num_cus * 64), 256, 0, st>>>(
-        s->labels, (u32 *)s->dist, s->pending, s->hub_aux, s->root_masks, s->mid_work, s->long_work,
+        s->labels, (u32 *)s->dist, s->pending, s->hub_aux, s->root_masks, s->mid_work, s->long_work, s->far_work,
         s->open_words, n);
     G500_HIP(hipGetLastError());
     G500_HIP(hipMemsetAsync(s->ctl, 0, sizeof(SsspBatchCtl), st));
@@ -1146,13 +1196,14 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
     a.perm = tv.perm;
     a.iperm = tv.iperm;
     a.split = s->split;
-    // deferred classes: 0 mid, 1 long; work arrays [6n]: pending masks, queue
-    // tiers 0-2, hub lengths, hub chunk counts
+    // deferred classes: 0 mid, 1 long, 2 far; work arrays [6n]: pending masks,
+    // queue tiers 0-2, hub lengths, hub chunk counts
     a.dsplit[0] = s->mid_split;
     a.dsplit[1] = s->long_split;
-    u32 *dwork[2] = {s->mid_work, s->long_work};
-    u64 *doff[2] = {s->mid_offsets, s->long_offsets};
-    for (int d = 0; d < 2; ++d) {
+    a.dsplit[2] = s->far_split;
+    u32 *dwork[kDefer] = {s->mid_work, s->long_work, s->far_work};
+    u64 *doff[kDefer] = {s->mid_offsets, s->long_offsets, s->far_offsets};
+    for (int d = 0; d < kDefer; ++d) {
         a.dpend[d] = dwork[d];
         for (int t = 0; t < 3; ++t) a.dq[d][t] = dwork[d] + (u64)(1 + t) * n;
         a.dq_len[d] = dwork[d] + 4 * n;
@@ -1185,6 +1236,7 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
     a.delta = s->delta;
     a.dw[0] = wmid;
     a.dw[1] = wlong;
+    a.dw[2] = wfar;
     const int block = ctx().sssp_block;
     if (grid <= 0) grid = cooperative_blocks((const void *)sssp_batch_kernel, block, 0);
     if (failed()) return false;
@@ -1222,8 +1274,9 @@ Sssp_Batch_State oned_graph_sssp_batch_create(Worker_State *ws, const Oned_Graph
     const u64 n = g->nlocalverts;
     s.max_nlocalverts = n;
     s.delta = 0.1f;
-    s.long_weight = 0.25f;
     s.mid_weight = 0.075f;
+    s.long_weight = 0.25f;
+    s.far_weight = 0.5f;
     s.dist = (f32 *)arena_alloc(ws, kBatchRoots * n * sizeof(f32), "batched SSSP dist");
     s.pred = (i64 *)arena_alloc(ws, kBatchRoots * n * sizeof(i64), "batched SSSP pred");
     s.labels = (u64 *)arena_alloc(ws, kBatchRoots * n * sizeof(u64), "batched SSSP labels");
@@ -1240,6 +1293,9 @@ Sssp_Batch_State oned_graph_sssp_batch_create(Worker_State *ws, const Oned_Graph
     s.mid_split = (u32 *)arena_alloc(ws, n * sizeof(u32), "batched SSSP mid split");
     s.mid_work = (u32 *)arena_alloc(ws, 6 * n * sizeof(u32), "batched SSSP deferred mid-edge queue");
     s.mid_offsets = (u64 *)arena_alloc(ws, n * sizeof(u64), "batched SSSP deferred mid hub offsets");
+    s.far_split = (u32 *)arena_alloc(ws, n * sizeof(u32), "batched SSSP far split");
+    s.far_work = (u32 *)arena_alloc(ws, 6 * n * sizeof(u32), "batched SSSP deferred far-edge queue");
+    s.far_offsets = (u64 *)arena_alloc(ws, n * sizeof(u64), "batched SSSP deferred far hub offsets");
     s.long_split = (u32 *)arena_alloc(ws, n * sizeof(u32), "batched SSSP long split");
     s.long_work = (u32 *)arena_alloc(ws, 6 * n * sizeof(u32), "batched SSSP deferred long-edge queue");
     s.long_offsets = (u64 *)arena_alloc(ws, n * sizeof(u64), "batched SSSP deferred hub offsets");
@@ -1293,7 +1349,7 @@ i32 g500_sssp_batch_step_log(const Sssp_Batch_State *s, i32 *kind, u64 *items, f
 void g500_sssp_batch_work(const Sssp_Batch_State *s, u64 *out) {
     static SsspBatchCtl h;
     g500_copy_to_host(&h, s->ctl, sizeof(h));
-    for (int k = 0; k < 5; ++k)
+    for (int k = 0; k < 3 + kDefer; ++k)
         for (int j = 0; j < 3; ++j) out[k * 3 + j] = h.work[k][j];
 }
 

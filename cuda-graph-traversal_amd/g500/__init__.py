@@ -65,10 +65,11 @@ class SsspBatchState(C.Structure):
     _fields_ = [("dist", vp), ("pred", vp), ("labels", vp), ("pending", vp), ("frontier", vp), ("heavy", vp),
                 ("hub_vertices", vp), ("hub_offsets", vp), ("hub_lengths", vp), ("hub_aux", vp), ("split", vp),
                 ("root_masks", vp), ("open_words", vp), ("mid_split", vp), ("mid_work", vp), ("mid_offsets", vp),
+                ("far_split", vp), ("far_work", vp), ("far_offsets", vp),
                 ("long_split", vp), ("long_work", vp), ("long_offsets", vp),
                 ("ctl", vp), ("split_graph", vp), ("delta", f32), ("split_delta", f32), ("long_weight", f32),
-                ("long_split_weight", f32), ("mid_weight", f32), ("mid_split_weight", f32), ("nroots", i32),
-                ("reserved", i32),
+                ("long_split_weight", f32), ("mid_weight", f32), ("mid_split_weight", f32), ("far_weight", f32),
+                ("far_split_weight", f32), ("nroots", i32), ("reserved", i32),
                 ("max_nlocalverts", u64)]
 
 
@@ -373,10 +374,10 @@ class Session:
         return list(out)
 
     def sssp_batch_work(self, s):
-        out = (u64 * 15)()
+        out = (u64 * 18)()
         self.lib.g500_sssp_batch_work(C.byref(s), out)
         return {k: tuple(int(x) for x in out[3 * i:3 * i + 3])
-                for i, k in enumerate(("light", "medium", "long", "scan", "mid"))}
+                for i, k in enumerate(("light", "medium", "scan", "mid", "long", "far"))}
 
     def sssp_batch_stats(self, s):
         b, li, r = u64(), u64(), u64()

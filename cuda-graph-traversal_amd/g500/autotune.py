@@ -7,7 +7,7 @@ that a benchmark measures. Candidates:
   plain BFS:     (bfs_alpha, bfs_beta) switch thresholds, bfs_block
   batched BFS:   batch_alpha, batch_heavy_degree
   plain SSSP:    delta, sssp_block
-  batched SSSP:  (delta, mid_weight, long_weight) triples
+  batched SSSP:  (delta, mid_weight, long_weight, far_weight) candidates
 Every candidate gives the same depths and distances (the tuner changes only
 schedules), so the choice is by time alone.
 """
@@ -18,7 +18,9 @@ BLOCKS = [256, 512]
 BATCH_ALPHA = [4.0, 15.0, 60.0]
 BATCH_HEAVY = [4096, 1 << 20]
 SSSP_DELTA = [0.025, 0.05, 0.1]
-BATCH_SSSP = [(0.025, 0.075, 0.25), (0.025, 0.05, 0.2), (0.0125, 0.05, 0.2), (0.05, 0.1, 0.3)]  # (delta, mid, long)
+# (delta, mid, long, far); far == long leaves the far class empty
+BATCH_SSSP = [(0.025, 0.075, 0.25, 0.5), (0.025, 0.075, 0.25, 0.25), (0.0125, 0.05, 0.2, 0.5),
+              (0.05, 0.1, 0.3, 0.6)]
 
 
 def tuning_roots(sess, g, count, seed=2024):
@@ -109,16 +111,17 @@ def _tune(sess, g, batch, roots_plain, seed, say):
         say(f"[tune] batched bfs {best_bb}")
         sb = sess.sssp_batch_create(g)
         res = []
-        for d, wm, wl in BATCH_SSSP:
-            sb.delta, sb.mid_weight, sb.long_weight = d, wm, wl
+        for d, wm, wl, wf in BATCH_SSSP:
+            sb.delta, sb.mid_weight, sb.long_weight, sb.far_weight = d, wm, wl, wf
             ms = sess.sssp_batch(g, sb, roots[:int(sess.get_param("sssp_batch_max"))])
-            res.append({"kind": "batch_sssp", "delta": d, "mid_weight": wm, "long_weight": wl, "ms": ms})
+            res.append({"kind": "batch_sssp", "delta": d, "mid_weight": wm, "long_weight": wl, "far_weight": wf,
+                        "ms": ms})
         best_bs = _best(res)
         trials += res
         say(f"[tune] batched sssp {best_bs}")
         params.update({"batch_alpha": best_bb["batch_alpha"], "batch_heavy_degree": best_bb["batch_heavy_degree"],
                        "batch_sssp_delta": best_bs["delta"], "batch_sssp_mid_weight": best_bs["mid_weight"],
-                       "batch_sssp_long_weight": best_bs["long_weight"]})
+                       "batch_sssp_long_weight": best_bs["long_weight"], "batch_sssp_far_weight": best_bs["far_weight"]})
     else:
         for k in ("batch_alpha", "batch_heavy_degree"):
             sess.set_param(k, old[k])

@@ -76,8 +76,9 @@ i32 g500_bfs_batch_level_log(const Bfs_Batch_State *b, i32 *modes, u64 *records,
 /* Up to 32 roots in one persistent kernel with one bucket sequence shared by
  * all roots; items are (vertex, root mask) pairs and the 32 distance probes of
  * a vertex share one 128-byte line. Edges with w >= mid_weight are relaxed in
- * deferred batches, in two classes: mid (mid_weight <= w < long_weight) and
- * long (w >= long_weight). A class-c relaxation lands at least W_c above the
+ * deferred batches, in three classes: mid (mid_weight <= w < long_weight),
+ * long (long_weight <= w < far_weight) and far (w >= far_weight). A class-c
+ * relaxation lands at least W_c above the
  * bucket that settled its source, so the kernel collects the roots a vertex
  * settles over a W_c-wide window and relaxes the class's edges of the vertex
  * once for all of them.
@@ -106,7 +107,10 @@ struct Sssp_Batch_State {
     u32 *mid_work;      /* [6n] deferred mid edges: pending root masks, queue tiers 0-2, hub lengths,
                            hub chunk counts */
     u64 *mid_offsets;   /* [n] hub work offsets of the deferred mid queue */
-    u32 *long_split;    /* [n] entries with w < W_long per row (W_long = max(long_weight, delta)) */
+    u32 *far_split;     /* [n] entries with w < W_far per row (W_far = max(far_weight, delta)) */
+    u32 *far_work;      /* [6n] deferred far edges, laid out as mid_work */
+    u64 *far_offsets;   /* [n] hub work offsets of the deferred far queue */
+    u32 *long_split;    /* [n] entries with w < W_long per row (W_long = long_weight clamped to [delta, W_far]) */
     u32 *long_work;     /* [6n] deferred long edges: pending root masks, queue tiers 0-2, hub lengths,
                            hub chunk counts */
     u64 *long_offsets;  /* [n] hub work offsets of the deferred queue */
@@ -118,6 +122,8 @@ struct Sssp_Batch_State {
     f32 long_split_weight;
     f32 mid_weight;     /* mid-edge threshold (default 0.075f; <= delta: no mid class) */
     f32 mid_split_weight;
+    f32 far_weight;     /* far-edge threshold (default 0.5f; <= long: no far class) */
+    f32 far_split_weight;
     i32 nroots;         /* roots of the last run */
     i32 reserved;
     u64 max_nlocalverts;
@@ -136,12 +142,12 @@ void g500_sssp_batch_run_stats(const Sssp_Batch_State *s, u64 *buckets, u64 *lig
  * passes, ms[5] = deferred-queue flushes; ms[6..8] = the dense steps' scan,
  * medium and flush parts. */
 void g500_sssp_batch_phase_ms(const Sssp_Batch_State *s, f64 ms[9]);
-/* Work of the last batched run per row segment k (0 light, 1 medium, 2 long,
- * 4 mid): out[3k + 0] items, out[3k + 1] row entries visited, out[3k + 2]
- * (entry, root) relaxations; out[9..11] of the dense scans: open 64-vertex
- * words walked, vertices that read their probe line, vertices settling a
- * root. */
-void g500_sssp_batch_work(const Sssp_Batch_State *s, u64 out[15]);
+/* Work of the last batched run per row segment k (0 light, 1 medium, 3 mid,
+ * 4 long, 5 far): out[3k + 0] items, out[3k + 1] row entries visited,
+ * out[3k + 2] (entry, root) relaxations; out[6..8] of the dense scans: open
+ * 64-vertex words walked, vertices that read their probe line, vertices
+ * settling a root. */
+void g500_sssp_batch_work(const Sssp_Batch_State *s, u64 out[18]);
 /* Per-phase log of the last batched run (first 4096 phases): kind (0 light
  * step, 1 dense scan, 2 medium relaxations, 3 deferred flush, 4 min + collect
  * steps), items processed and device milliseconds. Returns the entries. */

@@ -373,16 +373,18 @@ def test_sssp_batch_bitexact(arena, oracle, scale):
     s = arena.sssp_batch_create(g)
     roots = oracle.roots(ref)
     width = int(arena.get_param("sssp_batch_max"))
-    # (delta, mid_weight, long_weight): both deferral classes, mid below delta
-    # (no mid class), both below delta (no window), narrow and wide windows,
-    # no deferral at all, mid above long (clamped: no mid class)
-    for delta, wm, wl in ((0.02, 0.05, 0.2), (0.1, 0.05, 0.2), (0.25, 0.05, 0.2), (0.02, 0.0, 0.05),
-                          (0.02, 0.0, 0.0), (0.02, 0.1, 0.7), (0.02, 0.5, 0.2)):
-        s.delta, s.mid_weight, s.long_weight = delta, wm, wl
-        say("sssp batch", scale, delta, wm, wl)
+    # (delta, mid, long, far thresholds): all three deferral classes, mid below
+    # delta (no mid class), all below delta (no window), narrow and wide
+    # windows, no deferral at all, thresholds out of order (clamped: classes
+    # empty), far only
+    for delta, wm, wl, wf in ((0.02, 0.05, 0.2, 0.5), (0.1, 0.05, 0.2, 0.4), (0.25, 0.05, 0.2, 0.2),
+                              (0.02, 0.0, 0.05, 0.3), (0.02, 0.0, 0.0, 0.0), (0.02, 0.1, 0.7, 0.9),
+                              (0.02, 0.5, 0.2, 0.1), (0.02, 0.0, 0.0, 0.3)):
+        s.delta, s.mid_weight, s.long_weight, s.far_weight = delta, wm, wl, wf
+        say("sssp batch", scale, delta, wm, wl, wf)
         sssp_batch_check(arena, oracle, g, ref, s, roots[:width])
     # fewer roots, repeats, an isolated vertex as a root
-    s.delta, s.mid_weight, s.long_weight = 0.1, 0.05, 0.2
+    s.delta, s.mid_weight, s.long_weight, s.far_weight = 0.1, 0.05, 0.2, 0.5
     isolated = int(np.flatnonzero(np.diff(ref.rowstarts.astype(np.int64)) == 0)[0])
     some = np.array([roots[20], roots[20], isolated, roots[17]], np.uint64)
     say("sssp batch repeats", scale, some.tolist())
@@ -400,15 +402,15 @@ def test_sssp_batch_matches_plain(arena, oracle):
     for r in roots:
         arena.sssp(g, sp, int(r))
         plain.append(arena.sssp_result(sp)[0])
-    for delta, wm, wl in ((0.1, 0.05, 0.2), (0.0125, 0.05, 0.25)):
-        sb.delta, sb.mid_weight, sb.long_weight = delta, wm, wl
+    for delta, wm, wl, wf in ((0.1, 0.05, 0.2, 0.5), (0.0125, 0.05, 0.25, 0.5)):
+        sb.delta, sb.mid_weight, sb.long_weight, sb.far_weight = delta, wm, wl, wf
         arena.sssp_batch(g, sb, roots)
         flushes = arena.sssp_batch_phase_ms(sb)[5]
-        say("sssp batch S18", delta, wm, wl, "flushes", flushes)
+        say("sssp batch S18", delta, wm, wl, wf, "flushes", flushes)
         assert flushes > 0
         for i, r in enumerate(roots):
             d, p = arena.sssp_batch_result(g, sb, i)
-            assert np.array_equal(d.view(np.uint32), plain[i].view(np.uint32)), (delta, wm, wl, i)
+            assert np.array_equal(d.view(np.uint32), plain[i].view(np.uint32)), (delta, wm, wl, wf, i)
             assert oracle.sssp_check(ref, int(r), d, p) == 0
 
 
@@ -417,14 +419,14 @@ def test_sssp_batch_diagnostics(arena, oracle):
     e, w, ref = oracle_graph(14)
     g = arena.csr(arena.upload_tuples(e, w), 1 << 14)
     s = arena.sssp_batch_create(g, 0.02)
-    s.mid_weight, s.long_weight = 0.05, 0.2
+    s.mid_weight, s.long_weight, s.far_weight = 0.05, 0.2, 0.4
     roots = arena.roots(g)[:int(arena.get_param("sssp_batch_max"))]
     arena.sssp_batch(g, s, roots)
     wk = arena.sssp_batch_work(s)
-    for seg in ("light", "medium", "mid", "long"):
+    for seg in ("light", "medium", "mid", "long", "far"):
         items, entries, pairs = wk[seg]
         assert entries >= items and pairs >= entries, (seg, wk[seg])
-    assert wk["medium"][1] > 0 and wk["mid"][1] > 0 and wk["long"][1] > 0
+    assert wk["medium"][1] > 0 and wk["mid"][1] > 0 and wk["long"][1] > 0 and wk["far"][1] > 0
     kind, items, ms = arena.sssp_batch_step_log(s)
     ph = arena.sssp_batch_phase_ms(s)
     b, li, imp = arena.sssp_batch_stats(s)
@@ -451,6 +453,11 @@ def test_sssp_batch_errors_return(arena):
     s.long_weight = 0.2
     for wm in (-1.0, float("nan"), float("inf")):
         s.mid_weight = wm
+        with pytest.raises(g500.LibraryError):
+            arena.sssp_batch(g, s, np.array([1], np.uint64))
+    s.mid_weight = 0.05
+    for wf in (-1.0, float("nan"), float("inf")):
+        s.far_weight = wf
         with pytest.raises(g500.LibraryError):
             arena.sssp_batch(g, s, np.array([1], np.uint64))
 
@@ -495,7 +502,8 @@ def test_autotuner_applies_candidates(arena, oracle):
         p = out["params"]
         assert (p["bfs_alpha"], p["bfs_beta"]) in autotune.BFS_SWITCH and p["bfs_block"] in autotune.BLOCKS
         assert p["sssp_delta"] in autotune.SSSP_DELTA
-        assert (p["batch_sssp_delta"], p["batch_sssp_mid_weight"], p["batch_sssp_long_weight"]) in autotune.BATCH_SSSP
+        assert (p["batch_sssp_delta"], p["batch_sssp_mid_weight"], p["batch_sssp_long_weight"],
+                p["batch_sssp_far_weight"]) in autotune.BATCH_SSSP
         assert p["batch_alpha"] in autotune.BATCH_ALPHA and p["batch_heavy_degree"] in autotune.BATCH_HEAVY
         for k in keys:
             assert arena.get_param(k) == p[k]
@@ -504,6 +512,7 @@ def test_autotuner_applies_candidates(arena, oracle):
         bfs_check(arena, oracle, g, ref, int(roots[0]))
         sb = arena.sssp_batch_create(g, p["batch_sssp_delta"])
         sb.mid_weight, sb.long_weight = p["batch_sssp_mid_weight"], p["batch_sssp_long_weight"]
+        sb.far_weight = p["batch_sssp_far_weight"]
         sssp_batch_check(arena, oracle, g, ref, sb, roots[:16])
     finally:
         for k, v in old.items():
