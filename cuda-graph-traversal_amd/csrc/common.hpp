@@ -69,6 +69,9 @@ void timer_stop();   // synchronizes, stores ctx().last_ms
 inline void sync() { G500_HIP(hipStreamSynchronize(ctx().stream)); }
 
 // Grid size for a cooperative (persistent) kernel: every block co-resident.
+// The runtime's blocks-per-CU figure is capped by the kernel's LDS (static +
+// dyn_lds) rounded up to 2 KiB per block, so that a block count that only fits
+// the CU's LDS without allocation rounding is never taken.
 int cooperative_blocks(const void *kernel, int block, size_t dyn_lds);
 
 // Launches a persistent kernel whose grid fits on the device at once. With
@@ -79,7 +82,7 @@ int cooperative_blocks(const void *kernel, int block, size_t dyn_lds);
 // kernels' grid barriers are bounded, so a grid that is not fully resident
 // ends with an error instead of waiting forever. stream 0: the library stream.
 void launch_persistent(const void *kernel, int grid, int block, void **args, hipStream_t stream = nullptr,
-                       bool cooperative = true);
+                       bool cooperative = true, size_t dyn_lds = 0);
 
 // Traversal view of a graph: the degree-ordered copy when it exists, else the
 // original arrays. Kernels run in traversal ids and report original ids

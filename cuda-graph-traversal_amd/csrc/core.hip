@@ -47,6 +47,13 @@ void timer_stop() {
 int cooperative_blocks(const void *kernel, int block, size_t dyn_lds) {
     int per_cu = 0;
     G500_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, dyn_lds));
+    hipFuncAttributes attr{};
+    G500_HIP(hipFuncGetAttributes(&attr, kernel));
+    int lds_cu = 0;
+    G500_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, ctx().device));
+    const size_t lds = attr.sharedSizeBytes + dyn_lds;
+    constexpr size_t kLdsRound = 2048;
+    if (lds && lds_cu > 0) per_cu = std::min(per_cu, (int)((size_t)lds_cu / ((lds + kLdsRound - 1) / kLdsRound * kLdsRound)));
     if (per_cu <= 0) {
         fail("kernel cannot be resident with %d threads per block", block);
         return 1;
@@ -55,11 +62,11 @@ int cooperative_blocks(const void *kernel, int block, size_t dyn_lds) {
 }
 
 void launch_persistent(const void *kernel, int grid, int block, void **args, hipStream_t stream,
-                       bool cooperative) {
+                       bool cooperative, size_t dyn_lds) {
     Context &c = ctx();
     if (!stream) stream = c.stream;
     if (cooperative && c.coop_launch) {
-        hipError_t e = hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(block), args, 0, stream);
+        hipError_t e = hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(block), args, dyn_lds, stream);
         if (e == hipSuccess) return;
         (void)hipGetLastError();
         static bool warned = false;
@@ -69,7 +76,7 @@ void launch_persistent(const void *kernel, int grid, int block, void **args, hip
             warned = true;
         }
     }
-    G500_HIP(hipLaunchKernel(kernel, dim3(grid), dim3(block), args, 0, stream));
+    G500_HIP(hipLaunchKernel(kernel, dim3(grid), dim3(block), args, dyn_lds, stream));
 }
 
 void init_mrg_table();  // generator.hip

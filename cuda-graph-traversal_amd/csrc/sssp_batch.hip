@@ -153,18 +153,28 @@ __device__ __forceinline__ u64 wave_index() {
 // full (one atomic for the run) and at the end of the phase, where the block
 // sums its waves' counts and makes one atomic per list. Counters then see one
 // atomic per block and phase instead of one per wave and relaxation round.
+//
+// The buffers are dynamic LDS sized by the block's wave count, so that blocks
+// of fewer waves take proportionally less LDS and as many of them stay
+// co-resident as the waves-per-CU limit allows.
 constexpr u32 kQCap = 64;
-constexpr int kMaxBlockWaves = 512 / kWave;
 enum : int { kQLight0 = 0, kQDef0 = 2, kQ32 = 2 + 2 * kDefer, kQMed0 = 0, kQ64 = 2 };  // u32: light 0-1, class c 2+2c..3+2c
 
-struct QLds {
-    u32 v32[kMaxBlockWaves][kQ32][kQCap];
-    u64 v64[kMaxBlockWaves][kQ64][kQCap];
-    u32 cnt[kMaxBlockWaves][kQ32 + kQ64];  // counts at the phase end, then the waves' offsets
+struct QWave {  // one wave's staging buffers
+    u32 v32[kQ32][kQCap];
+    u64 v64[kQ64][kQCap];
 };
 
+// Dynamic LDS of a block of `block` threads: the waves' buffers, then the
+// per-wave counts [waves][kQ32 + kQ64] (counts at the phase end, then the
+// waves' offsets).
+constexpr size_t q_lds_bytes(int block) {
+    return (size_t)(block / kWave) * (sizeof(QWave) + (kQ32 + kQ64) * sizeof(u32));
+}
+
 struct WaveQ {
-    QLds *L;
+    QWave *W;              // this wave's buffers
+    volatile u32 *cnt;     // the block's counts
     int w;                 // wave of the block
     u32 n[kQ32 + kQ64];    // staged entries per list (u32 lists first)
     int buf, slot;         // light queue buffer and counter slot of the phase's pushes
@@ -210,7 +220,7 @@ __device__ __forceinline__ void q_push32(const SsspBatchArgs &a, WaveQ &q, bool 
     const u64 m = ballot(want);
     if (!m) return;
     const u32 k = (u32)__popcll(m);
-    volatile u32 *buf = q.L->v32[q.w][L];
+    volatile u32 *buf = q.W->v32[L];
     if (q.n[L] + k > kQCap) {  // spill the full buffer as one run
         u32 base = 0;
         if (lane_id() == 0) base = atomicAdd(q_counter(a, q, L), q.n[L]);
@@ -227,7 +237,7 @@ __device__ __forceinline__ void q_push64(const SsspBatchArgs &a, WaveQ &q, bool 
     const u64 m = ballot(want);
     if (!m) return;
     const u32 k = (u32)__popcll(m);
-    volatile u64 *buf = q.L->v64[q.w][L];
+    volatile u64 *buf = q.W->v64[L];
     if (q.n[kQ32 + L] + k > kQCap) {
         u32 base = 0;
         if (lane_id() == 0) base = atomicAdd(q_counter(a, q, kQ32 + L), q.n[kQ32 + L]);
@@ -243,7 +253,7 @@ __device__ __forceinline__ void q_push64(const SsspBatchArgs &a, WaveQ &q, bool 
 // with staged entries, then each wave writes its runs.
 __device__ __forceinline__ void q_flush(const SsspBatchArgs &a, WaveQ &q) {
     constexpr int kL = kQ32 + kQ64;
-    volatile u32 *cnt = &q.L->cnt[0][0];
+    volatile u32 *cnt = q.cnt;
     const int nw = (int)(blockDim.x / kWave);
     if (lane_id() == 0) {
 #pragma unroll
@@ -265,13 +275,13 @@ __device__ __forceinline__ void q_flush(const SsspBatchArgs &a, WaveQ &q) {
     __syncthreads();
 #pragma unroll
     for (int l = 0; l < kQ32; ++l) {
-        if (q.n[l]) q_copy((volatile u32 *)q.L->v32[q.w][l], q.n[l], q_dst32(a, q, l), cnt[q.w * kL + l], a.n);
+        if (q.n[l]) q_copy((volatile u32 *)q.W->v32[l], q.n[l], q_dst32(a, q, l), cnt[q.w * kL + l], a.n);
         q.n[l] = 0;
     }
 #pragma unroll
     for (int l = 0; l < kQ64; ++l) {
         if (q.n[kQ32 + l])
-            q_copy((volatile u64 *)q.L->v64[q.w][l], q.n[kQ32 + l], a.hq[l], cnt[q.w * kL + kQ32 + l], a.n);
+            q_copy((volatile u64 *)q.W->v64[l], q.n[kQ32 + l], a.hq[l], cnt[q.w * kL + kQ32 + l], a.n);
         q.n[kQ32 + l] = 0;
     }
     __syncthreads();  // the LDS buffers and counts are reused by the next phase
@@ -644,10 +654,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
     // work counts of this block: [light, medium, dense scans, deferred class c at 3 + c][3]
     __shared__ u64 wks[3 + kDefer][3];
     if (threadIdx.x < 3 * (3 + kDefer)) wks[threadIdx.x / 3][threadIdx.x % 3] = 0;
-    __shared__ QLds qlds;
+    extern __shared__ u64 q_lds[];
     WaveQ wq;
-    wq.L = &qlds;
     wq.w = wave_in_block();
+    wq.W = (QWave *)q_lds + wq.w;
+    wq.cnt = (volatile u32 *)((QWave *)q_lds + blockDim.x / kWave);
 #pragma unroll
     for (int l = 0; l < kQ32 + kQ64; ++l) wq.n[l] = 0;
     wq.buf = wq.slot = wq.hs = 0;
@@ -1238,16 +1249,17 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
     a.dw[1] = wlong;
     a.dw[2] = wfar;
     const int block = ctx().sssp_block;
-    if (grid <= 0) grid = cooperative_blocks((const void *)sssp_batch_kernel, block, 0);
+    const size_t dyn = q_lds_bytes(block);
+    if (grid <= 0) grid = cooperative_blocks((const void *)sssp_batch_kernel, block, dyn);
     if (failed()) return false;
     void *args[] = {&a};
     s->nroots = nroots;
     if (library) {
         timer_start();
-        launch_persistent((const void *)sssp_batch_kernel, grid, block, args);
+        launch_persistent((const void *)sssp_batch_kernel, grid, block, args, nullptr, true, dyn);
         timer_stop();
     } else {
-        launch_persistent((const void *)sssp_batch_kernel, grid, block, args, st, false);
+        launch_persistent((const void *)sssp_batch_kernel, grid, block, args, st, false, dyn);
     }
     return !failed();
 }
@@ -1260,7 +1272,10 @@ u32 sssp_batch_error(const Sssp_Batch_State *s, hipStream_t stream) {
     return e;
 }
 
-int sssp_batch_grid() { return cooperative_blocks((const void *)sssp_batch_kernel, ctx().sssp_block, 0); }
+int sssp_batch_grid() {
+    const int block = ctx().sssp_block;
+    return cooperative_blocks((const void *)sssp_batch_kernel, block, q_lds_bytes(block));
+}
 
 }  // namespace g500
 
