@@ -42,6 +42,7 @@ struct Context {
     int coop_launch = 1;          // 1: cooperative launch first; 0: plain launch of the co-resident grid
     int gen_blocks = 0;           // 0: one block per 256-edge tile (capped)
     int relabel = 1;              // 1: CSR builds add the degree-ordered traversal copy
+    int sssp_prune = 1;           // 1: and the SSSP copy without two-hop-dominated entries
 };
 
 Context &ctx();
@@ -97,6 +98,13 @@ inline TraversalView traversal_view(const Oned_Graph *g) {
     if (g->perm) return {g->rowstarts_t, g->column_t, g->weights_t, g->perm, g->iperm};
     return {g->rowstarts, g->column, g->weights, nullptr, nullptr};
 }
+// SSSP view: the pruned SSSP copy when it exists, else the traversal view.
+inline TraversalView sssp_view(const Oned_Graph *g) {
+    if (g->rowstarts_s) return {g->rowstarts_s, g->column_s, g->weights_s, g->perm, g->iperm};
+    return traversal_view(g);
+}
+// Distances at or above this bound void the pruned copy's guarantee (error 32).
+constexpr f32 kPruneDistBound = 2048.0f;
 __device__ __forceinline__ u32 orig_id(const u32 *iperm, u32 v) { return iperm ? iperm[v] : v; }
 __device__ __forceinline__ u32 trav_id(const u32 *perm, u64 v) { return perm ? perm[v] : (u32)v; }
 

@@ -275,6 +275,7 @@ i32 g500_set_param(const char *name, f64 value) {
     else if (n == "gen_blocks") c.gen_blocks = (int)value;
     else if (n == "coop_launch") c.coop_launch = (int)value;
     else if (n == "relabel") c.relabel = (int)value;
+    else if (n == "sssp_prune") c.sssp_prune = (int)value;
     else return -1;
     return 0;
 }
@@ -293,6 +294,7 @@ i32 g500_get_param(const char *name, f64 *value) {
     else if (n == "gen_blocks") *value = c.gen_blocks;
     else if (n == "coop_launch") *value = c.coop_launch;
     else if (n == "relabel") *value = c.relabel;
+    else if (n == "sssp_prune") *value = c.sssp_prune;
     else if (n == "num_cus") *value = c.num_cus;
     else if (n == "sssp_batch_max") *value = kSsspBatchRoots;
     else return -1;

@@ -10,6 +10,9 @@
 // are sorted by one of four tiers: a thread (<= 16 entries), a wave
 // (<= 64), a block in LDS (<= 4096), or a block-wide LSD radix sort through a
 // scratch buffer (longer rows).
+#include <algorithm>
+#include <vector>
+
 #include "common.hpp"
 
 namespace g500 {
@@ -471,6 +474,256 @@ __global__ __launch_bounds__(256) void relabel_rows_block(const u64 *__restrict_
     }
 }
 
+// ---------------------------------------------------------------- SSSP copy
+// An entry (u, v, w) of the traversal copy is left out of the SSSP copy when
+// a two-hop path u-x-v through one of the kPruneHubs highest-degree vertices
+// (ties to the smaller traversal id) is lighter by more than kPruneMargin:
+// hw[u][x] + hw[v][x] + kPruneMargin < w, with hw[y][x] the lightest weight of
+// an entry (y, x) (+inf without one), summed in f64 (exact for f32 weights
+// below kPruneMaxWeight that are multiples of 2^-31, as generated weights are;
+// otherwise off by far less than the margin).
+//
+// Why distances do not change: both hops of a dropped entry are lighter than
+// it, so by induction on weight every dropped entry is beaten by a path of
+// kept entries that is lighter by more than (hops - 1) margins. Along that
+// path f32 rounding costs at most half an ulp per addition, and comparing
+// with fl(d + w) costs another half: with every value below 2048 + 2, an ulp
+// is at most 2^-12, and (h - 1) * 2^-10 > (h + 1) * 2^-13 for every h >= 2.
+// So fl(d(u) + w) is never below what the kept path gives v, the entry never
+// decides a distance, and the f32 fixpoint (the distances) is the same with
+// or without it. Parents stay valid for the full graph's certificate.
+constexpr int kPruneHubs = 32;
+constexpr double kPruneMargin = 1.0 / 1024.0;
+constexpr f32 kPruneMaxWeight = 2.0f;
+
+// Hub slot of traversal id x (ids below nslot are looked up; kPruneHubs: none).
+__device__ __forceinline__ u32 hub_slot_of(const u8 *slot, u32 nslot, u32 x) {
+    return x < nslot ? (u32)slot[x] : (u32)kPruneHubs;
+}
+
+// hw[u][slot of x] (f32 bits; non-negative floats order as their bits): a
+// lane per row from row `first` on; a block per row for the rows before it.
+__global__ __launch_bounds__(256) void prune_hubs_lane(const u64 *__restrict__ rs, const u32 *__restrict__ col,
+                                                       const f32 *__restrict__ w, const u8 *__restrict__ slot,
+                                                       u32 nslot, u64 first, u64 n, u32 *__restrict__ hw) {
+    for (u64 u = first + (u64)blockIdx.x * blockDim.x + threadIdx.x; u < n; u += (u64)gridDim.x * blockDim.x) {
+        const u64 b = rs[u], e = rs[u + 1];
+        for (u64 k = b; k < e; ++k) {
+            const u32 x = hub_slot_of(slot, nslot, col[k]);
+            if (x < (u32)kPruneHubs) {
+                u32 *p = &hw[u * kPruneHubs + x];
+                const u32 wb = __float_as_uint(w[k]);
+                if (wb < *p) *p = wb;  // only this lane writes row u
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void prune_hubs_block(const u64 *__restrict__ rs, const u32 *__restrict__ col,
+                                                        const f32 *__restrict__ w, const u8 *__restrict__ slot,
+                                                        u32 nslot, u64 count, u32 *__restrict__ hw) {
+    for (u64 u = blockIdx.x; u < count; u += gridDim.x) {
+        const u64 b = rs[u], e = rs[u + 1];
+        for (u64 k = b + threadIdx.x; k < e; k += blockDim.x) {
+            const u32 x = hub_slot_of(slot, nslot, col[k]);
+            if (x < (u32)kPruneHubs) atomicMin(&hw[u * kPruneHubs + x], __float_as_uint(w[k]));
+        }
+    }
+}
+
+// Row u's hub weights, and their minimum (an entry lighter than it plus the
+// margin cannot be beaten).
+struct HubRow {
+    f32 h[kPruneHubs];
+    f32 hmin;
+};
+__device__ __forceinline__ void load_hub_row(const u32 *hw, u64 u, HubRow &r) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(hw + u * kPruneHubs);
+    r.hmin = __uint_as_float(0x7F800000u);
+#pragma unroll
+    for (int q = 0; q < kPruneHubs / 4; ++q) {
+        const uint4 c = p[q];
+        r.h[4 * q] = __uint_as_float(c.x);
+        r.h[4 * q + 1] = __uint_as_float(c.y);
+        r.h[4 * q + 2] = __uint_as_float(c.z);
+        r.h[4 * q + 3] = __uint_as_float(c.w);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) r.hmin = fminf(r.hmin, r.h[4 * q + i]);
+    }
+}
+
+// Whether entry (u, v, w) of row u is beaten by a two-hop path through a hub.
+__device__ __forceinline__ bool prune_entry(const HubRow &r, const u32 *hw, u32 v, f32 w) {
+    if (!(w < kPruneMaxWeight) || !((double)r.hmin + kPruneMargin < (double)w)) return false;
+    const uint4 *p = reinterpret_cast<const uint4 *>(hw + (u64)v * kPruneHubs);
+    double best = __builtin_inf();
+#pragma unroll
+    for (int q = 0; q < kPruneHubs / 4; ++q) {
+        const uint4 c = p[q];
+        best = fmin(best, (double)r.h[4 * q] + (double)__uint_as_float(c.x));
+        best = fmin(best, (double)r.h[4 * q + 1] + (double)__uint_as_float(c.y));
+        best = fmin(best, (double)r.h[4 * q + 2] + (double)__uint_as_float(c.z));
+        best = fmin(best, (double)r.h[4 * q + 3] + (double)__uint_as_float(c.w));
+    }
+    return best + kPruneMargin < (double)w;
+}
+
+// Kept entries per row (deg_s), or, with rs_s set, the kept entries written
+// in row order at rs_s[u].
+__global__ __launch_bounds__(256) void prune_rows_lane(const u64 *__restrict__ rs, const u32 *__restrict__ col,
+                                                       const f32 *__restrict__ w, const u32 *__restrict__ hw,
+                                                       u64 first, u64 n, u32 *__restrict__ deg_s,
+                                                       const u64 *__restrict__ rs_s, u32 *__restrict__ col_s,
+                                                       f32 *__restrict__ w_s) {
+    for (u64 u = first + (u64)blockIdx.x * blockDim.x + threadIdx.x; u < n; u += (u64)gridDim.x * blockDim.x) {
+        const u64 b = rs[u], e = rs[u + 1];
+        HubRow r;
+        load_hub_row(hw, u, r);
+        u64 o = rs_s ? rs_s[u] : 0;
+        u32 kept = 0;
+        for (u64 k = b; k < e; ++k) {
+            const u32 v = col[k];
+            const f32 wk = w[k];
+            if (prune_entry(r, hw, v, wk)) continue;
+            if (rs_s) {
+                col_s[o] = v;
+                w_s[o] = wk;
+                ++o;
+            }
+            ++kept;
+        }
+        if (!rs_s) deg_s[u] = kept;
+    }
+}
+
+__global__ __launch_bounds__(256) void prune_rows_block(const u64 *__restrict__ rs, const u32 *__restrict__ col,
+                                                        const f32 *__restrict__ w, const u32 *__restrict__ hw,
+                                                        u64 count, u32 *__restrict__ deg_s,
+                                                        const u64 *__restrict__ rs_s, u32 *__restrict__ col_s,
+                                                        f32 *__restrict__ w_s) {
+    __shared__ u32 wave_kept[256 / kWave];
+    const u32 wv = threadIdx.x / kWave, nw = blockDim.x / kWave;
+    for (u64 u = blockIdx.x; u < count; u += gridDim.x) {
+        const u64 b = rs[u], e = rs[u + 1];
+        HubRow r;
+        load_hub_row(hw, u, r);
+        u64 o = rs_s ? rs_s[u] : 0;
+        for (u64 base = b; base < e; base += blockDim.x) {
+            const u64 k = base + threadIdx.x;
+            bool keep = false;
+            u32 v = 0;
+            f32 wk = 0.0f;
+            if (k < e) {
+                v = col[k];
+                wk = w[k];
+                keep = !prune_entry(r, hw, v, wk);
+            }
+            const u64 m = ballot(keep);
+            if (lane_id() == 0) wave_kept[wv] = (u32)__popcll(m);
+            __syncthreads();
+            u32 before = 0, total = 0;
+            for (u32 i = 0; i < nw; ++i) {
+                before += i < wv ? wave_kept[i] : 0u;
+                total += wave_kept[i];
+            }
+            if (keep && rs_s) {
+                const u64 at = o + before + mask_rank(m);
+                col_s[at] = v;
+                w_s[at] = wk;
+            }
+            o += total;
+            __syncthreads();  // wave_kept is rewritten by the next chunk
+        }
+        if (!rs_s && threadIdx.x == 0) deg_s[u] = (u32)(o - (rs_s ? rs_s[u] : 0));
+    }
+}
+
+// Builds g's SSSP copy from its traversal copy (rows longer than kWave
+// entries, the first nlong, go to a block each; the first ntop rows are the
+// degree class above the exact range, in no particular order). Two passes,
+// each with its own hub table: kept counts and row offsets, then, with the
+// exact sizes allocated, the kept entries. Scratch is released after each pass.
+void build_sssp_copy(Worker_State *ws, Oned_Graph *g, u64 nlong, u64 ntop) {
+    Context &c = ctx();
+    const hipStream_t s = c.stream;
+    const u64 n = g->nlocalverts;
+    // the hubs: the kPruneHubs largest degrees among the first max(ntop, kPruneHubs)
+    // rows (every later row is no longer), ties to the smaller id
+    const u64 nhead = MIN(n, MAX(ntop, (u64)kPruneHubs));
+    std::vector<u64> head(nhead + 1);
+    G500_HIP(hipMemcpyAsync(head.data(), g->rowstarts_t, (nhead + 1) * sizeof(u64), hipMemcpyDeviceToHost, s));
+    sync();
+    if (failed()) return;
+    std::vector<u32> ids(nhead);
+    for (u64 i = 0; i < nhead; ++i) ids[i] = (u32)i;
+    const u64 nh = MIN(nhead, (u64)kPruneHubs);
+    std::partial_sort(ids.begin(), ids.begin() + nh, ids.end(), [&](u32 x, u32 y) {
+        const u64 dx = head[x + 1] - head[x], dy = head[y + 1] - head[y];
+        return dx != dy ? dx > dy : x < y;
+    });
+    u32 nslot = 0;
+    for (u64 i = 0; i < nh; ++i) nslot = MAX(nslot, ids[i] + 1);
+    std::vector<u8> slot_h(MAX(nslot, 1u), (u8)kPruneHubs);
+    for (u64 i = 0; i < nh; ++i) slot_h[ids[i]] = (u8)i;
+    u64 *rs_s = (u64 *)arena_alloc(ws, (n + 1) * sizeof(u64), "SSSP row offsets");
+    if (failed()) return;
+    const unsigned lane_grid = grid_for(n - nlong, 256, (u64)c.num_cus * 32);
+    const unsigned block_grid = (unsigned)MIN(MAX(nlong, (u64)1), (u64)c.num_cus * 8);
+    // hub table of the pass (slot map and weights, in the pass's scratch)
+    auto hub_table = [&](u32 *hw, u8 *slot) {
+        G500_HIP(hipMemcpyAsync(slot, slot_h.data(), slot_h.size(), hipMemcpyHostToDevice, s));
+        G500_HIP(hipMemsetD32Async(hw, 0x7F800000u, n * kPruneHubs, s));
+        if (nlong)
+            prune_hubs_block<<<block_grid, 256, 0, s>>>(g->rowstarts_t, g->column_t, g->weights_t, slot, nslot, nlong,
+                                                        hw);
+        prune_hubs_lane<<<lane_grid, 256, 0, s>>>(g->rowstarts_t, g->column_t, g->weights_t, slot, nslot, nlong, n,
+                                                  hw);
+        G500_HIP(hipGetLastError());
+        sync();  // slot_h is read by the copy
+    };
+    u64 mark = ws->gpu_arena.used;
+    u32 *hw = (u32 *)arena_alloc(ws, n * kPruneHubs * sizeof(u32), "SSSP prune hub table");
+    u8 *slot = (u8 *)arena_alloc(ws, slot_h.size(), "SSSP hub slots");
+    u32 *deg_s = (u32 *)arena_alloc(ws, n * sizeof(u32), "SSSP degrees");
+    const u64 ntiles = (n + kScanTile - 1) / kScanTile;
+    u64 *tiles = (u64 *)arena_alloc(ws, (ntiles + 1) * sizeof(u64), "SSSP scan tiles");
+    if (failed()) return;
+    hub_table(hw, slot);
+    if (nlong)
+        prune_rows_block<<<block_grid, 256, 0, s>>>(g->rowstarts_t, g->column_t, g->weights_t, hw, nlong, deg_s,
+                                                    nullptr, nullptr, nullptr);
+    prune_rows_lane<<<lane_grid, 256, 0, s>>>(g->rowstarts_t, g->column_t, g->weights_t, hw, nlong, n, deg_s,
+                                              nullptr, nullptr, nullptr);
+    scan_tile_sums<<<(unsigned)ntiles, kScanBlock, 0, s>>>(deg_s, n, tiles);
+    scan_tile_offsets<<<1, 1024, 0, s>>>(tiles, ntiles, rs_s, n);
+    scan_write<<<(unsigned)ntiles, kScanBlock, 0, s>>>(deg_s, n, tiles, rs_s);
+    G500_HIP(hipGetLastError());
+    u64 m_s = 0;
+    G500_HIP(hipMemcpyAsync(&m_s, rs_s + n, sizeof(u64), hipMemcpyDeviceToHost, s));
+    sync();
+    ws->gpu_arena.used = mark;
+    if (failed()) return;
+    u32 *col_s = (u32 *)arena_alloc(ws, MAX(m_s, (u64)1) * sizeof(u32), "SSSP columns");
+    f32 *w_s = (f32 *)arena_alloc(ws, MAX(m_s, (u64)1) * sizeof(f32), "SSSP weights");
+    mark = ws->gpu_arena.used;
+    hw = (u32 *)arena_alloc(ws, n * kPruneHubs * sizeof(u32), "SSSP prune hub table");
+    slot = (u8 *)arena_alloc(ws, slot_h.size(), "SSSP hub slots");
+    if (failed()) return;
+    hub_table(hw, slot);
+    if (nlong)
+        prune_rows_block<<<block_grid, 256, 0, s>>>(g->rowstarts_t, g->column_t, g->weights_t, hw, nlong, nullptr,
+                                                    rs_s, col_s, w_s);
+    prune_rows_lane<<<lane_grid, 256, 0, s>>>(g->rowstarts_t, g->column_t, g->weights_t, hw, nlong, n, nullptr,
+                                              rs_s, col_s, w_s);
+    G500_HIP(hipGetLastError());
+    sync();
+    ws->gpu_arena.used = mark;
+    if (failed()) return;
+    g->rowstarts_s = rs_s;
+    g->column_s = col_s;
+    g->weights_s = w_s;
+}
+
 // Builds g's traversal copy (perm, iperm, rowstarts_t, column_t, weights_t)
 // on the library stream; scratch is taken above the arena mark and released.
 void build_traversal_copy(Worker_State *ws, Oned_Graph *g) {
@@ -514,6 +767,7 @@ void build_traversal_copy(Worker_State *ws, Oned_Graph *g) {
     G500_HIP(hipGetLastError());
     sync();
     ws->gpu_arena.used = mark;
+    if (c.sssp_prune && !failed()) build_sssp_copy(ws, g, nlong, hc[0]);
 }
 
 }  // namespace g500

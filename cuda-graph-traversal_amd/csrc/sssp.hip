@@ -52,7 +52,8 @@ constexpr u32 kHub = 1024;
 constexpr u32 kStepLog = 1024;
 
 // Error bits: 1 barrier timeout, 2 hub list overflow, 4 queue entry out of
-// range, 8 queue length above n, 16 bucket invariant violated.
+// range, 8 queue length above n, 16 bucket invariant violated, 32 a distance
+// at or above the pruned SSSP copy's bound (the copy may then be inexact).
 struct SsspCtl {
     GridBarrier bar;
     // every counter that waves append to has a 256-byte line of its own
@@ -87,6 +88,7 @@ __device__ __forceinline__ void sync_traced(SsspCtl *ctl, u32 phase, u32 a, u32 
 
 struct SsspArgs {
     u64 n;
+    f32 dbound;  // distances at or above it set error 32 (+inf without the pruned copy)
     const u64 *rs;
     const u32 *col;
     const f32 *w;
@@ -690,6 +692,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
         const u64 l = a.lab[v];
         const bool reached = (u32)(l >> 32) < kInfBits;
         const u32 o = orig_id(a.iperm, (u32)v);
+        if (reached && label_dist(l) >= a.dbound) atomicOr(&ctl->bar.error, 32u);
         a.dist[o] = reached ? label_dist(l) : -1.0f;
         a.pred[o] = reached ? (i64)orig_id(a.iperm, (u32)l) : -1;
     }
@@ -784,7 +787,7 @@ void oned_graph_sssp_run(Worker_State *ws, const Oned_Graph *g, Sssp_State *s, u
         return;
     }
     const u64 n = g->nlocalverts;
-    const TraversalView tv = traversal_view(g);
+    const TraversalView tv = sssp_view(g);
     if (s->split_delta != s->delta || s->split_graph != (const void *)tv.rs) {  // light/heavy split of every row
         compute_split(tv.rs, tv.w, n, s->delta, s->split);
         s->split_delta = s->delta;
@@ -817,6 +820,7 @@ void oned_graph_sssp_run(Worker_State *ws, const Oned_Graph *g, Sssp_State *s, u
     a.dist = s->dist;
     a.root = root;
     a.delta = s->delta;
+    a.dbound = g->rowstarts_s ? kPruneDistBound : __builtin_inff();
     const int block = ctx().sssp_block;
     int grid = cooperative_blocks((const void *)sssp_kernel, block, 0);
     if (failed()) return;
@@ -836,7 +840,7 @@ void oned_graph_sssp_run(Worker_State *ws, const Oned_Graph *g, Sssp_State *s, u
         }
         const BlockTrace &tl = h.trace[at_lo], &th = h.trace[at_hi];
         fail("SSSP kernel error %u (1 barrier timeout, 2 hub list overflow, 4 bad queue entry, 8 queue overflow, "
-             "16 bucket invariant); %d blocks; barriers passed min %u (block %d phase %u tags %u/%u) max %u "
+             "16 bucket invariant, 32 distance at or above 2048 on the pruned SSSP copy: build with sssp_prune 0); %d blocks; barriers passed min %u (block %d phase %u tags %u/%u) max %u "
              "(block %d phase %u tags %u/%u)",
              h.bar.error, grid, lo, at_lo, tl.phase, tl.a, tl.b, hi, at_hi, th.phase, th.a, th.b);
     }

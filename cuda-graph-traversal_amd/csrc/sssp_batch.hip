@@ -55,7 +55,8 @@ constexpr u32 kBStepLog = 4096;
 constexpr int kDefer = 3;  // deferred edge classes: 0 mid, 1 long, 2 far
 
 // Error bits: 1 barrier timeout, 2 hub list overflow, 4 queue entry out of
-// range, 8 queue length above n, 16 bucket invariant violated.
+// range, 8 queue length above n, 16 bucket invariant violated, 32 a distance
+// at or above the pruned SSSP copy's bound (the copy may then be inexact).
 struct SsspBatchCtl {
     GridBarrier bar;
     // Every counter that waves append to has a 256-byte line of its own, so
@@ -117,6 +118,7 @@ struct SsspBatchArgs {
     u64 roots[kBatchRoots];
     i32 nroots;
     f32 delta;
+    f32 dbound;        // distances at or above it set error 32 (+inf without the pruned copy)
     f32 dw[kDefer];    // class floor weights delta <= W_0 (mid) <= W_1 (long) <= W_2 (far)
 };
 
@@ -341,12 +343,24 @@ __device__ __forceinline__ void relax_commit_push(const SsspBatchArgs &a, const 
                                                   const Probe (&pr)[R], const bool (&act)[R], const BTarget &t,
                                                   u64 &improved, WaveQ &wq) {
     bool tried[R];
-    u32 prev[R], out[R];
+    u32 prev[R], out[R], ow[R], lw[R];
 #pragma unroll
     for (int h = 0; h < R; ++h) tried[h] = act[h] && pr[h].cur > pr[h].cb;
 #pragma unroll
     for (int h = 0; h < R; ++h)
         prev[h] = tried[h] ? atomicMin(&a.dl[(u64)pr[h].v * kBatchRoots + r[h]], pr[h].cb) : 0u;
+    // Reads that depend on the target alone go out with the probe atomics, not
+    // after them, so that the chain of dependent accesses is shorter. The
+    // open-bit word (its 32-bit half holding v) is read when the plain probe
+    // read showed v unreached from r: probes only fall, so only then can the
+    // atomic report it unreached. The light split is read when the candidate
+    // lands below t.below, the only case in which v may be queued.
+#pragma unroll
+    for (int h = 0; h < R; ++h)
+        ow[h] = (tried[h] && pr[h].cur >= kInfBits) ? ((const u32 *)a.openw)[pr[h].v >> 5] : ~0u;
+#pragma unroll
+    for (int h = 0; h < R; ++h)
+        lw[h] = (tried[h] && __uint_as_float(pr[h].cb) < t.below) ? a.split[pr[h].v] : 0u;
     u32 nimp = 0;  // lowered probes of this lane; summed per wave below (a wave-uniform counter)
 #pragma unroll
     for (int h = 0; h < R; ++h) {
@@ -358,9 +372,10 @@ __device__ __forceinline__ void relax_commit_push(const SsspBatchArgs &a, const 
             atomicMin(&a.vmin[v], cb);
             if (prev[h] >= kInfBits) {  // v reached from root r: it has open work (again)
                 // bits are cleared only by dense scans, never during a relaxation phase, so a
-                // set bit read here is current; a clear one may race a concurrent OR, which is harmless
-                const u64 bit = 1ull << (v & 63);
-                if (!(a.openw[v >> 6] & bit)) atomicOr((unsigned long long *)&a.openw[v >> 6], bit);
+                // set bit read in this phase is current; a clear one may race a concurrent OR,
+                // which is harmless
+                const u32 bit = 1u << (v & 31);
+                if (!(ow[h] & bit)) atomicOr(&((u32 *)a.openw)[v >> 5], bit);
             }
             ++nimp;
             if (cand < t.floor) atomicOr(&a.ctl->bar.error, 16u);
@@ -378,9 +393,9 @@ __device__ __forceinline__ void relax_commit_push(const SsspBatchArgs &a, const 
         st_relaxed(&ctl->nnext[t.slot], 1u);
     // a vertex without light entries has nothing to do in light steps
     u32 *pend = t.buf ? a.pend[1] : a.pend[0];
-    u32 lw[R], po[R];
+    u32 po[R];
 #pragma unroll
-    for (int h = 0; h < R; ++h) lw[h] = out[h] ? a.split[pr[h].v] : 0u;
+    for (int h = 0; h < R; ++h) lw[h] = out[h] ? lw[h] : 0u;
 #pragma unroll
     for (int h = 0; h < R; ++h) po[h] = (out[h] && lw[h]) ? atomicOr(&pend[pr[h].v], out[h]) : 1u;
 #pragma unroll
@@ -1073,6 +1088,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
         const u64 l = a.lab[tt];
         const bool reached = (u32)(l >> 32) < kInfBits;
         const u64 o = (u64)r * a.n + orig_id(a.iperm, (u32)v);
+        if (reached && label_dist(l) >= a.dbound) atomicOr(&ctl->bar.error, 32u);
         a.dist[o] = reached ? label_dist(l) : -1.0f;
         a.pred[o] = reached ? (i64)orig_id(a.iperm, (u32)l) : -1;
     }
@@ -1181,7 +1197,7 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
     const bool library = stream == nullptr;
     const hipStream_t st = library ? c.stream : stream;
     // the split depends on the graph too: a state reused across graphs recomputes it
-    const TraversalView tv = traversal_view(g);
+    const TraversalView tv = sssp_view(g);
     if (s->split_delta != s->delta || s->long_split_weight != wlong || s->mid_split_weight != wmid ||
         s->far_split_weight != wfar || s->split_graph != (const void *)tv.rs) {
         compute_split(tv.rs, tv.w, n, s->delta, s->split, st);
@@ -1245,6 +1261,7 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
     for (i32 i = 0; i < nroots; ++i) a.roots[i] = roots[i];
     a.nroots = nroots;
     a.delta = s->delta;
+    a.dbound = g->rowstarts_s ? kPruneDistBound : __builtin_inff();
     a.dw[0] = wmid;
     a.dw[1] = wlong;
     a.dw[2] = wfar;
@@ -1325,7 +1342,8 @@ void oned_graph_sssp_batch_run(Worker_State *ws, const Oned_Graph *g, Sssp_Batch
     const u32 e = sssp_batch_error(s, nullptr);
     if (e)
         fail("batched SSSP kernel error %u (1 barrier timeout, 2 hub list overflow, 4 bad queue entry, 8 queue "
-             "overflow, 16 bucket invariant)",
+             "overflow, 16 bucket invariant, 32 distance at or above 2048 on the pruned SSSP copy: build with "
+             "sssp_prune 0)",
              e);
 }
 

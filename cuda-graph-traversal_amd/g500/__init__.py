@@ -40,7 +40,8 @@ class TupleGraph(C.Structure):
 class OnedGraph(C.Structure):
     _fields_ = [("nlocalverts", u64), ("nlocaledges", u64), ("nglobalverts", u64), ("notisolated", u64),
                 ("rowstarts", vp), ("column", vp), ("weights", vp), ("perm", vp), ("iperm", vp),
-                ("rowstarts_t", vp), ("column_t", vp), ("weights_t", vp)]
+                ("rowstarts_t", vp), ("column_t", vp), ("weights_t", vp),
+                ("rowstarts_s", vp), ("column_s", vp), ("weights_s", vp)]
 
 
 class BfsState(C.Structure):

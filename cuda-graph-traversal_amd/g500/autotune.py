@@ -19,8 +19,8 @@ BATCH_ALPHA = [4.0, 15.0, 60.0]
 BATCH_HEAVY = [4096, 1 << 20]
 SSSP_DELTA = [0.025, 0.05, 0.1]
 # (delta, mid, long, far); far == long leaves the far class empty
-BATCH_SSSP = [(0.025, 0.075, 0.25, 0.5), (0.025, 0.075, 0.25, 0.25), (0.0125, 0.05, 0.2, 0.5),
-              (0.05, 0.1, 0.3, 0.6)]
+BATCH_SSSP = [(0.025, 0.075, 0.25, 0.5), (0.025, 0.05, 0.15, 0.3), (0.025, 0.075, 0.25, 0.25),
+              (0.0125, 0.05, 0.2, 0.5), (0.05, 0.1, 0.3, 0.6)]
 
 
 def tuning_roots(sess, g, count, seed=2024):

@@ -53,6 +53,17 @@ struct Oned_Graph {
     u64 *rowstarts_t;   /* [n + 1] */
     u32 *column_t;      /* [nlocaledges] traversal ids */
     f32 *weights_t;     /* [nlocaledges] */
+    /* SSSP copy, built after the traversal copy (all NULL without it, or with
+     * g500_set_param("sssp_prune", 0)): the traversal rows, same order,
+     * without each entry (u, v, w) that a two-hop path u-x-v through one of
+     * the 32 highest-degree vertices beats by a margin,
+     * w(u,x) + w(x,v) + 2^-10 < w (w < 2). No such entry is needed by any shortest path while distances
+     * stay below 2048, so both SSSP kernels relax this copy (and report error
+     * 32 on a distance at or above 2048); BFS, validation and the TEPS counts
+     * use the full rows. */
+    u64 *rowstarts_s;   /* [n + 1] */
+    u32 *column_s;      /* [rowstarts_s[n]] traversal ids */
+    f32 *weights_s;     /* [rowstarts_s[n]] */
 };
 
 /* traversal.h:18-27 */

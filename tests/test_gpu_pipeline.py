@@ -113,7 +113,8 @@ def test_csr_edge_cases(arena, oracle):
     cases.append((3, np.array([[0, 1], [1, 2], [2, 0], [0, 1]], np.uint64), np.array([.1, .2, .3, .1], np.float32)))
     for n, e, w in cases:
         say("csr case", n, len(e))
-        check_csr(arena, oracle, n, e, w)
+        g = check_csr(arena, oracle, n, e, w)
+        check_sssp_copy(arena, g)
 
 
 def test_traversal_copy_is_a_relabeling(arena, oracle):
@@ -138,13 +139,93 @@ def test_traversal_copy_is_a_relabeling(arena, oracle):
     assert np.array_equal(w_t.view(np.uint32), ref.weights[src].view(np.uint32))
 
 
+def check_sssp_copy(sess, g):
+    """The SSSP copy is the traversal copy without exactly the entries that
+    the two-hop rule drops (hubs: traversal ids 0..31), rows and entry order
+    otherwise unchanged. Returns (kept, total) entries."""
+    from conftest import drop_entries, prune_rule
+    n, m = g.nlocalverts, g.nlocaledges
+    assert g.rowstarts_s and g.column_s and g.weights_s
+    rs_t = sess.to_host(g.rowstarts_t, n + 1, np.uint64)
+    col_t = sess.to_host(g.column_t, m, np.uint32)
+    w_t = sess.to_host(g.weights_t, m, np.float32)
+    # hubs: the 32 largest degrees of the traversal copy, ties to the smaller id
+    hubs = np.argsort(-np.diff(rs_t.astype(np.int64)), kind="stable")[:32]
+    drop = prune_rule(rs_t, col_t, w_t, hubs)
+    rs_k, col_k, w_k = drop_entries(rs_t, col_t, w_t, drop)
+    rs_s = sess.to_host(g.rowstarts_s, n + 1, np.uint64)
+    assert np.array_equal(rs_s, rs_k)
+    m_s = int(rs_s[-1])
+    assert np.array_equal(sess.to_host(g.column_s, m_s, np.uint32), col_k)
+    assert np.array_equal(sess.to_host(g.weights_s, m_s, np.float32).view(np.uint32), w_k.view(np.uint32))
+    return m_s, m
+
+
+@pytest.mark.parametrize("scale", [12, 14])
+def test_sssp_copy_drops_two_hop_dominated_entries(arena, oracle, scale):
+    e, w, ref = oracle_graph(scale)
+    g = check_csr(arena, oracle, 1 << scale, e, w)
+    kept, total = check_sssp_copy(arena, g)
+    say("sssp copy", scale, "kept", kept, "of", total)
+    assert kept < 0.8 * total
+
+
+def test_sssp_copy_hubs_above_exact_degree_range(arena, oracle):
+    """40 vertices of degree above 4095 share one degree class of the
+    traversal copy in no particular order: the SSSP copy still takes the 32
+    largest degrees as hubs, and SSSP distances match the oracle."""
+    rng = np.random.default_rng(11)
+    n = 60000
+    hubs = rng.choice(n, 40, replace=False)
+    parts = []
+    for i, h in enumerate(hubs):
+        d = 4100 + 97 * i
+        parts.append(np.stack([np.full(d, h, np.uint64), rng.integers(0, n, d).astype(np.uint64)], 1))
+    parts.append(rng.integers(0, n, (120000, 2)).astype(np.uint64))
+    e = np.concatenate(parts)
+    w = rng.random(len(e)).astype(np.float32)
+    g = check_csr(arena, oracle, n, e, w)
+    kept, total = check_sssp_copy(arena, g)
+    say("sssp copy, 40 hubs", kept, "of", total)
+    assert kept < total
+    ref = oracle.csr(n, e, w)
+    s = arena.sssp_create(g)
+    for r in (int(hubs[0]), int(e[-1, 0])):
+        arena.sssp(g, s, r)
+        d, p = arena.sssp_result(s)
+        d_ref, _ = oracle.dijkstra_f32(ref, r)
+        assert np.array_equal(d.view(np.uint32), d_ref.view(np.uint32))
+        assert oracle.sssp_check(ref, r, d, p) == 0
+
+
+def test_without_sssp_copy(arena, oracle):
+    """With pruning switched off both SSSP kernels relax the traversal copy."""
+    e, w, ref = oracle_graph(12)
+    arena.set_param("sssp_prune", 0)
+    try:
+        g = check_csr(arena, oracle, 1 << 12, e, w)
+        assert g.perm and not g.rowstarts_s and not g.column_s
+        s = arena.sssp_create(g)
+        sb = arena.sssp_batch_create(g)
+        roots = oracle.roots(ref)
+        for r in roots[:3]:
+            arena.sssp(g, s, int(r))
+            d, p = arena.sssp_result(s)
+            d_ref, _ = oracle.dijkstra_f32(ref, int(r))
+            assert np.array_equal(d.view(np.uint32), d_ref.view(np.uint32))
+            assert oracle.sssp_check(ref, int(r), d, p) == 0
+        sssp_batch_check(arena, oracle, g, ref, sb, roots[:16])
+    finally:
+        arena.set_param("sssp_prune", 1)
+
+
 def test_without_traversal_copy(arena, oracle):
     """With the copy switched off the kernels run on the CSR itself."""
     e, w, ref = oracle_graph(12)
     arena.set_param("relabel", 0)
     try:
         g = check_csr(arena, oracle, 1 << 12, e, w)
-        assert not g.perm and not g.column_t
+        assert not g.perm and not g.column_t and not g.rowstarts_s
         arena.bfs_state = arena.bfs_create(g)
         s = arena.sssp_create(g)
         sb = arena.sssp_batch_create(g)

@@ -278,3 +278,59 @@ def test_statistics(oracle, n):
     if n in (5, 64):
         assert np.isclose(s[1], np.percentile(x, 25, method="midpoint"))
         assert np.isclose(s[3], np.percentile(x, 75, method="midpoint"))
+
+
+# ------------------------------------------------------- two-hop pruned SSSP
+@pytest.mark.parametrize("scale,hubset", [(10, "top"), (12, "top"), (12, "random")])
+def test_two_hop_pruning_keeps_f32_distances(oracle, scale, hubset):
+    """Dropping the entries that a two-hop path through a hub beats by more
+    than 2^-10 leaves every f32 Dijkstra distance bit-identical, for any hub
+    set; the results still pass the certificate on the full graph."""
+    from conftest import drop_entries, prune_rule
+    e, w, g = oracle_graph(scale)
+    deg = np.diff(g.rowstarts.astype(np.int64))
+    if hubset == "top":
+        hubs = np.argsort(-deg, kind="stable")[:32]
+    else:
+        hubs = np.random.default_rng(5).choice(g.n, 32, replace=False)
+    drop = prune_rule(g.rowstarts, g.column, g.weights, hubs)
+    assert drop.sum() > (0.2 * g.m if hubset == "top" else 0)
+    rs_k, col_k, w_k = drop_entries(g.rowstarts, g.column, g.weights, drop)
+    pruned = oracle.Csr(g.n, rs_k, col_k.astype(np.uint32), w_k.astype(np.float32))
+    for r in oracle.roots(g)[:6]:
+        d_full, _ = oracle.dijkstra_f32(g, int(r))
+        d_pr, p_pr = oracle.dijkstra_f32(pruned, int(r))
+        assert np.array_equal(d_full.view(np.uint32), d_pr.view(np.uint32))
+        assert oracle.sssp_check(g, int(r), d_pr, p_pr) == 0
+
+
+def test_two_hop_rule_pins(oracle):
+    """Hand cases of the rule (dyadic weights, exact in f32 and f64): the path
+    1-0-2 through hub 0 (0.125 + 0.25) beats a direct entry 1-2 only by more
+    than the margin 2^-10, and never an entry of weight >= 2."""
+    from conftest import prune_rule
+    m = 2.0 ** -10
+    edges = [(1, 0, 0.125), (2, 0, 0.25),
+             (1, 2, 0.375 + m),            # beaten by exactly the margin: kept
+             (1, 2, 0.375 + m + 2 ** -20),  # beaten by more: dropped
+             (1, 2, 0.5),                  # dropped
+             (1, 2, 2.5)]                  # w >= 2: kept
+    n = 3
+    rows = [[] for _ in range(n)]
+    for u, v, x in edges:
+        rows[u].append((x, v))
+        rows[v].append((x, u))
+    rs = np.zeros(n + 1, np.int64)
+    col, wt = [], []
+    for u in range(n):
+        for x, v in sorted(rows[u]):
+            col.append(v)
+            wt.append(x)
+        rs[u + 1] = len(col)
+    wt = np.array(wt, np.float32)
+    assert np.array_equal(wt.astype(np.float64), np.array([x for u in range(n) for x, _ in sorted(rows[u])]))
+    drop = prune_rule(rs, np.array(col), wt, [0])
+    dropped = sorted(float(wt[k]) for u in range(n) for k in range(rs[u], rs[u + 1]) if drop[k] and u == 1)
+    assert dropped == [0.375 + m + 2 ** -20, 0.5]
+    # the reverse entries (row 2) get the same decisions
+    assert sorted(float(wt[k]) for k in range(rs[2], rs[3]) if drop[k]) == dropped
