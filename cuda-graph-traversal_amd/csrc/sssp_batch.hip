@@ -84,6 +84,29 @@ struct SsspBatchCtl {
     alignas(256) BlockTrace trace[kMaxTraceBlocks];
 };
 
+// A row's segment bounds in one 32-byte record, so that an item's segment
+// costs one memory transaction: row start, the light, mid, long and far
+// splits (entries below delta, W_mid, W_long, W_far, from the start), row end.
+struct RowInfo {
+    u64 start;
+    u32 light, cut[kDefer];
+    u64 end;
+};
+static_assert(sizeof(RowInfo) == 32, "one 32-byte record per row");
+
+__device__ __forceinline__ RowInfo load_row_info(const RowInfo *ri, u32 u) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(ri + u);
+    const uint4 x = p[0], y = p[1];
+    RowInfo r;
+    r.start = ((u64)x.y << 32) | x.x;
+    r.light = x.z;
+    r.cut[0] = x.w;
+    r.cut[1] = y.x;
+    r.cut[2] = y.y;
+    r.end = ((u64)y.w << 32) | y.z;
+    return r;
+}
+
 struct SsspBatchArgs {
     u64 n;
     const u64 *rs;
@@ -91,6 +114,7 @@ struct SsspBatchArgs {
     const f32 *w;
     const u32 *split;
     const u32 *dsplit[kDefer];  // [n] entries with w < W_c: class c is [dsplit[c], dsplit[c+1]), the last to the row end
+    const struct RowInfo *ri;   // [n] the row bounds and splits above in one 32-byte record
     const u32 *perm, *iperm;  // traversal <-> original ids (null: identity)
     u64 *lab;          // [n * kBatchRoots]
     u32 *dl;           // [n * kBatchRoots] probes (aliases dist)
@@ -432,17 +456,17 @@ enum : int { kSegLight = 0, kSegMedium = 1, kSegDef = 2 };
 
 template <int K>
 __device__ __forceinline__ void bsegment(const SsspBatchArgs &a, u32 u, u64 &b, u64 &e) {
-    const u64 r0 = a.rs[u];
+    const RowInfo r = load_row_info(a.ri, u);
     if (K == kSegLight) {
-        b = r0;
-        e = r0 + a.split[u];
+        b = r.start;
+        e = r.start + r.light;
     } else if (K == kSegMedium) {
-        b = r0 + a.split[u];
-        e = r0 + a.dsplit[0][u];
+        b = r.start + r.light;
+        e = r.start + r.cut[0];
     } else {
         constexpr int c = K - kSegDef;
-        b = r0 + a.dsplit[c][u];
-        e = c + 1 < kDefer ? r0 + a.dsplit[c + 1 < kDefer ? c + 1 : c][u] : a.rs[u + 1];
+        b = r.start + r.cut[c];
+        e = c + 1 < kDefer ? r.start + r.cut[c + 1 < kDefer ? c + 1 : c] : r.end;
     }
 }
 
@@ -901,11 +925,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                 u64 b = 0, e = 0;
                 u64 cut[kDefer + 1];  // class c is [cut[c], cut[c + 1])
                 if (H) {
-                    bsegment<kSegMedium>(a, (u32)v, b, e);
-                    const u64 r0 = a.rs[v];
+                    const RowInfo r = load_row_info(a.ri, (u32)v);
+                    b = r.start + r.light;
+                    e = r.start + r.cut[0];
 #pragma unroll
-                    for (int c = 0; c < kDefer; ++c) cut[c] = r0 + a.dsplit[c][v];
-                    cut[kDefer] = a.rs[v + 1];
+                    for (int c = 0; c < kDefer; ++c) cut[c] = r.start + r.cut[c];
+                    cut[kDefer] = r.end;
                 } else {
 #pragma unroll
                     for (int c = 0; c <= kDefer; ++c) cut[c] = 0;
@@ -1094,6 +1119,25 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
     }
 }
 
+// Packs each row's bounds and splits into its 32-byte record.
+__global__ __launch_bounds__(256) void sssp_batch_rowinfo_kernel(const u64 *__restrict__ rs,
+                                                                 const u32 *__restrict__ split,
+                                                                 const u32 *__restrict__ d0,
+                                                                 const u32 *__restrict__ d1,
+                                                                 const u32 *__restrict__ d2, u64 n,
+                                                                 RowInfo *__restrict__ ri) {
+    for (u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (u64)gridDim.x * blockDim.x) {
+        RowInfo r;
+        r.start = rs[v];
+        r.light = split[v];
+        r.cut[0] = d0[v];
+        r.cut[1] = d1[v];
+        r.cut[2] = d2[v];
+        r.end = rs[v + 1];
+        ri[v] = r;
+    }
+}
+
 __global__ __launch_bounds__(256) void sssp_batch_clear_kernel(u64 *__restrict__ lab, u32 *__restrict__ dl,
                                                                u32 *__restrict__ pend, u32 *__restrict__ done,
                                                                u32 *__restrict__ masks, u32 *__restrict__ mwork,
@@ -1204,6 +1248,9 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
         compute_split(tv.rs, tv.w, n, wmid, s->mid_split, st);
         compute_split(tv.rs, tv.w, n, wlong, s->long_split, st);
         compute_split(tv.rs, tv.w, n, wfar, s->far_split, st);
+        sssp_batch_rowinfo_kernel<<<grid_for(n, 256, (u64)c.num_cus * 32), 256, 0, st>>>(
+            tv.rs, s->split, s->mid_split, s->long_split, s->far_split, n, (RowInfo *)s->row_info);
+        G500_HIP(hipGetLastError());
         s->split_delta = s->delta;
         s->mid_split_weight = wmid;
         s->long_split_weight = wlong;
@@ -1228,6 +1275,7 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
     a.dsplit[0] = s->mid_split;
     a.dsplit[1] = s->long_split;
     a.dsplit[2] = s->far_split;
+    a.ri = (const RowInfo *)s->row_info;
     u32 *dwork[kDefer] = {s->mid_work, s->long_work, s->far_work};
     u64 *doff[kDefer] = {s->mid_offsets, s->long_offsets, s->far_offsets};
     for (int d = 0; d < kDefer; ++d) {
@@ -1332,6 +1380,7 @@ Sssp_Batch_State oned_graph_sssp_batch_create(Worker_State *ws, const Oned_Graph
     s.long_work = (u32 *)arena_alloc(ws, 6 * n * sizeof(u32), "batched SSSP deferred long-edge queue");
     s.long_offsets = (u64 *)arena_alloc(ws, n * sizeof(u64), "batched SSSP deferred hub offsets");
     s.ctl = arena_alloc(ws, sizeof(SsspBatchCtl), "batched SSSP control");
+    s.row_info = arena_alloc(ws, n * sizeof(RowInfo), "batched SSSP row records");
     return s;
 }
 

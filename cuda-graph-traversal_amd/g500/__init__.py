@@ -71,7 +71,7 @@ class SsspBatchState(C.Structure):
                 ("ctl", vp), ("split_graph", vp), ("delta", f32), ("split_delta", f32), ("long_weight", f32),
                 ("long_split_weight", f32), ("mid_weight", f32), ("mid_split_weight", f32), ("far_weight", f32),
                 ("far_split_weight", f32), ("nroots", i32), ("reserved", i32),
-                ("max_nlocalverts", u64)]
+                ("max_nlocalverts", u64), ("row_info", vp)]
 
 
 class Completion(C.Structure):

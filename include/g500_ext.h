@@ -127,6 +127,7 @@ struct Sssp_Batch_State {
     i32 nroots;         /* roots of the last run */
     i32 reserved;
     u64 max_nlocalverts;
+    void *row_info;     /* [n] 32-byte row records: start, light, mid, long and far splits, end */
 };
 Sssp_Batch_State oned_graph_sssp_batch_create(Worker_State *ws, const Oned_Graph *g);
 /* roots: host array of nroots (1..32, g500_get_param("sssp_batch_max")) vertex ids (repeats allowed). */
