@@ -481,7 +481,7 @@ __device__ __forceinline__ void bsegment(const SsspBatchArgs &a, u32 u, u64 &b, 
 template <int K>
 __device__ __forceinline__ void group_tier(const SsspBatchArgs &a, u32 *pend, const u32 *lq, const u64 *hq, u32 count,
                            const BTarget &t, u64 &improved, volatile int *slot, u64 *wk, WaveQ &wq) {
-    const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
+    const u64 gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
     const u64 wave_gid = wave_index(), nwaves = gthreads / kWave;
     for (u64 i0 = wave_gid * kWave; i0 < count; i0 += nwaves * kWave) {
@@ -573,7 +573,7 @@ __device__ __forceinline__ PairLanes pair_lanes(u32 M) {
 template <int K>
 __device__ __forceinline__ void wave_tier(const SsspBatchArgs &a, u32 *pend, const u32 *lq, const u64 *hq, u32 count,
                           const BTarget &t, u64 &improved, u64 *wk, WaveQ &wq) {
-    const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
+    const u64 gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
     const u64 wave_gid = wave_index(), nwaves = gthreads / kWave;
     for (u64 i = wave_gid; i < count; i += nwaves) {
@@ -628,7 +628,7 @@ __device__ __forceinline__ void wave_tier(const SsspBatchArgs &a, u32 *pend, con
 template <int K>
 __device__ __forceinline__ void hub_tier(const SsspBatchArgs &a, const u32 *hv, const u64 *hoff, const u32 *hlen, u32 *aux,
                          u32 *pend, u64 packed, const BTarget &t, u64 &improved, u64 *wk, WaveQ &wq) {
-    const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
+    const u64 gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
     const u64 wave_gid = wave_index(), nwaves = gthreads / kWave;
     const u64 count = hub_count(packed), total = hub_total(packed);
