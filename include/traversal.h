@@ -14,13 +14,15 @@
  *     (weight, column) order, so the CSR is deterministic (the reference's
  *     in-row order depends on atomic arrival, traversal.cu:51).
  *   - Oned_Graph also carries a degree-ordered traversal copy of the CSR
- *     (fields after `weights`); results are reported in the original ids.
+ *     (fields after `weights`) and an SSSP copy of it without two-hop-
+ *     dominated entries; results are reported in the original ids.
  *   - Bfs_State keeps pred/dist/max_nlocalverts; the frontier counters and
  *     grid-barrier words (traversal.h:23-26) live in the device control block
  *     `ctl`; frontiers are u32 queues plus bitmaps for bottom-up levels.
  *   - Sssp_State keeps pred/dist/max_nlocalverts; the working labels are one
- *     packed u64 per vertex (f32 distance bits << 32 | parent), updated with
- *     a compare-and-swap loop instead of dist_bits + pred_lock
+ *     packed u64 per vertex (f32 distance bits << 32 | parent), lowered by a
+ *     64-bit atomic min once a u32 distance probe (the dist array while the
+ *     kernel runs) is strictly lowered, instead of dist_bits + pred_lock
  *     (traversal.cu:211-240); delta is a field, 0.1f by default
  *     (traversal.cu:645).
  */
