@@ -198,6 +198,43 @@ def test_sssp_copy_hubs_above_exact_degree_range(arena, oracle):
         assert oracle.sssp_check(ref, r, d, p) == 0
 
 
+def test_sssp_copy_distance_bound(arena, oracle):
+    """Distances at or above 2048 void the SSSP copy's guarantee: on a path of
+    2199 edges of weight 0.99 both SSSP kernels finish and report error 32;
+    built without the copy, the same graph runs exact."""
+    import g500
+    n = 2200
+    e = np.stack([np.arange(n - 1, dtype=np.uint64), np.arange(1, n, dtype=np.uint64)], 1)
+    w = np.full(n - 1, 0.99, np.float32)
+    ref = oracle.csr(n, e, w)
+    g = check_csr(arena, oracle, n, e, w)
+    assert g.rowstarts_s
+    s = arena.sssp_create(g)
+    s.delta = 1.0
+    with pytest.raises(g500.LibraryError, match="error 32 "):
+        arena.sssp(g, s, 0)
+    sb = arena.sssp_batch_create(g)
+    sb.delta = 1.0
+    with pytest.raises(g500.LibraryError, match="error 32 "):
+        arena.sssp_batch(g, sb, np.array([0, 7], np.uint64))
+    arena.set_param("sssp_prune", 0)
+    try:
+        g = check_csr(arena, oracle, n, e, w)
+        assert not g.rowstarts_s
+        s = arena.sssp_create(g)
+        s.delta = 1.0
+        arena.sssp(g, s, 0)
+        d, p = arena.sssp_result(s)
+        d_ref, _ = oracle.dijkstra_f32(ref, 0)
+        assert np.array_equal(d.view(np.uint32), d_ref.view(np.uint32)) and d.max() > 2048
+        assert oracle.sssp_check(ref, 0, d, p) == 0
+        sb = arena.sssp_batch_create(g)
+        sb.delta = 1.0
+        sssp_batch_check(arena, oracle, g, ref, sb, np.array([0, 7], np.uint64))
+    finally:
+        arena.set_param("sssp_prune", 1)
+
+
 def test_without_sssp_copy(arena, oracle):
     """With pruning switched off both SSSP kernels relax the traversal copy."""
     e, w, ref = oracle_graph(12)
