@@ -11,6 +11,7 @@
 // (<= 64), a block in LDS (<= 4096), or a block-wide LSD radix sort through a
 // scratch buffer (longer rows).
 #include <algorithm>
+#include <cstdio>
 #include <vector>
 
 #include "common.hpp"
@@ -647,6 +648,15 @@ void build_sssp_copy(Worker_State *ws, Oned_Graph *g, u64 nlong, u64 ntop) {
     Context &c = ctx();
     const hipStream_t s = c.stream;
     const u64 n = g->nlocalverts;
+    // The copy is an optimization: without arena room for its largest moment
+    // (row offsets, entries at full size, hub table, degrees, alignment slack)
+    // the graph keeps none, and SSSP relaxes the traversal copy.
+    const u64 need = (n + 1) * 8 + g->nlocaledges * 8 + n * kPruneHubs * 4 + n * 8 + (8u << 20);
+    if (ws->gpu_arena.reserved - ws->gpu_arena.used < need) {
+        fprintf(stderr, "libg500: no arena room for the SSSP copy (%llu bytes); SSSP relaxes the traversal copy\n",
+                (unsigned long long)need);
+        return;
+    }
     // the hubs: the kPruneHubs largest degrees among the first max(ntop, kPruneHubs)
     // rows (every later row is no longer), ties to the smaller id
     const u64 nhead = MIN(n, MAX(ntop, (u64)kPruneHubs));

@@ -235,6 +235,27 @@ def test_sssp_copy_distance_bound(arena, oracle):
         arena.set_param("sssp_prune", 1)
 
 
+def test_sssp_copy_skipped_without_arena_room(arena, oracle):
+    """The SSSP copy is an optimization: with arena room for the CSR and the
+    traversal copy but not for the SSSP copy's hub table (128 B per vertex),
+    the graph keeps no SSSP copy and SSSP relaxes the traversal copy."""
+    e, w, ref = oracle_graph(12)
+    a = arena.ws.gpu_arena
+    saved = a.reserved
+    a.reserved = a.used + (6 << 20)  # the SSSP copy needs about 9.6 MB here
+    try:
+        g = check_csr(arena, oracle, 1 << 12, e, w)
+    finally:
+        a.reserved = saved
+    assert g.perm and not g.rowstarts_s
+    s = arena.sssp_create(g)
+    for r in oracle.roots(ref)[:2]:
+        arena.sssp(g, s, int(r))
+        d, p = arena.sssp_result(s)
+        d_ref, _ = oracle.dijkstra_f32(ref, int(r))
+        assert np.array_equal(d.view(np.uint32), d_ref.view(np.uint32))
+
+
 def test_without_sssp_copy(arena, oracle):
     """With pruning switched off both SSSP kernels relax the traversal copy."""
     e, w, ref = oracle_graph(12)
