@@ -202,7 +202,9 @@ void slot_loop(G500_Server *s, int k) {
                     bfs_batch_depth_enqueue(g, &slot.bfs, (i32)i, (i64 *)(s->pool_base + batch[i]->off0), slot.stream);
             bfs_batch_summary_enqueue(&slot.bfs, slot.summary, slot.stream);
         } else {
-            slot.sssp.delta = s->delta;  // block shape and grid share were fixed at start
+            // block shape and grid share were fixed at start; batches of more than
+            // 16 roots share more work per bucket and run at half the server's delta
+            slot.sssp.delta = roots.size() > 16 ? 0.5f * s->delta : s->delta;
             sssp_batch_enqueue(g, &slot.sssp, roots.data(), (i32)roots.size(), slot.stream, s->grid_sssp);
             err = sssp_batch_error(&slot.sssp, slot.stream);
             for (size_t i = 0; i < batch.size(); ++i) {

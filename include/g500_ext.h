@@ -185,7 +185,8 @@ G500_Server *g500_server_create(Worker_State *ws, i32 slots, u64 result_pool_byt
 /* Registers a resident graph (before start); returns its id. */
 i32 g500_server_add_graph(G500_Server *s, const Oned_Graph *g);
 /* Allocates the slot states (sized for the largest graph) and starts the
- * threads; delta is the SSSP bucket width. Returns 0, or -1 on error. */
+ * threads; delta is the SSSP bucket width of batches of up to 16 roots
+ * (larger batches run at delta / 2). Returns 0, or -1 on error. */
 i32 g500_server_start(G500_Server *s, f32 delta);
 /* Queues a request; thread-safe. out0: host buffer for BFS depths (i64[n]) or
  * SSSP distances (f32[n]); out1: SSSP parents (i64[n]); either may be NULL
