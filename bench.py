@@ -78,11 +78,11 @@ def main():
     ap.add_argument("--scale", type=int, default=26)
     ap.add_argument("--edgefactor", type=int, default=16)
     ap.add_argument("--delta", type=float, default=0.1, help="SSSP delta when not tuned")
-    ap.add_argument("--long-weight", type=float, default=0.25,
+    ap.add_argument("--long-weight", type=float, default=0.15,
                     help="batched SSSP long-edge threshold (deferred over windows this wide) when not tuned")
-    ap.add_argument("--mid-weight", type=float, default=0.075,
+    ap.add_argument("--mid-weight", type=float, default=0.05,
                     help="batched SSSP mid-edge threshold (deferred over windows this wide) when not tuned")
-    ap.add_argument("--far-weight", type=float, default=0.5,
+    ap.add_argument("--far-weight", type=float, default=0.3,
                     help="batched SSSP far-edge threshold (deferred over windows this wide) when not tuned")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (knobs at their defaults, delta from --delta)")
