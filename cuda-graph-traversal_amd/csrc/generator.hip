@@ -1,6 +1,6 @@
 // Graph500 Kronecker generator: MRG skip table, host RNG entry points, the
 // gfx950 edge kernel and the raw tuple dump/load format.
-// Reference: src/graph_generation.cu:96-441, src/mrg_transitions.cu:10-14.
+// Reference: src/graph_generation.cu:6-7, :100-396, src/mrg_transitions.cu:10-14.
 #include <fcntl.h>
 #include <unistd.h>
 
@@ -103,7 +103,7 @@ void init_mrg_table() {
     }
 }
 
-// graph_generation.cu:254-268: skip hi*2^128 + mid*2^64 + lo steps.
+// graph_generation.cu:209-223: skip hi*2^128 + mid*2^64 + lo steps.
 static void host_skip(Mrg_State &st, u64 hi, u64 mid, u64 lo) {
     build_host_table();
     const u64 parts[3] = {lo, mid, hi};
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void kronecker_kernel(
         const i64 ei = tile * 256 + threadIdx.x;
         if (ei < start || ei >= end) continue;
         if (threadIdx.x) mrg_apply(T[8 * 256 + threadIdx.x], st);
-        // graph_generation.cu:159-188, :231-248: one quadrant per level,
+        // graph_generation.cu:114-143, :186-207: one quadrant per level,
         // rejecting u < 0x7FFFFFFF mod 10000.
         u64 src = 0, tgt = 0;
         for (int lvl = logN - 1; lvl >= 0; --lvl) {
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(256) void kronecker_kernel(
         e.v0 = scramble(src, logN, sum01, mul0, mul1);
         e.v1 = scramble(tgt, logN, sum01, mul0, mul1);
         edges[ei - start] = e;
-        weights[ei - start] = (f32)mrg_next(st) * 0x1p-31f;  // graph_generation.cu:133-135
+        weights[ei - start] = (f32)mrg_next(st) * 0x1p-31f;  // graph_generation.cu:88-90
     }
 }
 
@@ -172,7 +172,7 @@ using namespace g500;
 extern "C" {
 
 void make_mrg_seed(u64 userseed1, u64 userseed2, u32 *seed) {
-    // graph_generation.cu:293-299 (Graph500 user seed mapping).
+    // graph_generation.cu:248-254 (Graph500 user seed mapping).
     seed[0] = (u32)(userseed1 & 0x3FFFFFFFu) + 1;
     seed[1] = (u32)((userseed1 >> 30) & 0x3FFFFFFFu) + 1;
     seed[2] = (u32)(userseed2 & 0x3FFFFFFFu) + 1;
@@ -203,7 +203,7 @@ void generate_kronecker_range(Worker_State *ws, u32 seed[5], u32 logN, i64 start
         return;
     }
     Mrg_State st = {seed[0], seed[1], seed[2], seed[3], seed[4]};
-    // graph_generation.cu:335-345: scramble values from the state skipped by
+    // graph_generation.cu:290-300: scramble values from the state skipped by
     // 50 * 2^128 + 7 * 2^64.
     Mrg_State s2 = st;
     host_skip(s2, 50, 7, 0);

@@ -25,7 +25,7 @@ G500_HD u32 fold(u64 x) {
     return r >= kP ? r - kP : r;
 }
 
-// One MRG step; returns the new z1 (graph_generation.cu:118-130).
+// One MRG step; returns the new z1 (graph_generation.cu:73-85).
 G500_HD u32 mrg_next(Mrg_State &s) {
     u32 nz = fold(kA1 * s.z1 + kA5 * s.z5);
     s.z5 = s.z4;
@@ -37,7 +37,7 @@ G500_HD u32 mrg_next(Mrg_State &s) {
 }
 
 // Applies a compressed transition {s,t,u,v,w,a,b,c,d} (the matrix layout of
-// graph_generation.cu:101-112):
+// graph_generation.cu:56-67):
 //   o1 = d z1 + Y (s z2 + a z3 + b z4 + c z5)
 //   o2 = c z1 + w z2 + Y (s z3 + a z4 + b z5)
 //   o3 = b z1 + v z2 + w z3 + Y (s z4 + a z5)

@@ -57,7 +57,7 @@ void make_random_numbers(i64 nvalues, u64 userseed1, u64 userseed2, i64 position
 
 /* graph_generation.h:42-43: raw little-endian files graph-refactor.bin
  * (Packed_Edge x E) and graph-refactor.bin.weights (f32 x E), written at the
- * byte offset of start_edge (graph_generation.cu:51-52, :359-441). The file
+ * byte offset of start_edge (graph_generation.cu:6-7, :314-396). The file
  * names can be overridden with G500_DUMP_PATH (".weights" is appended). */
 void tuple_graph_dump(const Tuple_Graph *tg, i64 start_edge);
 void tuple_graph_load(Worker_State *ws, Tuple_Graph *tg, i64 start_edge, i64 edge_count);

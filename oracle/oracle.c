@@ -13,6 +13,12 @@
  *   - Generator: every vertex < 2^SCALE; the scramble is a bijection (brute
  *     force, SCALE <= 16); unscrambled endpoints satisfy src <= tgt; level-0
  *     quadrant frequencies 0.57/0.38/0.05; weights are RN_f32(u) * 2^-31.
+ *     The scramble values (val0, val1), the scramble itself and whole edges
+ *     are restated in Python integers (matrix-power skip, string bit
+ *     reversal, products mod 2^64) and compared value by value, with
+ *     swapped, dropped or altered constants shown to change the result.
+ *   - Roots: the full 64-root loop restated in Python over the stepped MRG
+ *     stream (no skip matrices), with degrees from numpy.
  *   - CSR: rowstarts = degree prefix sum; per-row multisets equal the tuple
  *     list's (numpy), no self-loops, symmetric.
  *   - BFS: depths equal scipy.sparse.csgraph unweighted shortest paths.
@@ -43,7 +49,7 @@
 /* MRG                                                                      */
 /* ------------------------------------------------------------------------ */
 
-/* graph_generation.cu:118-130 (orig_step / get_uint_orig). */
+/* graph_generation.cu:73-85 (cuda_mrg_orig_step / cuda_mrg_get_uint_orig). */
 uint32_t o_mrg_next(o_mrg_state *st) {
     uint64_t nz = (MRG_A1 * st->z[0] + MRG_A5 * st->z[4]) % P31;
     st->z[4] = st->z[3];
@@ -109,7 +115,7 @@ void o_mrg_power(int b, int v, uint32_t out[25]) {
     memcpy(out, mrg_pow(b, v)->a, 25 * sizeof(uint32_t));
 }
 
-/* graph_generation.cu:254-268: one matrix per nonzero byte of the exponent. */
+/* graph_generation.cu:209-223: one matrix per nonzero byte of the exponent. */
 void o_mrg_skip(o_mrg_state *st, uint64_t hi, uint64_t mid, uint64_t lo) {
     uint64_t parts[3] = {lo, mid, hi};
     for (int p = 0; p < 3; ++p)
@@ -120,7 +126,7 @@ void o_mrg_skip(o_mrg_state *st, uint64_t hi, uint64_t mid, uint64_t lo) {
 }
 
 /* The compressed 9-word form {s,t,u,v,w,a,b,c,d} of a power of M (the layout
- * of mrg_transitions.cu:14 consumed by graph_generation.cu:101-112): row 5 is
+ * of mrg_transitions.cu:14 consumed by graph_generation.cu:56-67): row 5 is
  * (s,t,u,v,w) and the first column is (d,c,b,a,s). */
 void o_mrg_skip_table(uint32_t *out) {
     for (int b = 0; b < 24; ++b)
@@ -133,7 +139,7 @@ void o_mrg_skip_table(uint32_t *out) {
         }
 }
 
-/* graph_generation.cu:293-299 (Graph500 user seed -> MRG seed). */
+/* graph_generation.cu:248-254 (Graph500 user seed -> MRG seed). */
 void o_make_mrg_seed(uint64_t s1, uint64_t s2, uint32_t seed[5]) {
     seed[0] = (uint32_t)(s1 & 0x3FFFFFFFu) + 1;
     seed[1] = (uint32_t)((s1 >> 30) & 0x3FFFFFFFu) + 1;
@@ -142,10 +148,10 @@ void o_make_mrg_seed(uint64_t s1, uint64_t s2, uint32_t seed[5]) {
     seed[4] = (uint32_t)((s2 >> 60) << 4) + (uint32_t)(s1 >> 60) + 1;
 }
 
-/* 1/(2^31-1) rounded to double: the literal of graph_generation.cu:138. */
+/* 1/(2^31-1) rounded to double: the literal of graph_generation.cu:93. */
 static const double INV_P_F64 = 0x1.00000002p-31;
 
-/* graph_generation.cu:301-311. */
+/* graph_generation.cu:256-266. */
 void o_make_random_numbers(int64_t nvalues, uint64_t s1, uint64_t s2, int64_t pos, double *out) {
     uint32_t seed[5];
     o_make_mrg_seed(s1, s2, seed);
@@ -155,7 +161,7 @@ void o_make_random_numbers(int64_t nvalues, uint64_t s1, uint64_t s2, int64_t po
     for (int64_t i = 0; i < nvalues; ++i) out[i] = (double)o_mrg_next(&st) * INV_P_F64;
 }
 
-/* graph_generation.cu:133-135: RN_f32(u) times 2^-31 (the f32 literal rounds
+/* graph_generation.cu:88-90: RN_f32(u) times 2^-31 (the f32 literal rounds
  * to exactly 2^-31). */
 float o_mrg_weight(o_mrg_state *st) {
     uint32_t u = o_mrg_next(st);
@@ -166,7 +172,7 @@ float o_mrg_weight(o_mrg_state *st) {
 /* Kronecker generator                                                      */
 /* ------------------------------------------------------------------------ */
 
-/* graph_generation.cu:335-345. */
+/* graph_generation.cu:290-300 (val0, val1 of generate_kronecker_range). */
 void o_scramble_values(const uint32_t seed[5], uint64_t *val0, uint64_t *val1) {
     o_mrg_state st;
     memcpy(st.z, seed, 5 * sizeof(uint32_t));
@@ -185,7 +191,7 @@ static uint64_t brev64(uint64_t x) {
     return r;
 }
 
-/* graph_generation.cu:216-224. */
+/* graph_generation.cu:171-179 (cuda_scramble; bit reversal of :145-169). */
 uint64_t o_scramble(uint64_t v, int lgN, uint64_t val0, uint64_t val1) {
     v += val0 + val1;
     v *= (val0 | UINT64_C(0x4519840211493211));
@@ -195,7 +201,7 @@ uint64_t o_scramble(uint64_t v, int lgN, uint64_t val0, uint64_t val1) {
     return v;
 }
 
-/* graph_generation.cu:159-188 with SPK_NOISE_LEVEL 0: reject u below
+/* graph_generation.cu:100-143 with SPK_NOISE_LEVEL 0: reject u below
  * 0x7FFFFFFF mod 10000 = 3647, then u mod 10000 in [0,1900) -> 1 (B),
  * [1900,3800) -> 2 (C), [3800,9500) -> 0 (A), else 3 (D). */
 static int quadrant(o_mrg_state *st) {
@@ -208,7 +214,7 @@ static int quadrant(o_mrg_state *st) {
     return 3;
 }
 
-/* graph_generation.cu:231-252 plus the weight draw of :281. */
+/* graph_generation.cu:186-207 plus the weight draw of :236. */
 static void one_edge(o_mrg_state *st, int lgN, uint64_t val0, uint64_t val1, uint64_t *e, float *w) {
     uint64_t half = (uint64_t)1 << lgN, src = 0, tgt = 0;
     while (half > 1) {
@@ -224,7 +230,7 @@ static void one_edge(o_mrg_state *st, int lgN, uint64_t val0, uint64_t val1, uin
     *w = o_mrg_weight(st);
 }
 
-/* graph_generation.cu:270-283, :314-355. The state of edge ei is the seed
+/* graph_generation.cu:225-238, :269-310. The state of edge ei is the seed
  * skipped by ei * 2^64; consecutive edges share the skip over the upper bytes
  * of ei (matrix products of powers of M commute), so the state for the
  * 256-aligned block is computed once and the low byte applied per edge. */

@@ -22,10 +22,10 @@ extern "C" {
 typedef struct { uint32_t z[5]; } o_mrg_state;
 
 /* One step of x_n = (107374182 x_{n-1} + 104480 x_{n-5}) mod (2^31-1);
- * returns the new z1 (graph_generation.cu:118-130). */
+ * returns the new z1 (graph_generation.cu:73-85). */
 uint32_t o_mrg_next(o_mrg_state *st);
 /* Full 5x5 transition matrix M^e applied to the state, e = hi*2^128 + mid*2^64
- * + lo (graph_generation.cu:254-268; table-free: byte powers of M are built
+ * + lo (graph_generation.cu:209-223; table-free: byte powers of M are built
  * from M by repeated multiplication). */
 void o_mrg_skip(o_mrg_state *st, uint64_t hi, uint64_t mid, uint64_t lo);
 /* The 24x256 table of compressed 9-word transitions {s,t,u,v,w,a,b,c,d} for
@@ -35,9 +35,9 @@ void o_mrg_skip_table(uint32_t *out /* 24*256*9 */);
 void o_mrg_power(int b, int v, uint32_t out[25]);
 void o_make_mrg_seed(uint64_t s1, uint64_t s2, uint32_t seed[5]);
 void o_make_random_numbers(int64_t nvalues, uint64_t s1, uint64_t s2, int64_t pos, double *out);
-float o_mrg_weight(o_mrg_state *st);   /* get_float_orig, :133-135 */
+float o_mrg_weight(o_mrg_state *st);   /* get_float_orig, :88-90 */
 
-/* ---- Kronecker generator (graph_generation.cu:159-355) ----------------- */
+/* ---- Kronecker generator (graph_generation.cu:100-310) ----------------- */
 void o_scramble_values(const uint32_t seed[5], uint64_t *val0, uint64_t *val1);
 uint64_t o_scramble(uint64_t v, int lgN, uint64_t val0, uint64_t val1);
 /* edges: 2 u64 per edge (v0, v1); weights: f32 per edge. Edge indices

@@ -58,11 +58,13 @@ def test_generator_range_split_is_consistent(oracle):
 
 
 def test_single_edge_second_restatement(oracle):
-    """A Python restatement of one edge (matrix-power skip, big-integer MRG)."""
+    """A Python restatement of one edge (matrix-power skip, big-integer MRG,
+    Python scramble values and scramble)."""
     scale = 10
     seed = [3, 1, 4, 1, 1]
     from test_oracle_mrg import apply, mat_pow, py_steps
-    v0, v1 = oracle.scramble_values(seed)
+    from test_oracle_pins import py_scramble, py_scramble_values
+    v0, v1 = py_scramble_values(seed)
     e, w = oracle.kronecker(scale, 16, start=0, end=300)
     for ei in (0, 1, 255, 256, 299):
         z = apply(mat_pow(ei * 2**64), seed)
@@ -80,8 +82,8 @@ def test_single_edge_second_restatement(oracle):
             half //= 2
             s += half * so
             t += half * to
-        assert oracle.lib().o_scramble(s, scale, v0, v1) == int(e[ei, 0])
-        assert oracle.lib().o_scramble(t, scale, v0, v1) == int(e[ei, 1])
+        assert py_scramble(s, scale, v0, v1) == int(e[ei, 0])
+        assert py_scramble(t, scale, v0, v1) == int(e[ei, 1])
         z, (u,) = py_steps(z, 1)
         assert np.float32(w[ei]) == np.float32(np.float32(u) * np.float32(2.0**-31))
 

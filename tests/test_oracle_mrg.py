@@ -106,7 +106,7 @@ def test_table_composition(oracle):
 
 def test_nine_word_encoding_reconstructs_full_matrix(oracle):
     """The table's 9 words {s,t,u,v,w,a,b,c,d} (the layout that
-    graph_generation.cu:101-112 applies) rebuild the full power of M."""
+    graph_generation.cu:56-67 applies) rebuild the full power of M."""
     t = oracle.skip_table()
     for b, v in [(0, 1), (0, 77), (8, 200), (11, 3), (16, 250), (23, 255)]:
         s, tt, u, vv, w, a, bb, c, d = (int(x) for x in t[b, v])
