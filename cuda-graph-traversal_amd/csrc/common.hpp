@@ -30,6 +30,7 @@ struct Context {
     char msg[1024] = {0};
     double last_ms = 0.0;
     const Mrg_Transition_Matrix *d_mrg_table = nullptr;  // [24][256] on the device
+    u64 *d_counters = nullptr;    // 64 bytes of device counters for the validation entry points
     // tuning knobs
     double bfs_alpha = 15.0;      // top-down -> bottom-up when m_frontier > m_unexplored / alpha
     double bfs_beta = 18.0;       // bottom-up -> top-down when n_frontier < n / beta
@@ -47,10 +48,23 @@ struct Context {
 
 Context &ctx();
 
+// Error state of a library-internal thread (the serving slots and copier):
+// while a thread has a sink, its failures are recorded there instead of in
+// the context, so they never leak into the caller's status and never abort.
+struct ErrorSink {
+    int status = 0;
+    char msg[512] = {0};
+};
+extern thread_local ErrorSink *t_error_sink;
+
 // Records a failure. In abort mode prints and exits; in return mode keeps the
-// message and returns so the caller can unwind.
+// message and returns so the caller can unwind. Thread-safe.
 void fail(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
-inline bool failed() { return ctx().status != 0; }
+inline bool failed() { return t_error_sink ? t_error_sink->status != 0 : ctx().status != 0; }
+
+// Frees the library's own device resources (stream, events, skip table,
+// counters); worker_init creates them again.
+void release_library_resources();
 
 #define G500_HIP(call)                                                                   \
     do {                                                                                 \
@@ -119,9 +133,13 @@ int bfs_batch_grid();
 bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roots, i32 nroots, hipStream_t stream,
                         int grid);
 u32 sssp_batch_error(const Sssp_Batch_State *s, hipStream_t stream);  // synchronizes the stream
+// Computes s's light/mid/long/far row splits and row records for its delta and
+// class weights on g (only the split arrays and row_info of s are used),
+// unless they already hold them; returns the clamped class weights.
+void sssp_batch_prepare(const Oned_Graph *g, Sssp_Batch_State *s, hipStream_t st, f32 *wmid, f32 *wlong, f32 *wfar);
 int sssp_batch_grid();
 // Per-root summaries of the last batch: out[r] = reached vertices, out[R + r] =
-// depth sum (BFS, R = 64) or distance sum in 2^-24 fixed point (SSSP, R = 16).
+// depth sum (BFS, R = 64) or distance sum in 2^-24 fixed point (SSSP, R = 32).
 void bfs_batch_summary_enqueue(const Bfs_Batch_State *b, u64 *out, hipStream_t stream);
 void sssp_batch_summary_enqueue(const Oned_Graph *g, const Sssp_Batch_State *s, u64 *out, hipStream_t stream);
 

@@ -1,6 +1,7 @@
 // Library context, errors, arenas, single-rank worker and control entry points.
 // Reference: src/base.cu:1-78, src/worker.cu:6-109.
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "common.hpp"
@@ -12,10 +13,24 @@ Context &ctx() {
     return c;
 }
 
+thread_local ErrorSink *t_error_sink = nullptr;
+
+static std::mutex &error_mutex() {
+    static std::mutex m;
+    return m;
+}
+
 void fail(const char *fmt, ...) {
-    Context &c = ctx();
     va_list ap;
     va_start(ap, fmt);
+    if (ErrorSink *sink = t_error_sink) {
+        if (!sink->status) vsnprintf(sink->msg, sizeof(sink->msg), fmt, ap);
+        va_end(ap);
+        sink->status = 1;
+        return;
+    }
+    Context &c = ctx();
+    std::lock_guard<std::mutex> lk(error_mutex());
     vsnprintf(c.msg, sizeof(c.msg), fmt, ap);
     va_end(ap);
     if (c.error_mode == G500_ERRORS_ABORT) {
@@ -23,6 +38,20 @@ void fail(const char *fmt, ...) {
         ABORT();
     }
     if (!c.status) c.status = 1;
+}
+
+void release_library_resources() {
+    Context &c = ctx();
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    if (c.d_counters) (void)hipFree(c.d_counters);
+    if (c.d_mrg_table) (void)hipFree((void *)c.d_mrg_table);
+    if (c.ev_start) (void)hipEventDestroy(c.ev_start);
+    if (c.ev_stop) (void)hipEventDestroy(c.ev_stop);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    c.d_counters = nullptr;
+    c.d_mrg_table = nullptr;
+    c.ev_start = c.ev_stop = nullptr;
+    c.stream = nullptr;
 }
 
 void *arena_alloc(Worker_State *ws, u64 bytes, const char *what) {
@@ -205,9 +234,12 @@ Worker_State worker_init(i32 argc, char **argv, u64 gpu_memory_size) {
 }
 
 void worker_finalize(Worker_State *ws) {
+    // worker.cu:57-62 releases the arenas; the library's stream, events, skip
+    // table and counters go too, so nothing is left for exit-time teardown
     if (ctx().stream) (void)hipStreamSynchronize(ctx().stream);
     arena_release_gpu(&ws->gpu_arena);
     arena_release_cpu(&ws->cpu_arena);
+    release_library_resources();
 }
 
 void worker_abort(Worker_State *ws, i32 return_code) {
@@ -226,6 +258,7 @@ void g500_set_error_mode(i32 mode) { ctx().error_mode = mode; }
 i32 g500_status(void) { return ctx().status; }
 const char *g500_last_error(void) { return ctx().msg; }
 void g500_clear_error(void) {
+    std::lock_guard<std::mutex> lk(error_mutex());
     ctx().status = 0;
     ctx().msg[0] = 0;
 }

@@ -16,9 +16,25 @@
 // computes per-request summaries, and hands the requests to a copy thread,
 // which copies the results to the callers' host buffers, frees the pool
 // memory and completes the requests.
+//
+// Co-residency: a slot's persistent kernels synchronize their whole grid, so
+// every block of a launch must be resident at once. With 2 or 4 slots each
+// slot's stream carries a CU mask that gives it its own 1/slots of the CUs of
+// every XCD (mask bits i with (i / 8) % slots == k), and its launches are
+// sized to fill exactly those CUs; kernels of different slots then never
+// compete for a CU. Other slot counts share all CUs with launches of 1/slots
+// of the co-resident grid (residency then relies on the dispatcher spreading
+// blocks evenly; a grid that is not resident ends with a barrier timeout,
+// reported as the batch's status).
+//
+// Row splits and row records of the batched SSSP depend on the graph and on
+// the bucket width; the server computes them once per graph for both widths
+// at start, and every slot reads them from there.
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <condition_variable>
+#include <cstring>
 #include <deque>
 #include <map>
 #include <mutex>
@@ -120,13 +136,17 @@ struct G500_Server {
         Bfs_Batch_State bfs;
         Sssp_Batch_State sssp;
         u64 *summary = nullptr;  // device [128]
+        int grid_bfs = 0, grid_sssp = 0;
+        int cus = 0;             // CUs of the slot's mask (0: no mask)
         std::thread th;
     };
     std::vector<Slot> slots;
     std::thread copier;
     hipStream_t copy_stream = nullptr;
-    int grid_bfs = 0, grid_sssp = 0;
     f32 delta = 0.1f;
+    // per graph: SSSP row splits and row records for batches of up to 16 roots
+    // (delta) and of more (delta / 2)
+    std::vector<std::array<Sssp_Batch_State, 2>> splits;
     u64 launches = 0, launched_requests = 0;
 };
 
@@ -169,7 +189,31 @@ std::vector<Request *> take_batch(G500_Server *s) {
     return batch;
 }
 
+// Status of a request whose batch failed on the host side (launch refused,
+// copy failed); kernel errors are reported with their own bits (1..63).
+constexpr i32 kServeHostError = 1 << 16;
+
+// Points a slot's SSSP state at the server's splits of one graph and width.
+void use_splits(Sssp_Batch_State *st, const Sssp_Batch_State &c) {
+    st->delta = c.delta;
+    st->mid_weight = c.mid_weight;
+    st->long_weight = c.long_weight;
+    st->far_weight = c.far_weight;
+    st->split = c.split;
+    st->mid_split = c.mid_split;
+    st->long_split = c.long_split;
+    st->far_split = c.far_split;
+    st->row_info = c.row_info;
+    st->split_delta = c.split_delta;
+    st->mid_split_weight = c.mid_split_weight;
+    st->long_split_weight = c.long_split_weight;
+    st->far_split_weight = c.far_split_weight;
+    st->split_graph = c.split_graph;
+}
+
 void slot_loop(G500_Server *s, int k) {
+    ErrorSink sink;
+    t_error_sink = &sink;
     G500_HIP(hipSetDevice(ctx().device));
     G500_Server::Slot &slot = s->slots[k];
     std::vector<u64> roots;
@@ -193,33 +237,51 @@ void slot_loop(G500_Server *s, int k) {
         const bool bfs = batch[0]->kind == G500_BFS;
         roots.clear();
         for (Request *r : batch) roots.push_back(r->root);
-        u32 err;
+        sink.status = 0;
+        sink.msg[0] = 0;
+        u32 err = 0;
+        bool ok;
         if (bfs) {
-            bfs_batch_enqueue(g, &slot.bfs, roots.data(), (i32)roots.size(), slot.stream, s->grid_bfs);
-            err = bfs_batch_error(&slot.bfs, slot.stream);
-            for (size_t i = 0; i < batch.size(); ++i)
-                if (batch[i]->bytes0)
-                    bfs_batch_depth_enqueue(g, &slot.bfs, (i32)i, (i64 *)(s->pool_base + batch[i]->off0), slot.stream);
-            bfs_batch_summary_enqueue(&slot.bfs, slot.summary, slot.stream);
+            ok = bfs_batch_enqueue(g, &slot.bfs, roots.data(), (i32)roots.size(), slot.stream, slot.grid_bfs);
+            if (ok) err = bfs_batch_error(&slot.bfs, slot.stream);
+            if (ok && !err) {
+                for (size_t i = 0; i < batch.size(); ++i)
+                    if (batch[i]->bytes0)
+                        bfs_batch_depth_enqueue(g, &slot.bfs, (i32)i, (i64 *)(s->pool_base + batch[i]->off0),
+                                                slot.stream);
+                bfs_batch_summary_enqueue(&slot.bfs, slot.summary, slot.stream);
+            }
         } else {
             // block shape and grid share were fixed at start; batches of more than
             // 16 roots share more work per bucket and run at half the server's delta
-            slot.sssp.delta = roots.size() > 16 ? 0.5f * s->delta : s->delta;
-            sssp_batch_enqueue(g, &slot.sssp, roots.data(), (i32)roots.size(), slot.stream, s->grid_sssp);
-            err = sssp_batch_error(&slot.sssp, slot.stream);
-            for (size_t i = 0; i < batch.size(); ++i) {
-                Request *r = batch[i];
-                if (r->bytes0)
-                    G500_HIP(hipMemcpyAsync(s->pool_base + r->off0, slot.sssp.dist + i * n, n * sizeof(f32),
-                                            hipMemcpyDeviceToDevice, slot.stream));
-                if (r->bytes1)
-                    G500_HIP(hipMemcpyAsync(s->pool_base + r->off1, slot.sssp.pred + i * n, n * sizeof(i64),
-                                            hipMemcpyDeviceToDevice, slot.stream));
+            use_splits(&slot.sssp, s->splits[batch[0]->graph][roots.size() > 16 ? 1 : 0]);
+            ok = sssp_batch_enqueue(g, &slot.sssp, roots.data(), (i32)roots.size(), slot.stream, slot.grid_sssp);
+            if (ok) err = sssp_batch_error(&slot.sssp, slot.stream);
+            if (ok && !err) {
+                for (size_t i = 0; i < batch.size(); ++i) {
+                    Request *r = batch[i];
+                    if (r->bytes0)
+                        G500_HIP(hipMemcpyAsync(s->pool_base + r->off0, slot.sssp.dist + i * n, n * sizeof(f32),
+                                                hipMemcpyDeviceToDevice, slot.stream));
+                    if (r->bytes1)
+                        G500_HIP(hipMemcpyAsync(s->pool_base + r->off1, slot.sssp.pred + i * n, n * sizeof(i64),
+                                                hipMemcpyDeviceToDevice, slot.stream));
+                }
+                sssp_batch_summary_enqueue(g, &slot.sssp, slot.summary, slot.stream);
             }
-            sssp_batch_summary_enqueue(g, &slot.sssp, slot.summary, slot.stream);
         }
-        G500_HIP(hipMemcpyAsync(summary, slot.summary, sizeof(summary), hipMemcpyDeviceToHost, slot.stream));
-        G500_HIP(hipStreamSynchronize(slot.stream));
+        if (ok && !err) {
+            G500_HIP(hipMemcpyAsync(summary, slot.summary, sizeof(summary), hipMemcpyDeviceToHost, slot.stream));
+            G500_HIP(hipStreamSynchronize(slot.stream));
+        }
+        // a failed launch or copy leaves nothing valid to stage: the requests
+        // complete with an error status and no results
+        const i32 status = failed() ? kServeHostError : (i32)err;
+        if (status) {
+            fprintf(stderr, "libg500 server: slot %d batch of %zu %s requests on graph %d failed (status %d%s%s)\n", k,
+                    batch.size(), bfs ? "BFS" : "SSSP", batch[0]->graph, status, sink.status ? ": " : "", sink.msg);
+            (void)hipStreamSynchronize(slot.stream);
+        }
         const double t_staged = now_ms();
         const int R = bfs ? 64 : kSsspBatchRoots;
         {
@@ -229,9 +291,15 @@ void slot_loop(G500_Server *s, int k) {
                 r->t_launch = t_launch;
                 r->t_staged = t_staged;
                 r->batch = (i32)batch.size();
-                r->status = (i32)err;
-                r->reached = summary[i];
-                r->sum = bfs ? (f64)summary[R + i] : (f64)summary[R + i] / 16777216.0;
+                r->status = status;
+                r->reached = status ? 0 : summary[i];
+                r->sum = status ? 0.0 : bfs ? (f64)summary[R + i] : (f64)summary[R + i] / 16777216.0;
+                if (status) {  // nothing to copy out
+                    s->pool.release(r->off0, r->bytes0);
+                    s->pool.release(r->off1, r->bytes1);
+                    r->bytes0 = r->bytes1 = 0;
+                    r->off0 = r->off1 = ~0ull;
+                }
                 s->copying.push_back(r);
             }
         }
@@ -240,6 +308,8 @@ void slot_loop(G500_Server *s, int k) {
 }
 
 void copy_loop(G500_Server *s) {
+    ErrorSink sink;
+    t_error_sink = &sink;
     G500_HIP(hipSetDevice(ctx().device));
     for (;;) {
         Request *r;
@@ -250,11 +320,17 @@ void copy_loop(G500_Server *s) {
             r = s->copying.front();
             s->copying.pop_front();
         }
+        sink.status = 0;
         if (r->bytes0)
             G500_HIP(hipMemcpyAsync(r->out0, s->pool_base + r->off0, r->bytes0, hipMemcpyDeviceToHost, s->copy_stream));
         if (r->bytes1)
             G500_HIP(hipMemcpyAsync(r->out1, s->pool_base + r->off1, r->bytes1, hipMemcpyDeviceToHost, s->copy_stream));
         G500_HIP(hipStreamSynchronize(s->copy_stream));
+        if (sink.status) {
+            fprintf(stderr, "libg500 server: result copy of request %llu failed: %s\n", (unsigned long long)r->id,
+                    sink.msg);
+            if (!r->status) r->status = kServeHostError;
+        }
         const double t_done = now_ms();
         {
             std::lock_guard<std::mutex> lk(s->mu);
@@ -325,18 +401,59 @@ i32 g500_server_start(G500_Server *s, f32 delta) {
     const Oned_Graph *big = &s->graphs[0];
     for (const Oned_Graph &g : s->graphs)
         if (g.nlocalverts > big->nlocalverts) big = &g;
-    // each slot's persistent kernels get an equal share of the co-resident grid,
-    // so the launches of all slots are resident together
-    s->grid_bfs = std::max(1, bfs_batch_grid() / s->nslots);
-    s->grid_sssp = std::max(1, sssp_batch_grid() / s->nslots);
+    const int cus = ctx().num_cus;
+    const int bfs_per_cu = std::max(1, bfs_batch_grid() / cus), sssp_per_cu = std::max(1, sssp_batch_grid() / cus);
+    const bool masked = s->nslots == 2 || s->nslots == 4;
     s->slots.resize(s->nslots);
-    for (auto &slot : s->slots) {
-        G500_HIP(hipStreamCreateWithFlags(&slot.stream, hipStreamNonBlocking));
+    for (int k = 0; k < s->nslots; ++k) {
+        G500_Server::Slot &slot = s->slots[k];
+        if (masked) {
+            // the slot's own CUs: mask bits i with (i / 8) % slots == k, an equal
+            // share of every XCD (the runtime spreads mask bits round-robin over
+            // the XCDs); its launches fill exactly these CUs
+            std::vector<u32> mask((cus + 31) / 32, 0u);
+            for (int i = 0; i < cus; ++i)
+                if ((i / 8) % s->nslots == k) {
+                    mask[i / 32] |= 1u << (i % 32);
+                    ++slot.cus;
+                }
+            G500_HIP(hipExtStreamCreateWithCUMask(&slot.stream, (u32)mask.size(), mask.data()));
+            slot.grid_bfs = bfs_per_cu * slot.cus;
+            slot.grid_sssp = sssp_per_cu * slot.cus;
+        } else {
+            // an equal share of the co-resident grid for each slot
+            G500_HIP(hipStreamCreateWithFlags(&slot.stream, hipStreamNonBlocking));
+            slot.grid_bfs = std::max(1, bfs_per_cu * cus / s->nslots);
+            slot.grid_sssp = std::max(1, sssp_per_cu * cus / s->nslots);
+        }
         slot.bfs = oned_graph_bfs_batch_create(s->ws, big);
         slot.sssp = oned_graph_sssp_batch_create(s->ws, big);
         slot.summary = (u64 *)arena_alloc(s->ws, 128 * sizeof(u64), "server summaries");
     }
     G500_HIP(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
+    if (failed()) return -1;
+    // SSSP row splits and row records of every graph for both bucket widths
+    const Sssp_Batch_State &proto = s->slots[0].sssp;
+    s->splits.resize(s->graphs.size());
+    for (size_t gi = 0; gi < s->graphs.size(); ++gi)
+        for (int w = 0; w < 2; ++w) {
+            const u64 n = s->graphs[gi].nlocalverts;
+            Sssp_Batch_State &c = s->splits[gi][w];
+            memset(&c, 0, sizeof(c));
+            c.max_nlocalverts = n;
+            c.delta = w ? 0.5f * delta : delta;
+            c.mid_weight = proto.mid_weight;
+            c.long_weight = proto.long_weight;
+            c.far_weight = proto.far_weight;
+            c.split = (u32 *)arena_alloc(s->ws, n * sizeof(u32), "server SSSP light splits");
+            c.mid_split = (u32 *)arena_alloc(s->ws, n * sizeof(u32), "server SSSP mid splits");
+            c.long_split = (u32 *)arena_alloc(s->ws, n * sizeof(u32), "server SSSP long splits");
+            c.far_split = (u32 *)arena_alloc(s->ws, n * sizeof(u32), "server SSSP far splits");
+            c.row_info = arena_alloc(s->ws, n * 32, "server SSSP row records");
+            if (failed()) return -1;
+            sssp_batch_prepare(&s->graphs[gi], &c, ctx().stream, nullptr, nullptr, nullptr);
+        }
+    sync();
     if (failed()) return -1;
     s->started = true;
     for (int k = 0; k < s->nslots; ++k) s->slots[k].th = std::thread(slot_loop, s, k);

@@ -1197,6 +1197,38 @@ void sssp_batch_summary_enqueue(const Oned_Graph *g, const Sssp_Batch_State *s, 
     G500_HIP(hipGetLastError());
 }
 
+void sssp_batch_prepare(const Oned_Graph *g, Sssp_Batch_State *s, hipStream_t st, f32 *wmid_out, f32 *wlong_out,
+                        f32 *wfar_out) {
+    const Context &c = ctx();
+    // class floors: delta <= W_0 (mid) <= W_1 (long) <= W_2 (far); a class whose
+    // floor is clamped onto the next one is empty
+    auto clampw = [](f32 w, f32 lo, f32 hi) { return w < lo ? lo : (w > hi ? hi : w); };
+    const f32 wfar = s->far_weight > s->delta ? s->far_weight : s->delta;
+    const f32 wlong = clampw(s->long_weight, s->delta, wfar);
+    const f32 wmid = clampw(s->mid_weight, s->delta, wlong);
+    if (wmid_out) *wmid_out = wmid;
+    if (wlong_out) *wlong_out = wlong;
+    if (wfar_out) *wfar_out = wfar;
+    const u64 n = g->nlocalverts;
+    // the split depends on the graph too: a state reused across graphs recomputes it
+    const TraversalView tv = sssp_view(g);
+    if (s->split_delta != s->delta || s->long_split_weight != wlong || s->mid_split_weight != wmid ||
+        s->far_split_weight != wfar || s->split_graph != (const void *)tv.rs) {
+        compute_split(tv.rs, tv.w, n, s->delta, s->split, st);
+        compute_split(tv.rs, tv.w, n, wmid, s->mid_split, st);
+        compute_split(tv.rs, tv.w, n, wlong, s->long_split, st);
+        compute_split(tv.rs, tv.w, n, wfar, s->far_split, st);
+        sssp_batch_rowinfo_kernel<<<grid_for(n, 256, (u64)c.num_cus * 32), 256, 0, st>>>(
+            tv.rs, s->split, s->mid_split, s->long_split, s->far_split, n, (RowInfo *)s->row_info);
+        G500_HIP(hipGetLastError());
+        s->split_delta = s->delta;
+        s->mid_split_weight = wmid;
+        s->long_split_weight = wlong;
+        s->far_split_weight = wfar;
+        s->split_graph = (const void *)tv.rs;
+    }
+}
+
 bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roots, i32 nroots, hipStream_t stream,
                         int grid) {
     Context &c = ctx();
@@ -1226,12 +1258,6 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
         fail("batched SSSP far_weight must be non-negative and finite (got %g)", (double)s->far_weight);
         return false;
     }
-    // class floors: delta <= W_0 (mid) <= W_1 (long) <= W_2 (far); a class whose
-    // floor is clamped onto the next one is empty
-    auto clampw = [](f32 w, f32 lo, f32 hi) { return w < lo ? lo : (w > hi ? hi : w); };
-    const f32 wfar = s->far_weight > s->delta ? s->far_weight : s->delta;
-    const f32 wlong = clampw(s->long_weight, s->delta, wfar);
-    const f32 wmid = clampw(s->mid_weight, s->delta, wlong);
     const u64 n = g->nlocalverts;
     if (n > s->max_nlocalverts) {
         fail("batched SSSP state holds %llu vertices, graph has %llu", (unsigned long long)s->max_nlocalverts,
@@ -1240,23 +1266,9 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
     }
     const bool library = stream == nullptr;
     const hipStream_t st = library ? c.stream : stream;
-    // the split depends on the graph too: a state reused across graphs recomputes it
+    f32 wmid, wlong, wfar;
+    sssp_batch_prepare(g, s, st, &wmid, &wlong, &wfar);
     const TraversalView tv = sssp_view(g);
-    if (s->split_delta != s->delta || s->long_split_weight != wlong || s->mid_split_weight != wmid ||
-        s->far_split_weight != wfar || s->split_graph != (const void *)tv.rs) {
-        compute_split(tv.rs, tv.w, n, s->delta, s->split, st);
-        compute_split(tv.rs, tv.w, n, wmid, s->mid_split, st);
-        compute_split(tv.rs, tv.w, n, wlong, s->long_split, st);
-        compute_split(tv.rs, tv.w, n, wfar, s->far_split, st);
-        sssp_batch_rowinfo_kernel<<<grid_for(n, 256, (u64)c.num_cus * 32), 256, 0, st>>>(
-            tv.rs, s->split, s->mid_split, s->long_split, s->far_split, n, (RowInfo *)s->row_info);
-        G500_HIP(hipGetLastError());
-        s->split_delta = s->delta;
-        s->mid_split_weight = wmid;
-        s->long_split_weight = wlong;
-        s->far_split_weight = wfar;
-        s->split_graph = (const void *)tv.rs;
-    }
     sssp_batch_clear_kernel<<<grid_for(n * kBatchRoots, 256, (u64)c.num_cus * 64), 256, 0, st>>>(
         s->labels, (u32 *)s->dist, s->pending, s->hub_aux, s->root_masks, s->mid_work, s->long_work, s->far_work,
         s->open_words, n);

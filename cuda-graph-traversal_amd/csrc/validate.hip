@@ -94,9 +94,9 @@ __global__ __launch_bounds__(256) void edge_count_kernel(const u64 *__restrict__
 
 // Small device scratch for counters shared by the validation entry points.
 static u64 *counter_scratch() {
-    static u64 *p = nullptr;
-    if (!p) G500_HIP(hipMalloc(&p, 64));
-    return p;
+    Context &c = ctx();
+    if (!c.d_counters) G500_HIP(hipMalloc(&c.d_counters, 64));
+    return c.d_counters;
 }
 
 }  // namespace g500

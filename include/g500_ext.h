@@ -171,7 +171,8 @@ struct G500_Completion {
     u64 id;            /* from g500_server_submit */
     i32 graph, kind;
     u64 root;
-    i32 status;        /* 0: ok; otherwise the kernel's error bits */
+    i32 status;        /* 0: ok; 1..63: the kernel's error bits; 65536: the launch or a result copy
+                          failed on the host side (no results are copied) */
     i32 batch_size;    /* requests in the launch that served it */
     u64 reached;       /* vertices reached from the root */
     f64 sum;           /* BFS: sum of depths; SSSP: sum of distances (2^-24 fixed point) */

@@ -284,11 +284,12 @@ static float bitsf(uint32_t u) {
 uint64_t o_csr_build(uint64_t n, uint64_t nedges, const uint64_t *edges, const float *weights,
                      uint64_t *rowstarts, uint32_t *column, float *cw) {
     memset(rowstarts, 0, (n + 1) * sizeof(uint64_t));
+#pragma omp parallel for schedule(static)
     for (uint64_t i = 0; i < nedges; ++i) {
         uint64_t a = edges[2 * i], b = edges[2 * i + 1];
         if (a == b) continue;
-        rowstarts[a + 1]++;
-        rowstarts[b + 1]++;
+        __atomic_fetch_add(&rowstarts[a + 1], 1, __ATOMIC_RELAXED);
+        __atomic_fetch_add(&rowstarts[b + 1], 1, __ATOMIC_RELAXED);
     }
     for (uint64_t v = 0; v < n; ++v) rowstarts[v + 1] += rowstarts[v];
     uint64_t m = rowstarts[n];
@@ -390,6 +391,99 @@ uint64_t o_bfs_omp(uint64_t n, const uint64_t *rs, const uint32_t *col, uint64_t
     free(cur);
     free(nxt);
     return reached;
+}
+
+/* Depths of a level-synchronous top-down BFS (OpenMP): the textbook hop
+ * counts, -1 unreached (the depth convention of traversal.cu:505-563).
+ * Returns the reached count. */
+uint64_t o_bfs_depth_omp(uint64_t n, const uint64_t *rs, const uint32_t *col, uint64_t root,
+                         int64_t *depth, int nthreads) {
+    uint32_t *cur = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+    uint32_t *nxt = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+#pragma omp parallel for num_threads(nthreads)
+    for (uint64_t v = 0; v < n; ++v) depth[v] = -1;
+    depth[root] = 0;
+    cur[0] = (uint32_t)root;
+    uint64_t ncur = 1, reached = 1;
+    for (int64_t level = 1; ncur; ++level) {
+        uint64_t nnext = 0;
+#pragma omp parallel num_threads(nthreads)
+        {
+            uint32_t local[256];
+            int nl = 0;
+#pragma omp for schedule(dynamic, 64)
+            for (uint64_t i = 0; i < ncur; ++i) {
+                uint32_t u = cur[i];
+                for (uint64_t k = rs[u]; k < rs[u + 1]; ++k) {
+                    uint32_t v = col[k];
+                    if (__atomic_load_n(&depth[v], __ATOMIC_RELAXED) != -1) continue;
+                    int64_t expect = -1;
+                    if (!__atomic_compare_exchange_n(&depth[v], &expect, level, 0, __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED))
+                        continue;
+                    local[nl++] = v;
+                    if (nl == 256) {
+                        uint64_t at = __atomic_fetch_add(&nnext, 256, __ATOMIC_RELAXED);
+                        memcpy(nxt + at, local, sizeof(local));
+                        nl = 0;
+                    }
+                }
+            }
+            if (nl) {
+                uint64_t at = __atomic_fetch_add(&nnext, (uint64_t)nl, __ATOMIC_RELAXED);
+                memcpy(nxt + at, local, (size_t)nl * sizeof(uint32_t));
+            }
+        }
+        uint32_t *t = cur;
+        cur = nxt;
+        nxt = t;
+        ncur = nnext;
+        reached += nnext;
+    }
+    free(cur);
+    free(nxt);
+    return reached;
+}
+
+/* Order-free digest of a multiset of directed entries (u, v, weight bits):
+ * a sum mod 2^64 of a 64-bit mix of each entry. */
+static uint64_t mix64(uint64_t x) {
+    x ^= x >> 30;
+    x *= UINT64_C(0xbf58476d1ce4e5b9);
+    x ^= x >> 27;
+    x *= UINT64_C(0x94d049bb133111eb);
+    x ^= x >> 31;
+    return x;
+}
+static uint64_t entry_hash(uint64_t u, uint64_t v, uint32_t wb) {
+    return mix64(mix64((u << 32) ^ v) ^ ((uint64_t)wb * UINT64_C(0x9e3779b97f4a7c15)));
+}
+
+uint64_t o_tuple_digest(uint64_t nedges, const uint64_t *edges, const float *weights, uint64_t *entries) {
+    uint64_t sum = 0, cnt = 0;
+#pragma omp parallel for schedule(static) reduction(+ : sum, cnt)
+    for (uint64_t i = 0; i < nedges; ++i) {
+        uint64_t a = edges[2 * i], b = edges[2 * i + 1];
+        if (a == b) continue;
+        uint32_t wb;
+        memcpy(&wb, &weights[i], 4);
+        sum += entry_hash(a, b, wb) + entry_hash(b, a, wb);
+        cnt += 2;
+    }
+    if (entries) *entries = cnt;
+    return sum;
+}
+
+uint64_t o_csr_digest(uint64_t n, const uint64_t *rs, const uint32_t *col, const float *cw) {
+    uint64_t sum = 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : sum)
+    for (uint64_t u = 0; u < n; ++u)
+        for (uint64_t k = rs[u]; k < rs[u + 1]; ++k) {
+            uint32_t wb;
+            memcpy(&wb, &cw[k], 4);
+            sum += entry_hash(u, col[k], wb);
+        }
+    return sum;
 }
 
 /* traversal.cu:75-112, :505-563: level-by-level, a vertex v gets level L+1
@@ -685,10 +779,165 @@ void o_delta_stepping_f32(uint64_t n, const uint64_t *rs, const uint32_t *col, c
     free(nxt);
 }
 
+static int atomic_min_u32(uint32_t *p, uint32_t v) {
+    uint32_t old = __atomic_load_n(p, __ATOMIC_RELAXED);
+    while (v < old)
+        if (__atomic_compare_exchange_n(p, &old, v, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) return 1;
+    return 0;
+}
+
+/* CPU baseline: parallel delta-stepping. Inside a bucket [lo, fl(lo + delta))
+ * light edges (w < delta) are relaxed round by round until no distance in the
+ * bucket drops, then the heavy edges of the bucket's vertices are relaxed once
+ * (they land at or above the bucket's top); the next bucket starts at the
+ * smallest open distance. Distances are u32 bits lowered by an atomic min
+ * (non-negative floats order like their bits). A last pass takes for each
+ * reached vertex a neighbour whose distance plus the entry's weight gives its
+ * distance as the parent, discovering vertices breadth-first from the root so
+ * that no parent chain closes a cycle. The FP32 fixed point is unique, so the distances
+ * equal Dijkstra's (traversal.cu:638-711 schedules the same relaxations). */
+void o_delta_stepping_omp(uint64_t n, const uint64_t *rs, const uint32_t *col, const float *w,
+                          uint64_t root, float delta, float *dist, int64_t *pred, int nthreads) {
+    const uint32_t INF = 0x7f800000u;
+    uint32_t *db = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t)); /* distance bits */
+    uint32_t *cur = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+    uint32_t *nxt = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+    uint32_t *members = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+    uint32_t *mark = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));   /* last light round that queued v */
+    uint32_t *inb = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));    /* last bucket that listed v */
+    uint8_t *settled = (uint8_t *)calloc(n ? n : 1, 1);
+#pragma omp parallel for num_threads(nthreads)
+    for (uint64_t v = 0; v < n; ++v) db[v] = INF;
+    db[root] = 0;
+    float lo = 0.0f;
+    uint32_t round = 0, bucket = 0;
+    for (;;) {
+        const float hi = lo + delta;
+        uint32_t lo_b, hi_b;
+        memcpy(&lo_b, &lo, 4);
+        memcpy(&hi_b, &hi, 4);
+        ++bucket;
+        uint64_t ncur = 0, nmem = 0;
+        uint32_t next_min = INF;
+#pragma omp parallel num_threads(nthreads) reduction(min : next_min)
+        {
+#pragma omp for schedule(static)
+            for (uint64_t v = 0; v < n; ++v) {
+                if (settled[v] || db[v] == INF) continue;
+                if (db[v] >= lo_b && db[v] < hi_b) {
+                    uint64_t at = __atomic_fetch_add(&ncur, 1, __ATOMIC_RELAXED);
+                    cur[at] = (uint32_t)v;
+                } else if (db[v] < next_min) {
+                    next_min = db[v];
+                }
+            }
+        }
+        if (!ncur) {
+            if (next_min == INF) break;
+            memcpy(&lo, &next_min, 4);
+            continue;
+        }
+        while (ncur) {
+            ++round;
+            uint64_t nn = 0;
+#pragma omp parallel num_threads(nthreads)
+            {
+#pragma omp for schedule(dynamic, 64)
+                for (uint64_t i = 0; i < ncur; ++i) {
+                    uint32_t u = cur[i];
+                    if (__atomic_exchange_n(&inb[u], bucket, __ATOMIC_RELAXED) != bucket) {
+                        uint64_t at = __atomic_fetch_add(&nmem, 1, __ATOMIC_RELAXED);
+                        members[at] = u;
+                    }
+                    uint32_t dub = __atomic_load_n(&db[u], __ATOMIC_RELAXED);
+                    float du;
+                    memcpy(&du, &dub, 4);
+                    for (uint64_t k = rs[u]; k < rs[u + 1]; ++k) {
+                        if (!(w[k] < delta)) continue;
+                        volatile float cand = du + w[k];
+                        uint32_t cb;
+                        float c = cand;
+                        memcpy(&cb, &c, 4);
+                        uint32_t v = col[k];
+                        if (!atomic_min_u32(&db[v], cb) || !(cb < hi_b)) continue;
+                        if (__atomic_exchange_n(&mark[v], round, __ATOMIC_RELAXED) == round) continue;
+                        uint64_t at = __atomic_fetch_add(&nn, 1, __ATOMIC_RELAXED);
+                        nxt[at] = v;
+                    }
+                }
+            }
+            uint32_t *t = cur;
+            cur = nxt;
+            nxt = t;
+            ncur = nn;
+        }
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+        for (uint64_t i = 0; i < nmem; ++i) {
+            uint32_t u = members[i];
+            settled[u] = 1;
+            float du;
+            memcpy(&du, &db[u], 4);
+            for (uint64_t k = rs[u]; k < rs[u + 1]; ++k) {
+                if (w[k] < delta) continue;
+                volatile float cand = du + w[k];
+                uint32_t cb;
+                float c = cand;
+                memcpy(&cb, &c, 4);
+                atomic_min_u32(&db[col[k]], cb);
+            }
+        }
+        lo = hi;
+    }
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (uint64_t v = 0; v < n; ++v) {
+        float d = -1.0f;
+        if (db[v] != INF) memcpy(&d, &db[v], 4);
+        dist[v] = d;
+    }
+    /* parents: a BFS from the root over tight entries (fl(d[u] + w) == d[v]);
+     * equal-distance neighbours joined by tiny weights are tight both ways,
+     * so a parent is taken on first discovery only, which keeps chains acyclic */
+#pragma omp parallel for num_threads(nthreads)
+    for (uint64_t v = 0; v < n; ++v) pred[v] = -1;
+    pred[root] = (int64_t)root;
+    cur[0] = (uint32_t)root;
+    uint64_t ncur = 1;
+    while (ncur) {
+        uint64_t nn = 0;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+        for (uint64_t i = 0; i < ncur; ++i) {
+            uint32_t u = cur[i];
+            for (uint64_t k = rs[u]; k < rs[u + 1]; ++k) {
+                uint32_t v = col[k];
+                volatile float cand = dist[u] + w[k];
+                if (cand != dist[v] || __atomic_load_n(&pred[v], __ATOMIC_RELAXED) != -1) continue;
+                int64_t expect = -1;
+                if (!__atomic_compare_exchange_n(&pred[v], &expect, (int64_t)u, 0, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED))
+                    continue;
+                uint64_t at = __atomic_fetch_add(&nn, 1, __ATOMIC_RELAXED);
+                nxt[at] = v;
+            }
+        }
+        uint32_t *t = cur;
+        cur = nxt;
+        nxt = t;
+        ncur = nn;
+    }
+    free(db);
+    free(settled);
+    free(inb);
+    free(mark);
+    free(members);
+    free(nxt);
+    free(cur);
+}
+
 uint64_t o_sssp_check(uint64_t n, const uint64_t *rs, const uint32_t *col, const float *w,
                       uint64_t root, const float *dist, const int64_t *pred) {
     uint64_t err = 0;
     if (dist[root] != 0.0f || pred[root] != (int64_t)root) ++err;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : err)
     for (uint64_t u = 0; u < n; ++u) {
         int reached_u = dist[u] >= 0.0f;
         if (!reached_u) {

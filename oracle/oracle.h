@@ -60,6 +60,15 @@ uint64_t o_bfs_serial(uint64_t n, const uint64_t *rs, const uint32_t *col, uint6
 /* Level-synchronous top-down OpenMP BFS (CPU baseline). Returns reached. */
 uint64_t o_bfs_omp(uint64_t n, const uint64_t *rs, const uint32_t *col, uint64_t root,
                    int64_t *pred, int nthreads);
+/* Depths (hop counts, -1 unreached) of a level-synchronous OpenMP BFS.
+ * Returns the reached count. */
+uint64_t o_bfs_depth_omp(uint64_t n, const uint64_t *rs, const uint32_t *col, uint64_t root,
+                         int64_t *depth, int nthreads);
+/* Order-free digests (sum mod 2^64 of a 64-bit mix per directed entry
+ * (u, v, weight bits)) of the entries a tuple list stands for (both
+ * directions of each non-loop edge; *entries gets their count) and of a CSR. */
+uint64_t o_tuple_digest(uint64_t nedges, const uint64_t *edges, const float *weights, uint64_t *entries);
+uint64_t o_csr_digest(uint64_t n, const uint64_t *rs, const uint32_t *col, const float *cw);
 /* bfs_compute_dist_from_pred semantics (traversal.cu:75-112, :505-563):
  * -1 unreached, INT64_MAX reached-but-unresolved, root 0. */
 void o_dist_from_pred(uint64_t n, const uint64_t *rs, const uint32_t *col, const int64_t *pred,
@@ -90,6 +99,11 @@ void o_dijkstra_f32_timed(uint64_t n, const uint64_t *rs, const uint32_t *col, c
  * light/heavy split and bucket bounds as traversal.cu:638-711). */
 void o_delta_stepping_f32(uint64_t n, const uint64_t *rs, const uint32_t *col, const float *w,
                           uint64_t root, float delta, float *dist, int64_t *pred);
+/* CPU baseline: OpenMP delta-stepping (bucketed light rounds, heavy edges
+ * once per bucket, empty buckets skipped), then a parent per reached vertex.
+ * Same distances as Dijkstra; unreached -1.0f / -1, root -> root. */
+void o_delta_stepping_omp(uint64_t n, const uint64_t *rs, const uint32_t *col, const float *w,
+                          uint64_t root, float delta, float *dist, int64_t *pred, int nthreads);
 /* Parent certificate for FP32 SSSP output: counts violations of
  * (a) dist[v] <= fl(dist[u]+w) on every edge, (b) tight parent edge,
  * (c) acyclic parent chains reaching the root, (d) unreached consistency. */

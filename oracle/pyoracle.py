@@ -65,6 +65,14 @@ def lib():
         L.o_sssp_check.argtypes = [C.c_uint64, u64p, u32p, f32p, C.c_uint64, f32p, i64p]
         L.o_sssp_check.restype = C.c_uint64
         L.o_statistics.argtypes = [f64p, C.c_int, f64p]
+        L.o_bfs_depth_omp.argtypes = [C.c_uint64, u64p, u32p, C.c_uint64, i64p, C.c_int]
+        L.o_bfs_depth_omp.restype = C.c_uint64
+        L.o_tuple_digest.argtypes = [C.c_uint64, u64p, f32p, C.POINTER(C.c_uint64)]
+        L.o_tuple_digest.restype = C.c_uint64
+        L.o_csr_digest.argtypes = [C.c_uint64, u64p, u32p, f32p]
+        L.o_csr_digest.restype = C.c_uint64
+        L.o_delta_stepping_omp.argtypes = [C.c_uint64, u64p, u32p, f32p, C.c_uint64, C.c_float, f32p, i64p,
+                                           C.c_int]
         _LIB = L
     return _LIB
 
@@ -157,6 +165,42 @@ def bfs_omp(g, root, nthreads):
     pred = np.zeros(g.n, np.int64)
     lib().o_bfs_omp(g.n, g.rowstarts, _c(g.column, np.uint32), root, pred, nthreads)
     return pred
+
+
+def bfs_depth_omp(g, root, nthreads=16):
+    """Hop counts (-1 unreached) from a parallel level-synchronous BFS."""
+    depth = np.zeros(g.n, np.int64)
+    lib().o_bfs_depth_omp(g.n, g.rowstarts, _c(g.column, np.uint32), root, depth, nthreads)
+    return depth
+
+
+def tuple_digest(edges, weights):
+    """(digest, entries) of the directed entries a tuple list stands for."""
+    edges = np.ascontiguousarray(edges, np.uint64).reshape(-1)
+    weights = np.ascontiguousarray(weights, np.float32)
+    cnt = C.c_uint64()
+    d = lib().o_tuple_digest(weights.shape[0], edges, weights, C.byref(cnt))
+    return d, cnt.value
+
+
+def csr_digest(g):
+    return lib().o_csr_digest(g.n, g.rowstarts, _c(g.column, np.uint32), _c(g.weights, np.float32))
+
+
+def rowstarts_of(n, edges):
+    """Row offsets of the CSR a tuple list gives (degree prefix sum, loops excluded)."""
+    edges = np.ascontiguousarray(edges, np.uint64).reshape(-1)
+    rs = np.zeros(n + 1, np.uint64)
+    lib().o_csr_build(n, edges.shape[0] // 2, edges, np.zeros(1, np.float32), rs, None, None)
+    return rs
+
+
+def delta_stepping_omp(g, root, delta=0.1, nthreads=16):
+    dist = np.zeros(g.n, np.float32)
+    pred = np.zeros(g.n, np.int64)
+    lib().o_delta_stepping_omp(g.n, g.rowstarts, _c(g.column, np.uint32), _c(g.weights, np.float32), root,
+                               delta, dist, pred, nthreads)
+    return dist, pred
 
 
 def dist_from_pred(g, pred, root):

@@ -336,3 +336,66 @@ def test_two_hop_rule_pins(oracle):
     assert dropped == [0.375 + m + 2 ** -20, 0.5]
     # the reverse entries (row 2) get the same decisions
     assert sorted(float(wt[k]) for k in range(rs[2], rs[3]) if drop[k]) == dropped
+
+
+# ------------------------------------------- checkers for the large graphs
+@pytest.mark.parametrize("scale", [10, 12, 14])
+def test_bfs_depth_omp_equals_serial(oracle, scale):
+    _, _, g = oracle_graph(scale)
+    for r in oracle.roots(g)[:6]:
+        _, depth = oracle.bfs(g, int(r))
+        for nt in (1, 3, 8):
+            assert np.array_equal(oracle.bfs_depth_omp(g, int(r), nt), depth)
+
+
+@pytest.mark.parametrize("scale", [10, 13])
+def test_edge_digests_agree_and_catch_mistakes(oracle, scale):
+    e, w, g = oracle_graph(scale)
+    d, cnt = oracle.tuple_digest(e, w)
+    assert cnt == g.m and d == oracle.csr_digest(g)
+    assert np.array_equal(oracle.rowstarts_of(g.n, e), g.rowstarts)
+    rng = np.random.default_rng(scale)
+    k = int(rng.integers(0, g.m))
+    col = g.column.copy()
+    col[k] = (col[k] + 1) % g.n                                        # one wrong column
+    assert oracle.csr_digest(oracle.Csr(g.n, g.rowstarts, col, g.weights)) != d
+    wt = g.weights.copy()
+    wt.view(np.uint32)[k] ^= 1                                         # one weight ulp off
+    assert oracle.csr_digest(oracle.Csr(g.n, g.rowstarts, g.column, wt)) != d
+    u = int(np.searchsorted(g.rowstarts.astype(np.int64), k, side="right") - 1)
+    rs = g.rowstarts.copy()
+    rs[u + 1:] -= np.uint64(1)                                         # entry k dropped
+    keep = np.ones(g.m, bool)
+    keep[k] = False
+    assert oracle.csr_digest(oracle.Csr(g.n, rs, g.column[keep], g.weights[keep])) != d
+    e2 = e.copy()
+    e2[0] = e2[0][::-1]                                                # endpoints of one edge swapped: same multiset
+    assert oracle.tuple_digest(e2, w)[0] == d
+
+
+@pytest.mark.parametrize("scale", [12, 14])
+def test_delta_stepping_omp_equals_dijkstra(oracle, scale):
+    _, _, g = oracle_graph(scale)
+    for r in oracle.roots(g)[:3]:
+        r = int(r)
+        d_ref, _ = oracle.dijkstra_f32(g, r)
+        for delta in (0.003, 0.05, 0.1, 0.25, 4.0):
+            for nt in (1, 8):
+                d, p = oracle.delta_stepping_omp(g, r, delta, nt)
+                assert np.array_equal(d.view(np.uint32), d_ref.view(np.uint32)), (r, delta, nt)
+                assert oracle.sssp_check(g, r, d, p) == 0
+
+
+def test_delta_stepping_omp_parents_acyclic_on_tiny_weights(oracle):
+    """Vertices at equal distance joined by a weight below half an ulp are
+    tight both ways; parents still form a tree (a per-vertex choice of any
+    tight neighbour would close 1 <-> 2 here)."""
+    e = np.array([[0, 1], [0, 2], [1, 2], [2, 3], [1, 3]], np.uint64)
+    w = np.array([0.5, 0.5, 1e-9, 0.25, 0.25], np.float32)
+    g = oracle.csr(4, e, w)
+    for nt in (1, 4):
+        d, p = oracle.delta_stepping_omp(g, 0, 0.1, nt)
+        assert list(d) == [0.0, 0.5, 0.5, 0.75]
+        assert oracle.sssp_check(g, 0, d, p) == 0
+    bad = np.array([0, 2, 1, 1], np.int64)  # 1 <-> 2 cycle
+    assert oracle.sssp_check(g, 0, d, bad) > 0
