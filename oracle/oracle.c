@@ -486,6 +486,40 @@ uint64_t o_csr_digest(uint64_t n, const uint64_t *rs, const uint32_t *col, const
     return sum;
 }
 
+/* traversal.cu:15-34 over one chunk of a tuple list: both endpoints of every
+ * non-loop edge count once. Endpoints outside [0, n) are counted and skipped. */
+uint64_t o_degree_add(uint64_t n, uint64_t nedges, const uint64_t *edges, uint64_t *deg) {
+    uint64_t bad = 0;
+#pragma omp parallel for schedule(static) reduction(+ : bad)
+    for (uint64_t i = 0; i < nedges; ++i) {
+        uint64_t a = edges[2 * i], b = edges[2 * i + 1];
+        if (a >= n || b >= n) { ++bad; continue; }
+        if (a == b) continue;
+        __atomic_fetch_add(&deg[a], 1, __ATOMIC_RELAXED);
+        __atomic_fetch_add(&deg[b], 1, __ATOMIC_RELAXED);
+    }
+    return bad;
+}
+
+/* The canonical row order of o_csr_build: rowstarts start at 0 and never
+ * decrease; inside a row, keys (weight bits << 32 | column) ascend. Counts the
+ * decreasing rowstarts steps plus the rows with a descending adjacent pair. */
+uint64_t o_csr_order_errors(uint64_t n, const uint64_t *rs, const uint32_t *col, const float *cw) {
+    uint64_t err = rs[0] != 0;
+    for (uint64_t u = 0; u < n; ++u) err += rs[u + 1] < rs[u];
+    if (err) return err;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : err)
+    for (uint64_t u = 0; u < n; ++u) {
+        uint64_t prev = 0;
+        for (uint64_t k = rs[u]; k < rs[u + 1]; ++k) {
+            uint64_t key = ((uint64_t)fbits(cw[k]) << 32) | col[k];
+            if (k > rs[u] && key < prev) { ++err; break; }
+            prev = key;
+        }
+    }
+    return err;
+}
+
 /* traversal.cu:75-112, :505-563: level-by-level, a vertex v gets level L+1
  * when the entry v appears in the row of u = pred[v] and u is at level L. */
 void o_dist_from_pred(uint64_t n, const uint64_t *rs, const uint32_t *col, const int64_t *pred,

@@ -69,6 +69,13 @@ uint64_t o_bfs_depth_omp(uint64_t n, const uint64_t *rs, const uint32_t *col, ui
  * directions of each non-loop edge; *entries gets their count) and of a CSR. */
 uint64_t o_tuple_digest(uint64_t nedges, const uint64_t *edges, const float *weights, uint64_t *entries);
 uint64_t o_csr_digest(uint64_t n, const uint64_t *rs, const uint32_t *col, const float *cw);
+/* Degree histogram of one chunk of a tuple list added into deg[n] (both
+ * endpoints of each non-loop edge, traversal.cu:15-34); returns the edges with
+ * an endpoint outside [0, n), which are skipped. */
+uint64_t o_degree_add(uint64_t n, uint64_t nedges, const uint64_t *edges, uint64_t *deg);
+/* Violations of the canonical CSR order (rowstarts from 0, non-decreasing;
+ * rows ascending by (weight bits << 32 | column)). */
+uint64_t o_csr_order_errors(uint64_t n, const uint64_t *rs, const uint32_t *col, const float *cw);
 /* bfs_compute_dist_from_pred semantics (traversal.cu:75-112, :505-563):
  * -1 unreached, INT64_MAX reached-but-unresolved, root 0. */
 void o_dist_from_pred(uint64_t n, const uint64_t *rs, const uint32_t *col, const int64_t *pred,

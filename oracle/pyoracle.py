@@ -71,6 +71,10 @@ def lib():
         L.o_tuple_digest.restype = C.c_uint64
         L.o_csr_digest.argtypes = [C.c_uint64, u64p, u32p, f32p]
         L.o_csr_digest.restype = C.c_uint64
+        L.o_degree_add.argtypes = [C.c_uint64, C.c_uint64, u64p, u64p]
+        L.o_degree_add.restype = C.c_uint64
+        L.o_csr_order_errors.argtypes = [C.c_uint64, u64p, u32p, f32p]
+        L.o_csr_order_errors.restype = C.c_uint64
         L.o_delta_stepping_omp.argtypes = [C.c_uint64, u64p, u32p, f32p, C.c_uint64, C.c_float, f32p, i64p,
                                            C.c_int]
         _LIB = L
@@ -185,6 +189,17 @@ def tuple_digest(edges, weights):
 
 def csr_digest(g):
     return lib().o_csr_digest(g.n, g.rowstarts, _c(g.column, np.uint32), _c(g.weights, np.float32))
+
+
+def degree_add(n, edges, deg):
+    """Adds the degrees of a chunk of tuples into deg (u64[n]); returns the
+    edges with an endpoint outside [0, n)."""
+    edges = np.ascontiguousarray(edges, np.uint64).reshape(-1)
+    return int(lib().o_degree_add(n, edges.shape[0] // 2, edges, deg))
+
+
+def csr_order_errors(g):
+    return int(lib().o_csr_order_errors(g.n, g.rowstarts, _c(g.column, np.uint32), _c(g.weights, np.float32)))
 
 
 def rowstarts_of(n, edges):

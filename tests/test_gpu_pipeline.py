@@ -696,3 +696,98 @@ def test_plain_launch_path(arena, oracle):
             assert np.array_equal(d.view(np.uint32), d_ref.view(np.uint32))
     finally:
         arena.set_param("coop_launch", 1)
+
+
+# ------------------------------------------------------------ tuple dump / load
+def _dump(sess, tg, start, path):
+    os.environ["G500_DUMP_PATH"] = path
+    try:
+        sess.lib.tuple_graph_dump(C.byref(tg), int(start))
+        sess.check()
+    finally:
+        del os.environ["G500_DUMP_PATH"]
+
+
+def _load(sess, start, count, path):
+    import g500
+    tg = g500.TupleGraph()
+    os.environ["G500_DUMP_PATH"] = path
+    try:
+        sess.lib.tuple_graph_load(C.byref(sess.ws), C.byref(tg), int(start), int(count))
+        sess.check()
+    finally:
+        del os.environ["G500_DUMP_PATH"]
+    return tg
+
+
+def _file_tuples(path, a, b):
+    e = np.fromfile(path, dtype="<u8", count=2 * (b - a), offset=16 * a).reshape(-1, 2)
+    w = np.fromfile(path + ".weights", dtype="<f4", count=b - a, offset=4 * a)
+    return e, w
+
+
+def test_tuple_dump_load_roundtrip(arena, oracle, tmp_path):
+    """graph_generation.cu:314-396: raw little-endian {u64 v0, u64 v1} per edge
+    in <path>, f32 per edge in <path>.weights, each file nglobaledges long, a
+    range written at its start_edge offset. Two ranges dumped into the same
+    files (as two ranks would) give the whole list; loads return the bytes."""
+    scale, ef = 12, 16
+    E = ef << scale
+    eo, wo = oracle.kronecker(scale, ef)
+    path = str(tmp_path / "s12.bin")
+    half = E // 2 + 123
+    for a, b in ((0, half), (half, E)):
+        tg = arena.generate(scale, ef, start=a, end=b)
+        _dump(arena, tg, a, path)
+    assert os.path.getsize(path) == 16 * E and os.path.getsize(path + ".weights") == 4 * E
+    e, w = _file_tuples(path, 0, E)
+    assert np.array_equal(e, eo) and np.array_equal(w.view(np.uint32), wo.view(np.uint32))
+    for a, b in ((0, E), (777, 3001), (E - 5, E)):
+        tg = _load(arena, a, b - a, path)
+        assert tg.edges_size == b - a and tg.max_edges_size == b - a
+        e, w = arena.tuples_to_host(tg)
+        assert np.array_equal(e, eo[a:b]) and np.array_equal(w.view(np.uint32), wo[a:b].view(np.uint32))
+    # a range dumped alone: the rest of the files reads as zeros
+    path2 = str(tmp_path / "range.bin")
+    tg = arena.generate(scale, ef, start=1000, end=5000)
+    _dump(arena, tg, 1000, path2)
+    e, w = _file_tuples(path2, 0, E)
+    assert np.array_equal(e[1000:5000], eo[1000:5000]) and not e[:1000].any() and not e[5000:].any()
+    assert np.array_equal(w[1000:5000].view(np.uint32), wo[1000:5000].view(np.uint32)) and not w[5000:].any()
+    # a graph loaded from the files builds the oracle's CSR
+    tg = _load(arena, 0, E, path)
+    tg.nglobaledges = E
+    g = arena.csr(tg, 1 << scale)
+    rs, col, _ = arena.csr_to_host(g)
+    ref = oracle_graph(scale)[2]
+    assert np.array_equal(rs, ref.rowstarts) and np.array_equal(col, ref.column)
+    with pytest.raises(Exception):
+        _load(arena, 0, 10, str(tmp_path / "missing.bin"))
+
+
+def test_tuple_dump_load_past_2gib(arena, oracle, tmp_path):
+    """A range whose byte count passes 2^31 (the reference casts the count
+    to int, graph_generation.cu:340): written and read whole."""
+    import shutil
+    scale, ef = 24, 16
+    cnt = (1 << 27) + 4096                     # 2 GiB + 64 KiB of edges
+    if shutil.disk_usage(str(tmp_path)).free < 8 << 30:
+        pytest.skip("needs 8 GiB of free space in the temporary directory")
+    path = str(tmp_path / "big.bin")
+    tg = arena.generate(scale, ef, start=0, end=cnt)
+    _dump(arena, tg, 0, path)
+    assert os.path.getsize(path) == 16 * (ef << scale)
+    for a, b in ((0, 4096), (cnt - 8192, cnt)):
+        e, w = _file_tuples(path, a, b)
+        eo, wo = oracle.kronecker(scale, ef, start=a, end=b)
+        assert np.array_equal(e, eo) and np.array_equal(w.view(np.uint32), wo.view(np.uint32))
+    m = arena.mark()
+    back = _load(arena, 0, cnt, path)
+    for a, b in ((0, 4096), ((1 << 27) - 100, cnt)):
+        e = arena.to_host(back.edges + 16 * a, 2 * (b - a), np.uint64).reshape(-1, 2)
+        w = arena.to_host(back.weights + 4 * a, b - a, np.float32)
+        eo, wo = oracle.kronecker(scale, ef, start=a, end=b)
+        assert np.array_equal(e, eo) and np.array_equal(w.view(np.uint32), wo.view(np.uint32))
+    arena.release(m)
+    os.remove(path)
+    os.remove(path + ".weights")

@@ -373,6 +373,41 @@ def test_edge_digests_agree_and_catch_mistakes(oracle, scale):
     assert oracle.tuple_digest(e2, w)[0] == d
 
 
+@pytest.mark.parametrize("scale", [10, 13])
+def test_chunked_degrees_and_row_order(oracle, scale):
+    """Degrees added chunk by chunk equal numpy's bincount of the non-loop
+    endpoints (and so the CSR's rowstarts); the oracle CSR has no order
+    violations, and each planted one is found."""
+    e, w, g = oracle_graph(scale)
+    n = g.n
+    deg = np.zeros(n, np.uint64)
+    for a in range(0, len(e), 3001):                                   # ragged chunks
+        assert oracle.degree_add(n, e[a:a + 3001], deg) == 0
+    keep = e[:, 0] != e[:, 1]
+    ref = np.bincount(e[keep].reshape(-1).astype(np.int64), minlength=n)
+    assert np.array_equal(deg, ref.astype(np.uint64))
+    assert np.array_equal(np.concatenate([[0], np.cumsum(deg)]).astype(np.uint64), g.rowstarts)
+    bad = e[:5].copy()
+    bad[2, 1] = n                                                      # endpoint out of range
+    assert oracle.degree_add(n, bad, np.zeros(n, np.uint64)) == 1
+    assert oracle.csr_order_errors(g) == 0
+    rs = g.rowstarts.astype(np.int64)
+    u = int(np.argmax(np.diff(rs)))                                    # the longest row
+    b = rs[u]
+    col = g.column.copy()
+    col[[b, b + 1]] = col[[b + 1, b]]                                  # two entries swapped
+    wt = g.weights.copy()
+    wt[[b, b + 1]] = wt[[b + 1, b]]
+    same = g.weights[b] == g.weights[b + 1] and g.column[b] == g.column[b + 1]
+    assert oracle.csr_order_errors(oracle.Csr(n, g.rowstarts, col, wt)) == (0 if same else 1)
+    wt2 = g.weights.copy()
+    wt2[b] = np.float32(2.0)                                           # a heavier first entry
+    assert oracle.csr_order_errors(oracle.Csr(n, g.rowstarts, g.column, wt2)) == 1
+    rs2 = g.rowstarts.copy()
+    rs2[u + 1] = rs2[u] - np.uint64(1) if rs2[u] > 0 else rs2[u + 1]   # a backwards step
+    assert oracle.csr_order_errors(oracle.Csr(n, rs2, g.column, g.weights)) >= (1 if rs2[u] > 0 else 0)
+
+
 @pytest.mark.parametrize("scale", [12, 14])
 def test_delta_stepping_omp_equals_dijkstra(oracle, scale):
     _, _, g = oracle_graph(scale)
