@@ -309,6 +309,25 @@ __global__ __launch_bounds__(256) void depth_final_kernel(i64 *__restrict__ dist
         if (dist[v] == kPending) dist[v] = kUnresolved;
 }
 
+void depth_from_pred(const Oned_Graph *g, const i64 *pred, i64 *dist, u64 root, u32 *changed) {
+    Context &c = ctx();
+    const u64 n = g->nlocalverts;
+    const hipStream_t s = c.stream;
+    depth_init_kernel<<<grid_for(n * kWave, 256, (u64)c.num_cus * 32), 256, 0, s>>>(g->rowstarts, g->column, pred,
+                                                                                   dist, n, root);
+    for (u64 iter = 0; iter <= n; ++iter) {
+        G500_HIP(hipMemsetAsync(changed, 0, sizeof(u32), s));
+        depth_step_kernel<<<grid_for(n, 256, (u64)c.num_cus * 32), 256, 0, s>>>(pred, dist, n, changed);
+        u32 h = 0;
+        G500_HIP(hipMemcpyAsync(&h, changed, sizeof(u32), hipMemcpyDeviceToHost, s));
+        sync();
+        if (!h || failed()) break;
+    }
+    depth_final_kernel<<<grid_for(n, 256, (u64)c.num_cus * 32), 256, 0, s>>>(dist, n);
+    G500_HIP(hipGetLastError());
+    sync();
+}
+
 }  // namespace g500
 
 using namespace g500;
@@ -404,7 +423,6 @@ void oned_graph_bfs_run(Worker_State *ws, const Oned_Graph *g, Bfs_State *bfs, u
 }
 
 i64 *bfs_compute_dist_from_pred(Worker_State *ws, const Oned_Graph *g, Bfs_State *bfs, u64 root) {
-    Context &c = ctx();
     if (root >= g->nglobalverts) {
         fail("BFS root %llu is outside graph vertex range [0, %llu)", (unsigned long long)root,
              (unsigned long long)g->nglobalverts);
@@ -412,23 +430,9 @@ i64 *bfs_compute_dist_from_pred(Worker_State *ws, const Oned_Graph *g, Bfs_State
     }
     const u64 n = g->nlocalverts;
     if (!bfs->dist) bfs->dist = (i64 *)arena_alloc(ws, n * sizeof(i64), "BFS depth");
-    u32 *changed = &((BfsCtl *)bfs->ctl)->depth_changed;
     if (failed()) return nullptr;
-    const hipStream_t s = c.stream;
-    depth_init_kernel<<<grid_for(n * kWave, 256, (u64)c.num_cus * 32), 256, 0, s>>>(g->rowstarts, g->column,
-                                                                                   bfs->pred, bfs->dist, n, root);
-    for (u64 iter = 0; iter <= n; ++iter) {
-        G500_HIP(hipMemsetAsync(changed, 0, sizeof(u32), s));
-        depth_step_kernel<<<grid_for(n, 256, (u64)c.num_cus * 32), 256, 0, s>>>(bfs->pred, bfs->dist, n, changed);
-        u32 h = 0;
-        G500_HIP(hipMemcpyAsync(&h, changed, sizeof(u32), hipMemcpyDeviceToHost, s));
-        sync();
-        if (!h || failed()) break;
-    }
-    depth_final_kernel<<<grid_for(n, 256, (u64)c.num_cus * 32), 256, 0, s>>>(bfs->dist, n);
-    G500_HIP(hipGetLastError());
-    sync();
-    return bfs->dist;
+    depth_from_pred(g, bfs->pred, bfs->dist, root, &((BfsCtl *)bfs->ctl)->depth_changed);
+    return failed() ? nullptr : bfs->dist;
 }
 
 void oned_graph_bfs_destroy(Bfs_State *bfs) {

@@ -66,6 +66,15 @@ inline bool failed() { return t_error_sink ? t_error_sink->status != 0 : ctx().s
 // counters); worker_init creates them again.
 void release_library_resources();
 
+// Depth of every vertex from a parent array (bfs_compute_dist_from_pred's
+// semantics) into dist[n] on the library stream; `changed` is a device u32
+// the level loop uses. Blocking.
+void depth_from_pred(const Oned_Graph *g, const i64 *pred, i64 *dist, u64 root, u32 *changed);
+
+// Resolves, for the current device, the device address of the static scratch
+// array of the no-allocation validation entry point (once per device).
+void resolve_noalloc_scratch();
+
 #define G500_HIP(call)                                                                   \
     do {                                                                                 \
         hipError_t e_ = (call);                                                          \

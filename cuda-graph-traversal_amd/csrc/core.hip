@@ -228,6 +228,7 @@ Worker_State worker_init(i32 argc, char **argv, u64 gpu_memory_size) {
     }
     if (failed()) return ws;
     init_mrg_table();
+    resolve_noalloc_scratch();
     ws.cpu_arena = arena_create_cpu(MEGABYTE(256));
     ws.gpu_arena = arena_create_gpu(gpu_memory_size);
     return ws;

@@ -4,26 +4,38 @@
 // :84-169 (validation kernels), :171-201 (edge count), :203-255 (driver).
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
 
 namespace g500 {
 
+// Confirmations are one bit per vertex (u32 words), set with atomic ORs; the
+// entry points clear the n / 32 words before the init kernel.
+__device__ __forceinline__ void confirm(u32 *bits, u64 v) { atomicOr(&bits[v >> 5], 1u << (v & 31)); }
+__device__ __forceinline__ bool is_confirmed(const u32 *bits, u64 v) { return (bits[v >> 5] >> (v & 31)) & 1u; }
+
+// Clears the confirmation words and the error counter.
+__global__ __launch_bounds__(256) void validate_clear_kernel(u32 *__restrict__ confirmed, u64 nwords,
+                                                             u64 *__restrict__ errors) {
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += (u64)gridDim.x * blockDim.x)
+        confirmed[i] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *errors = 0;
+}
+
 // Rules 1 and 2 (graph500_runner.cu:84-109).
 __global__ __launch_bounds__(256) void validate_init_kernel(const i64 *__restrict__ pred, const i64 *__restrict__ dist,
-                                                            u8 *__restrict__ confirmed, u64 *__restrict__ errors,
+                                                            u32 *__restrict__ confirmed, u64 *__restrict__ errors,
                                                             u64 n, u64 root) {
     u64 err = 0;
     for (u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (u64)gridDim.x * blockDim.x) {
-        u8 conf = 0;
         const i64 p = pred[v];
         if (p != -1 && (p < 0 || (u64)p >= n)) ++err;
         if (v == root) {
             if (p != (i64)root || dist[v] != 0) ++err;
-            else conf = 1;
+            else confirm(confirmed, v);
         }
-        confirmed[v] = conf;
     }
     err = wave_sum(err);
     if (lane_id() == 0 && err) atomicAdd((unsigned long long *)errors, (unsigned long long)err);
@@ -35,7 +47,7 @@ __device__ __forceinline__ i64 wrap_inc(i64 x) { return (i64)((u64)x + 1u); }
 // (graph500_runner.cu:111-153).
 __global__ __launch_bounds__(256) void validate_edges_kernel(const u64 *__restrict__ rs, const u32 *__restrict__ col,
                                                              const i64 *__restrict__ pred, const i64 *__restrict__ dist,
-                                                             u8 *__restrict__ confirmed, u64 *__restrict__ errors, u64 n) {
+                                                             u32 *__restrict__ confirmed, u64 *__restrict__ errors, u64 n) {
     const u32 lane = lane_id();
     const u64 wave = ((u64)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const u64 nw = (u64)gridDim.x * blockDim.x / kWave;
@@ -54,9 +66,9 @@ __global__ __launch_bounds__(256) void validate_edges_kernel(const u64 *__restri
             if ((sd == -1 && dd != -1) || (sd != -1 && dd == -1)) ++err;
             if (sd >= 0 && dd >= 0 && (wrap_inc(sd) < dd || wrap_inc(dd) < sd)) ++err;
             if (sp == (i64)d) self_ok = true;
-            if (dp == (i64)s) confirmed[d] = 1;
+            if (dp == (i64)s) confirm(confirmed, d);
         }
-        if (ballot(self_ok) && lane == 0) confirmed[s] = 1;
+        if (ballot(self_ok) && lane == 0) confirm(confirmed, s);
     }
     err = wave_sum(err);
     if (lane == 0 && err) atomicAdd((unsigned long long *)errors, (unsigned long long)err);
@@ -65,12 +77,12 @@ __global__ __launch_bounds__(256) void validate_edges_kernel(const u64 *__restri
 // Rule 7 (graph500_runner.cu:155-169).
 __global__ __launch_bounds__(256) void validate_confirmed_kernel(const i64 *__restrict__ pred,
                                                                  const i64 *__restrict__ dist,
-                                                                 const u8 *__restrict__ confirmed,
+                                                                 const u32 *__restrict__ confirmed,
                                                                  u64 *__restrict__ errors, u64 n, u64 root) {
     u64 err = 0;
     for (u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (u64)gridDim.x * blockDim.x) {
         if (pred[v] == -1) continue;
-        if (dist[v] == INT64_MAX || (v != root && !confirmed[v])) ++err;
+        if (dist[v] == INT64_MAX || (v != root && !is_confirmed(confirmed, v))) ++err;
     }
     err = wave_sum(err);
     if (lane_id() == 0 && err) atomicAdd((unsigned long long *)errors, (unsigned long long)err);
@@ -99,6 +111,65 @@ static u64 *counter_scratch() {
     return c.d_counters;
 }
 
+// Rules 1-7 on (pred, dist) with the confirmation words and error counter
+// given; returns the error count (blocking).
+static u64 validate_enqueue(const Oned_Graph *g, const i64 *pred, const i64 *dist, u64 root, u32 *confirmed,
+                            u64 *errors) {
+    Context &c = ctx();
+    const u64 n = g->nlocalverts, nwords = (n + 31) / 32;
+    const hipStream_t s = c.stream;
+    const unsigned gv = grid_for(n, 256, (u64)c.num_cus * 32), ge = grid_for(n * kWave, 256, (u64)c.num_cus * 32);
+    validate_clear_kernel<<<grid_for(nwords, 256, (u64)c.num_cus * 8), 256, 0, s>>>(confirmed, nwords, errors);
+    validate_init_kernel<<<gv, 256, 0, s>>>(pred, dist, confirmed, errors, n, root);
+    validate_edges_kernel<<<ge, 256, 0, s>>>(g->rowstarts, g->column, pred, dist, confirmed, errors, n);
+    validate_confirmed_kernel<<<gv, 256, 0, s>>>(pred, dist, confirmed, errors, n, root);
+    G500_HIP(hipGetLastError());
+    u64 h = 0;
+    g500_copy_to_host(&h, errors, sizeof(u64));
+    return h;
+}
+
+// ------------------------------------------- scratch of the no-allocation path
+// oned_graph_bfs_validate_noalloc keeps its scratch in this file-scope array,
+// one copy per device, held until the process ends: 256 bytes of counters
+// (the error count at 0, the depth loop's flag at 8) and then the
+// confirmation bits of up to kNoAllocMaxVerts vertices.
+constexpr u64 kNoAllocMaxVerts = 1ull << 27;
+constexpr u64 kNoAllocCounters = 256;
+constexpr u64 kNoAllocBytes = kNoAllocCounters + kNoAllocMaxVerts / 8;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) u8 scratch[kNoAllocBytes];
+
+static std::mutex &scratch_mutex() {
+    static std::mutex m;
+    return m;
+}
+static u8 *g_scratch[kMaxDevices];
+
+// The device address of `scratch` on the current device (resolved once per
+// device; the host-side name is not a device address). nullptr after fail().
+static u8 *noalloc_scratch() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        fail("no-allocation validation: no current HIP device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(scratch_mutex());
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            fail("no-allocation validation: hipGetSymbolAddress of the scratch array failed on device %d: %s", dev,
+                 hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch[dev] = (u8 *)p;
+    }
+    return g_scratch[dev];
+}
+
+void resolve_noalloc_scratch() { (void)noalloc_scratch(); }
+
 }  // namespace g500
 
 using namespace g500;
@@ -121,24 +192,53 @@ u64 oned_graph_edge_count(Worker_State *ws, const Oned_Graph *g, const i64 *pred
 }
 
 b32 oned_graph_bfs_validate(Worker_State *ws, const Oned_Graph *g, Bfs_State *bfs, u64 root, u64 *errors_out) {
-    Context &c = ctx();
     const i64 *dist = bfs_compute_dist_from_pred(ws, g, bfs, root);
     if (!dist || failed()) return 0;
     const u64 n = g->nlocalverts;
     const u64 mark = ws->gpu_arena.used;
-    u8 *confirmed = (u8 *)arena_alloc(ws, n, "validation confirmations");
+    u32 *confirmed = (u32 *)arena_alloc(ws, (n + 31) / 32 * sizeof(u32), "validation confirmations");
     u64 *errors = counter_scratch();
     if (failed()) return 0;
-    const hipStream_t s = c.stream;
-    G500_HIP(hipMemsetAsync(errors, 0, sizeof(u64), s));
-    const unsigned gv = grid_for(n, 256, (u64)c.num_cus * 32), ge = grid_for(n * kWave, 256, (u64)c.num_cus * 32);
-    validate_init_kernel<<<gv, 256, 0, s>>>(bfs->pred, dist, confirmed, errors, n, root);
-    validate_edges_kernel<<<ge, 256, 0, s>>>(g->rowstarts, g->column, bfs->pred, dist, confirmed, errors, n);
-    validate_confirmed_kernel<<<gv, 256, 0, s>>>(bfs->pred, dist, confirmed, errors, n, root);
-    G500_HIP(hipGetLastError());
-    u64 h = 0;
-    g500_copy_to_host(&h, errors, sizeof(u64));
+    const u64 h = validate_enqueue(g, bfs->pred, dist, root, confirmed, errors);
     ws->gpu_arena.used = mark;
+    if (failed()) return 0;
+    if (errors_out) *errors_out = h;
+    if (h) fprintf(stderr, "Validation Error: BFS validation found %llu error(s)\n", (unsigned long long)h);
+    return h == 0;
+}
+
+b32 oned_graph_bfs_validate_noalloc(const Oned_Graph *g, const i64 *pred, u64 root, i64 *depth, u64 *errors_out) {
+    Context &c = ctx();
+    if (!c.stream) {
+        fail("oned_graph_bfs_validate_noalloc: call worker_init first");
+        return 0;
+    }
+    const u64 n = g->nlocalverts;
+    if (n > kNoAllocMaxVerts) {
+        fail("oned_graph_bfs_validate_noalloc: %llu vertices exceed the no-allocation maximum of %llu "
+             "(use oned_graph_bfs_validate)",
+             (unsigned long long)n, (unsigned long long)kNoAllocMaxVerts);
+        return 0;
+    }
+    if (root >= g->nglobalverts) {
+        fail("BFS root %llu is outside graph vertex range [0, %llu)", (unsigned long long)root,
+             (unsigned long long)g->nglobalverts);
+        return 0;
+    }
+    if (!pred || !depth) {
+        fail("oned_graph_bfs_validate_noalloc: pred and depth must be device arrays of %llu entries",
+             (unsigned long long)n);
+        return 0;
+    }
+    u8 *base = noalloc_scratch();
+    if (!base || failed()) return 0;
+    u64 *errors = (u64 *)base;
+    u32 *changed = (u32 *)(base + 8);
+    u32 *confirmed = (u32 *)(base + kNoAllocCounters);
+    depth_from_pred(g, pred, depth, root, changed);
+    if (failed()) return 0;
+    const u64 h = validate_enqueue(g, pred, depth, root, confirmed, errors);
+    if (failed()) return 0;
     if (errors_out) *errors_out = h;
     if (h) fprintf(stderr, "Validation Error: BFS validation found %llu error(s)\n", (unsigned long long)h);
     return h == 0;

@@ -121,6 +121,7 @@ _SIGS = {
     "generate_bfs_roots": (i32, [P(WorkerState), P(OnedGraph), u64, u64, P(u64), i32]),
     "oned_graph_edge_count": (u64, [P(WorkerState), P(OnedGraph), vp]),
     "oned_graph_bfs_validate": (i32, [P(WorkerState), P(OnedGraph), P(BfsState), u64, P(u64)]),
+    "oned_graph_bfs_validate_noalloc": (i32, [P(OnedGraph), vp, u64, vp, P(u64)]),
     "get_statistics": (None, [P(f64), i32, P(f64)]),
     "oned_graph_bfs_batch_create": (BfsBatchState, [P(WorkerState), P(OnedGraph)]),
     "oned_graph_bfs_batch_run": (None, [P(WorkerState), P(OnedGraph), P(BfsBatchState), P(u64), i32]),

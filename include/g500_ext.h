@@ -27,6 +27,18 @@ u64 oned_graph_edge_count(Worker_State *ws, const Oned_Graph *g, const i64 *pred
  * error is found; *errors_out (may be NULL) receives the error count. */
 b32 oned_graph_bfs_validate(Worker_State *ws, const Oned_Graph *g, Bfs_State *bfs, u64 root,
                             u64 *errors_out);
+/* No counterpart in the reference. The same validation for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): it calls no hipMalloc, hipFree or
+ * arena push. Inputs: the graph and a device parent array pred[n]; outputs:
+ * depth[n] (device, bfs_compute_dist_from_pred's values) and *errors_out.
+ * Its scratch (confirmation bits and counters, 16 MiB + 256 B) is a static
+ * device array that each GPU holds until the process ends, so graphs are
+ * limited to 2^27 vertices (larger ones fail with an error). Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU race. Blocking;
+ * needs worker_init first. */
+b32 oned_graph_bfs_validate_noalloc(const Oned_Graph *g, const i64 *pred, u64 root, i64 *depth,
+                                    u64 *errors_out);
 /* graph500_runner.cu:22-41: min, q1, median, q3, max, mean, stddev. */
 void get_statistics(const f64 *x, i32 n, f64 r[7]);
 
