@@ -40,30 +40,69 @@ def hm(times_s, edges):
     return len(times_s) / float(np.sum(np.asarray(times_s) / np.asarray(edges, dtype=np.float64)))
 
 
-def cpu_baseline(sess, g, roots, m_c, seconds, nthreads):
-    """The CPU oracle on the host cores, on a bounded sample of the same graph:
-    an OpenMP level-synchronous BFS from the first root (whole traversal) and
-    a binary-heap FP32 Dijkstra from the same root stopped after `seconds`."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sess, g, roots, m_c, budget_s, nthreads, full=False):
+    """The CPU oracle on the host cores, over complete traversals of the same
+    graph and giant-component roots (in root order): OpenMP level-synchronous
+    BFS and OpenMP delta-stepping (delta 0.1, the reference's value), each from
+    as many roots as fit half the time budget (at least 2 and 1). With `full`,
+    the planned baseline: OpenMP BFS from all giant roots, delta-stepping from
+    8, and serial queue BFS and binary-heap Dijkstra from 2 roots each.
+    Timing is the traversal call alone, as the runner times its kernels."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
     rs, col, w = sess.csr_to_host(g)
     ref = O.Csr(int(g.nlocalverts), rs, col, w)
-    root = int(roots[0])
-    t0 = time.perf_counter()
-    O.bfs_omp(ref, root, nthreads)
-    tb = time.perf_counter() - t0
-    scanned, settled, ts = O.dijkstra_f32_timed(ref, root, seconds)
-    teps_b = m_c[0] / tb
-    teps_s = (scanned / 2.0) / ts  # each undirected edge is scanned from both ends
-    value = 2.0 / (1.0 / teps_b + 1.0 / teps_s) / 1e9
-    return {
-        "value": value, "unit": "GTEPS", "cores": nthreads, "kind": "port",
-        "sample": (f"oracle on the same SCALE {int(np.log2(g.nlocalverts))} CSR, root {root}: OpenMP "
-                   f"top-down BFS ({nthreads} threads, whole traversal, {tb:.2f} s, "
-                   f"{teps_b / 1e9:.3f} GTEPS) and serial binary-heap FP32 Dijkstra (1 thread) stopped "
-                   f"after {ts:.1f} s with {settled} vertices settled ({teps_s / 1e9:.4f} GTEPS as scanned "
-                   f"entries/2 per second); value = 2/(1/BFS + 1/SSSP)"),
+    giant = [i for i in range(len(roots)) if m_c[i] >= 0.5 * m_c.max()]
+
+    def run(fn, limit, at_least, at_most):
+        done, spent = [], 0.0
+        for i in giant[:at_most]:
+            if len(done) >= at_least and spent >= limit:
+                break
+            t0 = time.perf_counter()
+            fn(int(roots[i]))
+            dt = time.perf_counter() - t0
+            done.append((i, dt))
+            spent += dt
+        return done
+
+    half = budget_s / 2
+    bfs = run(lambda r: O.bfs_omp(ref, r, nthreads), half, 2, len(giant) if full else 64)
+    ds = run(lambda r: O.delta_stepping_omp(ref, r, 0.1, nthreads), half, 8 if full else 1, 8 if full else 64)
+    gteps = lambda runs: sum(m_c[i] for i, _ in runs) / sum(t for _, t in runs) / 1e9
+    b, d = gteps(bfs), gteps(ds)
+    out = {
+        "value": 2.0 / (1.0 / b + 1.0 / d), "unit": "GTEPS", "cores": nthreads, "kind": "port",
+        "cpu": cpu_model(), "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+        "bfs_omp": {"roots": [int(roots[i]) for i, _ in bfs], "s": [round(t, 3) for _, t in bfs],
+                    "gteps": round(b, 4)},
+        "sssp_omp": {"roots": [int(roots[i]) for i, _ in ds], "s": [round(t, 3) for _, t in ds],
+                     "gteps": round(d, 4), "delta": 0.1},
     }
+    if full:
+        ser = run(lambda r: O.bfs(ref, r, False), 0.0, 2, 2)
+        dij = run(lambda r: O.dijkstra_f32(ref, r), 0.0, 2, 2)
+        out["bfs_serial"] = {"roots": [int(roots[i]) for i, _ in ser], "s": [round(t, 3) for _, t in ser],
+                             "gteps": round(gteps(ser), 4)}
+        out["dijkstra_serial"] = {"roots": [int(roots[i]) for i, _ in dij], "s": [round(t, 3) for _, t in dij],
+                                  "gteps": round(gteps(dij), 4)}
+    scale = int(np.log2(g.nlocalverts))
+    out["sample"] = (f"oracle on the same SCALE {scale} CSR ({out['cpu']}, nproc {out['nproc']}, {nthreads} threads "
+                     f"used): complete OpenMP level-synchronous BFS from {len(bfs)} giant-component roots "
+                     f"({sum(t for _, t in bfs):.1f} s, {b:.3f} GTEPS) and complete OpenMP delta-stepping "
+                     f"(delta 0.1) from {len(ds)} ({sum(t for _, t in ds):.1f} s, {d:.4f} GTEPS); "
+                     f"value = 2/(1/BFS + 1/SSSP), GTEPS = sum of m_c / sum of times")
+    return out
 
 
 def main():
@@ -88,7 +127,11 @@ def main():
                     help="skip the autotuner (knobs at their defaults, delta from --delta)")
     ap.add_argument("--arena-gib", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=30.0,
+                    help="time budget of the CPU baseline's complete traversals")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="the full CPU baseline: OpenMP BFS from every giant root, delta-stepping from 8, "
+                         "serial BFS and Dijkstra from 2 (minutes at SCALE 26)")
     ap.add_argument("--serve-requests", type=int, default=512,
                     help="requests of the serving leg (0: skip); resident graphs S16-S24")
     ap.add_argument("--serve-rate", type=float, default=1000.0, help="Poisson arrivals per second")
@@ -153,12 +196,26 @@ def main():
     log(f"[bench] {nroots} roots; edges per root ~{np.median(m_c):.4g}, reached ~{np.median(n_c):.4g}; "
         f"smallest components (edges): {sorted(m_c)[:4]}")
 
-    def plain_pass(idx, label):
+    checks = {"plain_bfs": 0, "plain_bfs_failures": 0, "plain_sssp": 0, "plain_sssp_failures": 0,
+              "batched_bfs": 0, "batched_bfs_failures": 0, "batched_sssp": 0, "batched_sssp_failures": 0}
+
+    def plain_pass(idx, label, validate=False):
+        """Times single-root BFS and SSSP from roots[idx]; with `validate`, each
+        result is checked after its timed run (rules 1-7 for the BFS tree, the
+        certificate for the SSSP distances and parents)."""
         tb = np.zeros(len(idx))
         ts = np.zeros(len(idx))
         for i, ri in enumerate(idx):
-            tb[i] = sess.bfs(g, bfs, roots[ri]) * 1e-3
-            ts[i] = sess.sssp(g, sssp, roots[ri]) * 1e-3
+            r = int(roots[ri])
+            tb[i] = sess.bfs(g, bfs, r) * 1e-3
+            if validate:
+                ok, _ = sess.bfs_validate(g, bfs, r)
+                checks["plain_bfs"] += 1
+                checks["plain_bfs_failures"] += int(not ok)
+            ts[i] = sess.sssp(g, sssp, r) * 1e-3
+            if validate:
+                checks["plain_sssp"] += 1
+                checks["plain_sssp_failures"] += int(sess.sssp_check(g, sssp.dist, sssp.pred, r) != 0)
         log(f"[bench] {label}: bfs ms median {np.median(tb)*1e3:.3f} max {tb.max()*1e3:.3f}; "
             f"sssp ms median {np.median(ts)*1e3:.3f} max {ts.max()*1e3:.3f}")
         return tb, ts
@@ -173,6 +230,7 @@ def main():
         t0 = time.perf_counter()
         tb, ts = plain_pass(idx, "plain")
         t1 = time.perf_counter()
+        plain_pass(idx[:64], "plain, validated (untimed)", validate=True)
         me, nv = m_c[idx], n_c[idx]
         edges_total = 2.0 * me.sum()
         kern_time = tb.sum() + ts.sum()
@@ -181,6 +239,7 @@ def main():
         kern["sssp_kernel"] = {"bytes": float((24.0 * me + 16.0 * nv).mean()), "time": float(ts.mean()),
                                "total": float(ts.sum()), "units": "one root"}
         plain = None
+        sssp16 = None
         workload = (f"plain single-root BFS + delta-stepping SSSP per step (one root), Kronecker SCALE "
                     f"{args.scale} edgefactor {args.edgefactor}, roots from the Graph500 root rule")
     else:
@@ -212,6 +271,37 @@ def main():
             me[i] = m_c[grp].sum()
             nv[i] = n_c[grp].sum()
         t1 = time.perf_counter()
+        # untimed: every result of one more step is checked on the device (the
+        # BFS depths by the depth certificate, the SSSP distances and parents by
+        # the SSSP certificate), launch by launch
+        n = int(g.nlocalverts)
+        mark_v = sess.mark()
+        depth_dev = lib._worker_arena_push_array(g500.C.byref(sess.ws), n, 8, 256, g500.C.byref(g500.u64()))
+        rr = roots[rsets[0]]
+        for k in range(0, len(rr), BFS_GROUP):
+            part = rr[k:k + BFS_GROUP]
+            sess.batch(g, bb, part)
+            for i, r in enumerate(part):
+                lib.oned_graph_bfs_batch_depth(g500.C.byref(sess.ws), g500.C.byref(g), g500.C.byref(bb), i, depth_dev)
+                checks["batched_bfs"] += 1
+                checks["batched_bfs_failures"] += int(sess.depth_check(g, depth_dev, int(r)) != 0)
+        sess.release(mark_v)
+        for k in range(0, len(rr), SSSP_GROUP):
+            part = rr[k:k + SSSP_GROUP]
+            sess.sssp_batch(g, sb, part)
+            for i, r in enumerate(part):
+                checks["batched_sssp"] += 1
+                checks["batched_sssp_failures"] += int(
+                    sess.sssp_check(g, sb.dist + 4 * n * i, sb.pred + 8 * n * i, int(r)) != 0)
+        log(f"[bench] validation: {checks}")
+        # the 16-root SSSP batches of the batched workload (reported, not in value)
+        t16, m16 = [], 0.0
+        for k in range(0, len(rr), 16):
+            t16.append(sess.sssp_batch(g, sb, rr[k:k + 16]) * 1e-3)
+            m16 += float(m_c[rsets[0][k:k + 16]].sum())
+        sssp16 = {"launches": len(t16), "roots_per_launch": 16, "ms_per_launch": round(float(np.mean(t16)) * 1e3, 3),
+                  "gteps": round(m16 / sum(t16) / 1e9, 4)}
+        log(f"[bench] 16-root SSSP launches: {sssp16}")
         nb = -(-len(rsets[0]) // BFS_GROUP)
         ns = -(-len(rsets[0]) // SSSP_GROUP)
         log(f"[bench] batched: bfs ms per step {np.median(tb)*1e3:.2f} ({nb} launch); "
@@ -227,13 +317,17 @@ def main():
         plain = None
         if args.plain_roots:
             idx = list(range(min(args.plain_roots, nroots)))
-            ptb, pts = plain_pass(idx, "plain reference")
+            ptb, pts = plain_pass(idx, "plain reference", validate=True)
             pm = m_c[idx]
+            giant = pm >= 0.5 * pm.max()
             pw = 2.0 * pm.sum() / (ptb.sum() + pts.sum()) / 1e9
             plain = {"roots": len(idx), "gteps": round(pw, 4),
                      "bfs_gteps": round(pm.sum() / ptb.sum() / 1e9, 4),
                      "sssp_gteps": round(pm.sum() / pts.sum() / 1e9, 4),
                      "bfs_hm_gteps": round(hm(ptb, pm) / 1e9, 6), "sssp_hm_gteps": round(hm(pts, pm) / 1e9, 6),
+                     "giant_roots": int(giant.sum()),
+                     "bfs_hm_gteps_giant": round(hm(ptb[giant], pm[giant]) / 1e9, 4),
+                     "sssp_hm_gteps_giant": round(hm(pts[giant], pm[giant]) / 1e9, 4),
                      "bfs_ms_median": round(float(np.median(ptb)) * 1e3, 3),
                      "sssp_ms_median": round(float(np.median(pts)) * 1e3, 3)}
         workload = (f"batched: per step the {BFS_GROUP} Graph500 roots, one 64-bit-mask BFS launch and "
@@ -249,14 +343,21 @@ def main():
     def roof(name):
         k = kern[name]
         ach = k["bytes"] / k["time"] / 1e9
+        t = traffic.get(name)
         out = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-               "frac": round(ach * 1e9 / HBM_PEAK, 4), "traffic": traffic.get(name),
+               "frac": round(ach * 1e9 / HBM_PEAK, 4), "traffic": t,
                "kernel": name, "bytes_per_launch": k["bytes"], "avg_launch_ms": k["time"] * 1e3,
                "launch_covers": k["units"]}
+        if t:
+            # HBM bytes actually moved (rocprofv3 counters of the committed profile) over this run's launch time
+            out["traffic_gbs"] = round(t / k["time"] / 1e9, 2)
+            out["traffic_frac"] = round(t / k["time"] / HBM_PEAK, 4)
+            out["traffic_source"] = (f"profiles/{os.path.basename(tpath)}: FETCH_SIZE x2 + WRITE_SIZE per dispatch, "
+                                     f"{traffic.get('commit', 'commit not recorded')}")
         if name.startswith("batch") or name.startswith("sssp_batch"):
             out["note"] = ("algorithmic bytes = per-root bytes (BFS 16 m_c + 16 n_c, SSSP 24 m_c + 16 n_c) summed "
                            "over the roots of a launch; the roots share one pass over the CSR, so the figure "
-                           "can exceed what one traversal moves")
+                           "can exceed what one traversal moves (traffic_frac is the bandwidth actually used)")
         return out
 
     other = [k for k in kern if k != dom][0]
@@ -287,6 +388,8 @@ def main():
         "csr_build_ms": round(csr_ms, 3),
         "roofline": roof(dom),
         "roofline_other": roof(other),
+        "sssp16": sssp16,
+        "validation": checks,
     }
     if args.mode == "plain":
         out["bfs_hm_gteps"] = round(hm(tb, me) / 1e9, 6)
@@ -313,9 +416,10 @@ def main():
         out["serving"] = None
         out["serving_sweep"] = []
     if not args.no_cpu:
-        nthreads = min(16, len(os.sched_getaffinity(0)))
-        log(f"[bench] CPU baseline sample ({nthreads} threads)")
-        out["cpu_baseline"] = cpu_baseline(sess, g, roots, m_c, args.cpu_seconds, nthreads)
+        # the host share of one GPU: OMP_NUM_THREADS where set (16 on the GPU hosts), else the affinity mask
+        nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+        log(f"[bench] CPU baseline ({nthreads} threads, {cpu_model()}, nproc {os.cpu_count()})")
+        out["cpu_baseline"] = cpu_baseline(sess, g, roots, m_c, args.cpu_seconds, nthreads, args.cpu_full)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -104,6 +104,113 @@ __global__ __launch_bounds__(256) void edge_count_kernel(const u64 *__restrict__
     if (lane == 0 && c) atomicAdd((unsigned long long *)out, (unsigned long long)c);
 }
 
+// ------------------------------------------------------------ certificates
+// Depth certificate, one wave per vertex: d is the BFS depth array of root
+// iff d[root] = 0, every reached v != root has a neighbour at d[v] - 1, and
+// every entry of a reached u joins a reached vertex at most one level away
+// (values below -1 count as errors too).
+__global__ __launch_bounds__(256) void depth_check_kernel(const u64 *__restrict__ rs, const u32 *__restrict__ col,
+                                                          const i64 *__restrict__ depth, u64 n, u64 root,
+                                                          u64 *__restrict__ errors) {
+    const u32 lane = lane_id();
+    const u64 wave = ((u64)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const u64 nw = (u64)gridDim.x * blockDim.x / kWave;
+    u64 err = 0;
+    for (u64 u = wave; u < n; u += nw) {
+        const i64 du = depth[u];
+        if (du < -1) {
+            err += lane == 0;
+            continue;
+        }
+        if (du == -1) continue;
+        bool parent = false;
+        for (u64 k = rs[u] + lane; k < rs[u + 1]; k += kWave) {
+            const i64 dv = depth[col[k]];
+            if (dv < 0 || dv > du + 1 || dv < du - 1) ++err;
+            parent |= dv == du - 1;
+        }
+        const bool any_parent = ballot(parent) != 0;  // u is the same in every lane
+        if (lane == 0 && u != root && !any_parent) ++err;
+    }
+    err = wave_sum(err);
+    if (lane == 0 && err) atomicAdd((unsigned long long *)errors, (unsigned long long)err);
+}
+
+// SSSP certificate over the full rows, one wave per vertex: the root has
+// distance 0 and parent root; an unreached vertex has distance -1.0f and
+// parent -1; every entry (u, v, w) of a reached u has v reached with
+// d[v] <= fl(d[u] + w); every reached u != root has an in-range reached parent
+// p and an entry (u, p, w) with fl(d[p] + w) == d[u]. flat[0] counts the
+// tight parent entries that do not lower the distance (d[p] == d[u]): only
+// those can close a parent cycle.
+__global__ __launch_bounds__(256) void sssp_check_kernel(const u64 *__restrict__ rs, const u32 *__restrict__ col,
+                                                         const f32 *__restrict__ w, const f32 *__restrict__ dist,
+                                                         const i64 *__restrict__ pred, u64 n, u64 root,
+                                                         u64 *__restrict__ errors, u64 *__restrict__ flat) {
+    const u32 lane = lane_id();
+    const u64 wave = ((u64)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const u64 nw = (u64)gridDim.x * blockDim.x / kWave;
+    u64 err = 0, nflat = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (dist[root] != 0.0f || pred[root] != (i64)root)) ++err;
+    for (u64 u = wave; u < n; u += nw) {
+        const f32 du = dist[u];
+        const i64 p = pred[u];
+        if (!(du >= 0.0f)) {
+            if (lane == 0 && (__float_as_uint(du) != __float_as_uint(-1.0f) || p != -1)) ++err;
+            continue;
+        }
+        const bool has_p = u != root && p >= 0 && (u64)p < n;
+        const f32 dp = has_p ? dist[p] : -1.0f;
+        const bool p_ok = has_p && dp >= 0.0f;
+        bool tight = false;
+        for (u64 k = rs[u] + lane; k < rs[u + 1]; k += kWave) {
+            const u32 v = col[k];
+            const f32 wk = w[k];
+            const f32 dv = dist[v];
+            const f32 cand = __fadd_rn(du, wk);
+            if (!(dv >= 0.0f) || dv > cand) ++err;
+            if (p_ok && v == (u32)p && __fadd_rn(dp, wk) == du) tight = true;
+        }
+        const bool any_tight = ballot(tight) != 0;
+        if (lane == 0 && u != root) {
+            if (!p_ok || !any_tight) ++err;
+            else if (dp == du) ++nflat;
+        }
+    }
+    err = wave_sum(err);
+    nflat = wave_sum(nflat);
+    if (lane == 0 && err) atomicAdd((unsigned long long *)errors, (unsigned long long)err);
+    if (lane == 0 && nflat) atomicAdd((unsigned long long *)flat, (unsigned long long)nflat);
+}
+
+// Parent chains by pointer jumping: anc[v] starts at pred[v] (root: root; no
+// valid parent: the sentinel n) and is replaced by anc[anc[v]] until every
+// chain of up to n steps has been followed; a reached vertex whose chain does
+// not end at the root is one error.
+__global__ __launch_bounds__(256) void chain_init_kernel(const i64 *__restrict__ pred, u64 *__restrict__ anc, u64 n,
+                                                         u64 root) {
+    for (u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (u64)gridDim.x * blockDim.x) {
+        const i64 p = pred[v];
+        anc[v] = v == root ? root : (p >= 0 && (u64)p < n ? (u64)p : n);
+    }
+}
+
+__global__ __launch_bounds__(256) void chain_jump_kernel(const u64 *__restrict__ src, u64 *__restrict__ dst, u64 n) {
+    for (u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (u64)gridDim.x * blockDim.x) {
+        const u64 a = src[v];
+        dst[v] = a < n ? src[a] : n;
+    }
+}
+
+__global__ __launch_bounds__(256) void chain_count_kernel(const u64 *__restrict__ anc, const f32 *__restrict__ dist,
+                                                          u64 n, u64 root, u64 *__restrict__ errors) {
+    u64 err = 0;
+    for (u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (u64)gridDim.x * blockDim.x)
+        err += dist[v] >= 0.0f && anc[v] != root;
+    err = wave_sum(err);
+    if (lane_id() == 0 && err) atomicAdd((unsigned long long *)errors, (unsigned long long)err);
+}
+
 // Small device scratch for counters shared by the validation entry points.
 static u64 *counter_scratch() {
     Context &c = ctx();
@@ -242,6 +349,68 @@ b32 oned_graph_bfs_validate_noalloc(const Oned_Graph *g, const i64 *pred, u64 ro
     if (errors_out) *errors_out = h;
     if (h) fprintf(stderr, "Validation Error: BFS validation found %llu error(s)\n", (unsigned long long)h);
     return h == 0;
+}
+
+u64 oned_graph_bfs_depth_check(Worker_State *ws, const Oned_Graph *g, const i64 *depth, u64 root) {
+    (void)ws;
+    Context &c = ctx();
+    if (root >= g->nglobalverts) {
+        fail("BFS root %llu is outside graph vertex range [0, %llu)", (unsigned long long)root,
+             (unsigned long long)g->nglobalverts);
+        return 0;
+    }
+    u64 *errors = counter_scratch();
+    if (failed()) return 0;
+    const u64 n = g->nlocalverts;
+    G500_HIP(hipMemsetAsync(errors, 0, sizeof(u64), c.stream));
+    depth_check_kernel<<<grid_for(n * kWave, 256, (u64)c.num_cus * 32), 256, 0, c.stream>>>(g->rowstarts, g->column,
+                                                                                          depth, n, root, errors);
+    G500_HIP(hipGetLastError());
+    u64 err = 0;
+    g500_copy_to_host(&err, errors, sizeof(u64));
+    i64 dr = 0;
+    g500_copy_to_host(&dr, depth + root, sizeof(i64));
+    return err + (dr != 0);
+}
+
+u64 oned_graph_sssp_check(Worker_State *ws, const Oned_Graph *g, const f32 *dist, const i64 *pred, u64 root) {
+    Context &c = ctx();
+    if (root >= g->nglobalverts) {
+        fail("SSSP root %llu is outside graph vertex range [0, %llu)", (unsigned long long)root,
+             (unsigned long long)g->nglobalverts);
+        return 0;
+    }
+    u64 *cnt = counter_scratch();
+    if (failed()) return 0;
+    const u64 n = g->nlocalverts;
+    const hipStream_t s = c.stream;
+    G500_HIP(hipMemsetAsync(cnt, 0, 2 * sizeof(u64), s));
+    sssp_check_kernel<<<grid_for(n * kWave, 256, (u64)c.num_cus * 32), 256, 0, s>>>(
+        g->rowstarts, g->column, g->weights, dist, pred, n, root, cnt, cnt + 1);
+    G500_HIP(hipGetLastError());
+    u64 h[2] = {0, 0};
+    g500_copy_to_host(h, cnt, sizeof(h));
+    if (failed() || (h[0] == 0 && h[1] == 0)) return h[0];
+    // Errors or parent entries that keep the distance: follow the chains.
+    // (Without either, distances fall strictly along every parent entry, so
+    // every chain ends at the root.)
+    const u64 mark = ws->gpu_arena.used;
+    u64 *a = (u64 *)arena_alloc(ws, n * sizeof(u64), "chain ancestors");
+    u64 *b = (u64 *)arena_alloc(ws, n * sizeof(u64), "chain ancestors");
+    if (failed()) return h[0];
+    const unsigned gv = grid_for(n, 256, (u64)c.num_cus * 32);
+    chain_init_kernel<<<gv, 256, 0, s>>>(pred, a, n, root);
+    for (u64 span = 1; span < n + 1; span *= 2) {
+        chain_jump_kernel<<<gv, 256, 0, s>>>(a, b, n);
+        std::swap(a, b);
+    }
+    G500_HIP(hipMemsetAsync(cnt, 0, sizeof(u64), s));
+    chain_count_kernel<<<gv, 256, 0, s>>>(a, dist, n, root, cnt);
+    G500_HIP(hipGetLastError());
+    u64 chains = 0;
+    g500_copy_to_host(&chains, cnt, sizeof(u64));
+    ws->gpu_arena.used = mark;
+    return h[0] + chains;
 }
 
 i32 generate_bfs_roots(Worker_State *ws, const Oned_Graph *g, u64 seed1, u64 seed2, u64 *roots, i32 max_roots) {

@@ -122,6 +122,8 @@ _SIGS = {
     "oned_graph_edge_count": (u64, [P(WorkerState), P(OnedGraph), vp]),
     "oned_graph_bfs_validate": (i32, [P(WorkerState), P(OnedGraph), P(BfsState), u64, P(u64)]),
     "oned_graph_bfs_validate_noalloc": (i32, [P(OnedGraph), vp, u64, vp, P(u64)]),
+    "oned_graph_bfs_depth_check": (u64, [P(WorkerState), P(OnedGraph), vp, u64]),
+    "oned_graph_sssp_check": (u64, [P(WorkerState), P(OnedGraph), vp, vp, u64]),
     "get_statistics": (None, [P(f64), i32, P(f64)]),
     "oned_graph_bfs_batch_create": (BfsBatchState, [P(WorkerState), P(OnedGraph)]),
     "oned_graph_bfs_batch_run": (None, [P(WorkerState), P(OnedGraph), P(BfsBatchState), P(u64), i32]),
@@ -405,6 +407,18 @@ class Session:
         out = (f64 * 5)()
         self.lib.g500_sssp_phase_ms(C.byref(s), out)
         return list(out)
+
+    def depth_check(self, g, depth_dev, root):
+        """Violations of the BFS depth certificate (device depths)."""
+        c = self.lib.oned_graph_bfs_depth_check(C.byref(self.ws), C.byref(g), depth_dev, int(root))
+        self.check()
+        return c
+
+    def sssp_check(self, g, dist_dev, pred_dev, root):
+        """Violations of the SSSP certificate (device distances and parents)."""
+        c = self.lib.oned_graph_sssp_check(C.byref(self.ws), C.byref(g), dist_dev, pred_dev, int(root))
+        self.check()
+        return c
 
     def edge_count(self, g, pred_dev):
         c = self.lib.oned_graph_edge_count(C.byref(self.ws), C.byref(g), pred_dev)

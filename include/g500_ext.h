@@ -39,6 +39,19 @@ b32 oned_graph_bfs_validate(Worker_State *ws, const Oned_Graph *g, Bfs_State *bf
  * needs worker_init first. */
 b32 oned_graph_bfs_validate_noalloc(const Oned_Graph *g, const i64 *pred, u64 root, i64 *depth,
                                     u64 *errors_out);
+/* Device-side certificates, no counterpart in the reference (its runner
+ * validates BFS parent trees only, graph500_runner.cu:84-169); used to check
+ * batched results, which carry depths or (distance, parent) pairs. Both return
+ * the violation count (0: the result is the exact one), arrays on the device.
+ * Depths: d[root] = 0, every reached v != root has a neighbour at d[v] - 1,
+ * every entry of a reached vertex joins a reached vertex at most one level
+ * away (values below -1 are violations); this holds exactly for the BFS
+ * depths. SSSP (FP32 distances, -1.0f unreached; parents, -1 unreached, root
+ * -> root): d[v] <= fl(d[u] + w) on every entry of a reached u, a tight
+ * reached parent entry for every reached v != root, parent chains that end at
+ * the root; this holds exactly for the FP32 shortest-path distances. */
+u64 oned_graph_bfs_depth_check(Worker_State *ws, const Oned_Graph *g, const i64 *depth, u64 root);
+u64 oned_graph_sssp_check(Worker_State *ws, const Oned_Graph *g, const f32 *dist, const i64 *pred, u64 root);
 /* graph500_runner.cu:22-41: min, q1, median, q3, max, mean, stddev. */
 void get_statistics(const f64 *x, i32 n, f64 r[7]);
 

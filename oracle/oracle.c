@@ -28,6 +28,8 @@
  *     fixed point). Dijkstra f32: the certificate (o_sssp_check, independently
  *     vectorised in numpy in the tests) which fixes the FP32 fixed point
  *     uniquely; delta-stepping emulation equals Dijkstra bit for bit.
+ *   - Depth certificate (o_depth_check): zero exactly on the BFS depth array,
+ *     checked by brute force over perturbed depth arrays of small graphs.
  *   - Statistics: numpy percentiles / mean / std(ddof=1) on odd and even n.
  */
 #define _POSIX_C_SOURCE 199309L
@@ -1010,11 +1012,36 @@ uint64_t o_sssp_check(uint64_t n, const uint64_t *rs, const uint32_t *col, const
             path[len++] = (uint32_t)x;
             x = (uint64_t)p;
         }
-        for (uint64_t i = 0; i < len; ++i) state[path[i]] = verdict;
-        if (verdict == 3) err += len;
+        for (uint64_t i = 0; i < len; ++i) {
+            state[path[i]] = verdict;
+            if (verdict == 3 && dist[path[i]] >= 0.0f) ++err; /* reached vertices whose chain fails */
+        }
     }
     free(path);
     free(state);
+    return err;
+}
+
+/* Depth certificate: d is the BFS depth array (hop counts, -1 unreached) of
+ * root iff d[root] = 0, every reached v != root has a neighbour at d[v] - 1,
+ * and no entry joins a reached vertex to an unreached one or to one more than
+ * a level away (a path of length d[v] exists, and along any shortest path d
+ * grows by at most one per hop). */
+uint64_t o_depth_check(uint64_t n, const uint64_t *rs, const uint32_t *col, const int64_t *depth, uint64_t root) {
+    uint64_t err = depth[root] != 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : err)
+    for (uint64_t u = 0; u < n; ++u) {
+        int64_t du = depth[u];
+        if (du < -1) { ++err; continue; }
+        if (du == -1) continue;
+        int has_parent = u == root;
+        for (uint64_t k = rs[u]; k < rs[u + 1]; ++k) {
+            int64_t dv = depth[col[k]];
+            if (dv < 0 || dv > du + 1 || dv < du - 1) ++err;
+            if (dv == du - 1) has_parent = 1;
+        }
+        if (!has_parent) ++err;
+    }
     return err;
 }
 

@@ -113,9 +113,16 @@ void o_delta_stepping_omp(uint64_t n, const uint64_t *rs, const uint32_t *col, c
                           uint64_t root, float delta, float *dist, int64_t *pred, int nthreads);
 /* Parent certificate for FP32 SSSP output: counts violations of
  * (a) dist[v] <= fl(dist[u]+w) on every edge, (b) tight parent edge,
- * (c) acyclic parent chains reaching the root, (d) unreached consistency. */
+ * (c) acyclic parent chains reaching the root (one count per reached vertex
+ * whose chain does not), (d) unreached consistency. */
 uint64_t o_sssp_check(uint64_t n, const uint64_t *rs, const uint32_t *col, const float *w,
                       uint64_t root, const float *dist, const int64_t *pred);
+
+/* BFS depth certificate: violations of d[root] = 0, a neighbour at d[v] - 1
+ * for every reached v != root, and |d[u] - d[v]| <= 1 with both reached on
+ * every entry of a reached u (values below -1 count too). Zero exactly for
+ * the BFS depths of root. */
+uint64_t o_depth_check(uint64_t n, const uint64_t *rs, const uint32_t *col, const int64_t *depth, uint64_t root);
 
 /* ---- Statistics (graph500_runner.cu:14-41) ------------------------------ */
 /* out: min, q1, median, q3, max, mean, stddev */

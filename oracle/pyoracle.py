@@ -71,6 +71,8 @@ def lib():
         L.o_tuple_digest.restype = C.c_uint64
         L.o_csr_digest.argtypes = [C.c_uint64, u64p, u32p, f32p]
         L.o_csr_digest.restype = C.c_uint64
+        L.o_depth_check.argtypes = [C.c_uint64, u64p, u32p, i64p, C.c_uint64]
+        L.o_depth_check.restype = C.c_uint64
         L.o_degree_add.argtypes = [C.c_uint64, C.c_uint64, u64p, u64p]
         L.o_degree_add.restype = C.c_uint64
         L.o_csr_order_errors.argtypes = [C.c_uint64, u64p, u32p, f32p]
@@ -272,6 +274,10 @@ def delta_stepping_f32(g, root, delta=0.1):
 def sssp_check(g, root, dist, pred):
     return int(lib().o_sssp_check(g.n, g.rowstarts, _c(g.column, np.uint32), _c(g.weights, np.float32),
                                   root, _c(dist, np.float32), _c(pred, np.int64)))
+
+
+def depth_check(g, depth, root):
+    return int(lib().o_depth_check(g.n, g.rowstarts, _c(g.column, np.uint32), _c(depth, np.int64), root))
 
 
 def statistics(x):

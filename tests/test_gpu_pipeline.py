@@ -791,3 +791,81 @@ def test_tuple_dump_load_past_2gib(arena, oracle, tmp_path):
     arena.release(m)
     os.remove(path)
     os.remove(path + ".weights")
+
+
+# ------------------------------------------------------------- certificates
+def _dev(sess, arr):
+    import g500
+    arr = np.ascontiguousarray(arr)
+    p = sess.lib._worker_arena_push_array(C.byref(sess.ws), arr.size, arr.itemsize, 256, C.byref(g500.u64()))
+    assert p
+    sess.to_device(p, arr)
+    return p
+
+
+def test_depth_check_matches_oracle(arena, oracle):
+    """The device depth certificate counts what the oracle's counts: zero on
+    the BFS depths (plain and batched), the same count on planted mistakes."""
+    e, w, ref = oracle_graph(12)
+    g = check_csr(arena, oracle, 1 << 12, e, w)
+    roots = oracle.roots(ref)[:4]
+    bb = arena.batch_create(g)
+    arena.batch(g, bb, roots)
+    rng = np.random.default_rng(2)
+    for i, r in enumerate(roots):
+        r = int(r)
+        _, truth = oracle.bfs(ref, r)
+        cases = [truth]
+        for _ in range(12):
+            d = truth.copy()
+            vs = rng.choice(ref.n, int(rng.integers(1, 4)), replace=False)
+            d[vs] = rng.integers(-3, 8, len(vs))
+            cases.append(d)
+        for d in cases:
+            m = arena.mark()
+            assert arena.depth_check(g, _dev(arena, d), r) == oracle.depth_check(ref, d, r)
+            arena.release(m)
+        assert arena.depth_check(g, _dev(arena, arena.batch_depth(g, bb, i)), r) == 0
+
+
+def test_sssp_check_matches_oracle(arena, oracle):
+    """The device SSSP certificate counts what the oracle's counts: zero on
+    FP32 Dijkstra results, the same count on planted mistakes, parent cycles
+    through zero-gain entries included."""
+    e, w, ref = oracle_graph(12)
+    g = check_csr(arena, oracle, 1 << 12, e, w)
+    rng = np.random.default_rng(4)
+    for r in oracle.roots(ref)[:3]:
+        r = int(r)
+        d0, p0 = oracle.dijkstra_f32(ref, r)
+        reached = np.flatnonzero((d0 > 0))
+        cases = [(d0, p0)]
+        for kind in range(7):
+            d, p = d0.copy(), p0.copy()
+            v = int(rng.choice(reached))
+            if kind == 0: d[v] = np.nextafter(d[v], np.float32(0))
+            if kind == 1: d[v] = np.nextafter(d[v], np.float32(9))
+            if kind == 2: p[v] = v
+            if kind == 3: p[v] = ref.n + 3
+            if kind == 4: d[v], p[v] = -1.0, -1
+            if kind == 5: p[r] = v
+            if kind == 6: p[int(rng.choice(reached))] = v; p[v] = int(rng.choice(reached))
+            cases.append((d, p))
+        for d, p in cases:
+            m = arena.mark()
+            got = arena.sssp_check(g, _dev(arena, d), _dev(arena, p), r)
+            arena.release(m)
+            assert got == oracle.sssp_check(ref, r, d, p), r
+    # zero-weight entries: tight both ways at equal distance
+    e2 = np.array([[0, 1], [1, 2], [2, 3], [0, 3]], np.uint64)
+    w2 = np.array([0.5, 0.0, 0.25, 2.0], np.float32)
+    g2 = check_csr(arena, oracle, 4, e2, w2)
+    ref2 = oracle.csr(4, e2, w2)
+    d = np.array([0.0, 0.5, 0.5, 0.75], np.float32)
+    for p in ([0, 0, 1, 2], [0, 2, 1, 2], [0, 0, 1, 0]):
+        p = np.array(p, np.int64)
+        m = arena.mark()
+        got = arena.sssp_check(g2, _dev(arena, d), _dev(arena, p), 0)
+        arena.release(m)
+        assert got == oracle.sssp_check(ref2, 0, d, p), p
+    assert oracle.sssp_check(ref2, 0, d, np.array([0, 2, 1, 2], np.int64)) == 3  # the 1 <-> 2 cycle and 3 below it

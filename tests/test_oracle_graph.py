@@ -373,6 +373,38 @@ def test_edge_digests_agree_and_catch_mistakes(oracle, scale):
     assert oracle.tuple_digest(e2, w)[0] == d
 
 
+def test_depth_certificate_is_exact(oracle, g10):
+    """o_depth_check is zero on the BFS depths and only there: every depth
+    array over {-1..4} of a 6-vertex graph (with an isolated vertex) is
+    enumerated; on SCALE 10 the scipy depths pass and single-vertex
+    perturbations are all caught."""
+    import itertools
+    e = np.array([[0, 1], [1, 2], [2, 3], [0, 3], [3, 4], [1, 1]], np.uint64)
+    g = oracle.csr(6, e, np.full(len(e), 0.5, np.float32))
+    adj = sp.csr_matrix((np.ones(g.m), g.column.astype(np.int64), g.rowstarts.astype(np.int64)), shape=(6, 6))
+    for root in (0, 3, 5):
+        sd = cg.shortest_path(adj, unweighted=True, indices=root)
+        truth = np.where(np.isinf(sd), -1, sd).astype(np.int64)
+        passing = [d for d in itertools.product(range(-1, 5), repeat=6)
+                   if oracle.depth_check(g, np.array(d, np.int64), root) == 0]
+        assert passing == [tuple(truth)], root
+    _, _, g = g10
+    adj = sp.csr_matrix((np.ones(g.m), g.column.astype(np.int64), g.rowstarts.astype(np.int64)), shape=(g.n, g.n))
+    rng = np.random.default_rng(3)
+    for root in oracle.roots(g)[:3]:
+        root = int(root)
+        sd = cg.shortest_path(adj, unweighted=True, indices=root)
+        truth = np.where(np.isinf(sd), -1, sd).astype(np.int64)
+        assert oracle.depth_check(g, truth, root) == 0
+        for v in rng.choice(g.n, 60, replace=False):
+            for dv in (-2, -1, 0, 1, truth[v] - 1, truth[v] + 1, truth[v] + 2):
+                if dv == truth[v]:
+                    continue
+                d = truth.copy()
+                d[v] = dv
+                assert oracle.depth_check(g, d, root) > 0, (root, v, dv)
+
+
 @pytest.mark.parametrize("scale", [10, 13])
 def test_chunked_degrees_and_row_order(oracle, scale):
     """Degrees added chunk by chunk equal numpy's bincount of the non-loop
