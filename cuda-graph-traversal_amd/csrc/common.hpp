@@ -44,6 +44,9 @@ struct Context {
     int gen_blocks = 0;           // 0: one block per 256-edge tile (capped)
     int relabel = 1;              // 1: CSR builds add the degree-ordered traversal copy
     int sssp_prune = 1;           // 1: and the SSSP copy without two-hop-dominated entries
+    int sssp_screen = 0;          // 1: single-root SSSP relaxations check the settled bitmap before the probe
+    double sssp_pull = 0.05;       // single-root SSSP pulls a bucket's heavy relaxations when the bucket's light
+                                  // items reach this share of the unsettled vertices (0: always push)
 };
 
 Context &ctx();
@@ -190,6 +193,15 @@ __device__ __forceinline__ T wave_sum(T v) {
 __device__ __forceinline__ u32 wave_min(u32 v) {
     for (int o = 32; o > 0; o >>= 1) {
         const u32 y = __shfl_xor(v, o);
+        v = y < v ? y : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ u64 wave_min64(u64 v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const u32 lo = __shfl_xor((u32)v, o), hi = __shfl_xor((u32)(v >> 32), o);
+        const u64 y = ((u64)hi << 32) | lo;
         v = y < v ? y : v;
     }
     return v;

@@ -35,6 +35,16 @@
 // later candidate is >= the current bucket floor > a settled distance, so
 // relaxations into settled vertices skip the label (the bitmap is 1/64 of the
 // label array and stays in cache).
+// Pull form of a fused step, for buckets that hold a large share of the
+// unsettled graph (the first bucket of a Kronecker graph reaches millions of
+// vertices through light entries of hubs): instead of every vertex of the
+// bucket pushing its heavy entries, every unsettled vertex v looks for the
+// bucket's vertices among the heavy entries of its own row, lightest first.
+// The SSSP rows are symmetric, so these are the same relaxations. The scan of
+// v stops at the first entry with fl(lo + w) >= v's best distance so far:
+// every later candidate fl(d_u + w') has d_u >= lo and w' >= w, and rounding
+// is monotone. Rows up to kPullLane heavy entries are scanned by one lane,
+// longer ones by a wave in chunks of 64.
 // Work is mapped by row-segment length: a lane per vertex (<= 32 entries), a
 // wave per vertex (<= 1024), or chunks of a flattened hub work space spread
 // over the whole grid (longer).
@@ -48,6 +58,7 @@
 namespace g500 {
 
 constexpr u32 kSmall = 32;
+constexpr u64 kPullLane = 32;
 constexpr u32 kHub = 1024;
 constexpr u32 kStepLog = 1024;
 
@@ -63,7 +74,8 @@ struct SsspCtl {
     CounterLine<u64> hhub[3];      // heavy hub-tier counters [slot]
     alignas(256) u32 nmin[3];      // smallest distance bits >= hi, per slot
     u32 nnext[3];                  // 1: a label entered [hi, hi + delta) (the bucket is not empty), per slot
-    alignas(256) u64 buckets, light_iters, improvements, scans;
+    alignas(256) u64 nset[3];      // vertices settled by a fused step, per slot
+    alignas(256) u64 buckets, light_iters, improvements, scans, pulls;
     u64 ticks[4];                  // constant-clock ticks of block 0: light, fused, min+collect, kernel
     // per-step log (block 0): kind (0 light, 1 fused, 2 min, 3 collect), queue
     // entries of tiers 0/1, packed hub counter, constant-clock ticks
@@ -106,11 +118,14 @@ struct SsspArgs {
     u32 *stamp;
     u64 *settled;    // bit per vertex: distance final (its bucket is done)
     u64 *reached;    // bit per vertex: label finite (set on the first relaxation)
+    u64 nact;        // vertices that can have entries: [0, nact) (the degree order puts isolated ones last)
+    f32 pull_ratio;  // pull a bucket's heavy relaxations when its light items >= pull_ratio * unsettled (0: never)
     SsspCtl *ctl;
     i64 *pred;
     f32 *dist;
     u64 root;
     f32 delta;
+    u32 screen;      // 1: relaxations skip targets marked in `settled` before probing them
 };
 
 __device__ __forceinline__ int tier_of(u32 len) { return len <= kSmall ? 0 : (len <= kHub ? 1 : 2); }
@@ -247,8 +262,10 @@ __device__ __forceinline__ void visit_edges(const SsspArgs &a, const Target &t, 
         cand[q] = du[q] + a.w[kk];
     }
     // a settled vertex has d_v < floor <= cand: skip its label
+    if (a.screen) {
 #pragma unroll
-    for (int q = 0; q < kQ; ++q) live[q] = live[q] && !((a.settled[v[q] >> 6] >> (v[q] & 63)) & 1);
+        for (int q = 0; q < kQ; ++q) live[q] = live[q] && !((a.settled[v[q] >> 6] >> (v[q] & 63)) & 1);
+    }
     // a plain read may be stale, i.e. larger: it only lets an atomic through
     u32 old[kQ];
 #pragma unroll
@@ -461,6 +478,110 @@ __device__ __forceinline__ void scan_open(const SsspArgs &a, F &&f) {
     }
 }
 
+// Pull form of fused step x for the bucket [lo, hi): settles the bucket and
+// lowers every other unsettled vertex to its best candidate through a heavy
+// entry from the bucket; vertices ending in [hi, hi2) are queued for the next
+// bucket. Two phases: a lane per vertex (rows of up to kPullLane heavy
+// entries; longer rows go to the wave list), then a wave per listed vertex.
+// Counters: settled vertices into nset[(x + 1) % 3], the wave list in
+// hlen[(x + 1) % 3]. Ends after the second barrier.
+__device__ __forceinline__ void pull_step(const SsspArgs &a, u32 x, f32 lo, f32 hi, f32 hi2, u64 &improved,
+                                       Stage &sq) {
+    SsspCtl *ctl = a.ctl;
+    const u32 lane = lane_id();
+    const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
+    const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
+    const int hs = (int)((x + 1) % 3);
+    const Target t{(int)((x + 1) & 1), hs, x + 2, hi2, hi};
+    sq.buf = t.buf;
+    sq.slot = t.slot;
+    sq.hs = hs;
+    const u32 lo_b = __float_as_uint(lo), hi_b = __float_as_uint(hi), hi2_b = __float_as_uint(hi2);
+    u64 nset = 0;
+    for (u64 v0 = wave_gid * kWave; v0 < a.nact; v0 += nwaves * kWave) {  // the wave owns the word of v0
+        const u64 v = v0 + lane, word = v0 >> 6;
+        const u64 sw = a.settled[word];
+        const bool open = v < a.nact && !((sw >> lane) & 1);
+        const u32 lb = open ? a.dl[v] : kInfBits;
+        const bool mine = open && lb >= lo_b && lb < hi_b;
+        const u64 sm = ballot(mine);
+        if (sm && lane == 0) {
+            atomicOr((unsigned long long *)&a.settled[word], (unsigned long long)sm);
+            nset += (u64)__popcll(sm);
+        }
+        const bool act = open && !mine;
+        u64 b = 0, e = 0;
+        if (act) {
+            b = a.rs[v] + a.split[v];
+            e = a.rs[v + 1];
+        }
+        const bool longrow = act && e - b > kPullLane;
+        stage_push<2>(a, sq, longrow, (u32)v);
+        u32 best = lb, bu = 0;
+        if (act && !longrow) {
+            for (u64 k = b; k < e; ++k) {
+                const f32 wk = a.w[k];
+                if (__float_as_uint(__fadd_rn(lo, wk)) >= best) break;
+                const u32 u = a.col[k];
+                const u32 du = a.dl[u];
+                if (du >= lo_b && du < hi_b) {
+                    const u32 c = __float_as_uint(__fadd_rn(__uint_as_float(du), wk));
+                    if (c < best) {
+                        best = c;
+                        bu = u;
+                    }
+                }
+            }
+        }
+        const bool imp = best < lb;
+        if (imp) {
+            a.dl[v] = best;
+            a.lab[v] = ((u64)best << 32) | bu;
+            ++improved;
+        }
+        const u64 rm = ballot(imp && lb >= kInfBits);
+        if (rm && lane == 0) atomicOr((unsigned long long *)&a.reached[word], (unsigned long long)rm);
+        push_light(a, act && !longrow && best >= hi_b && best < hi2_b, (u32)v, t, sq);
+    }
+    stage_flush(a, sq);
+    if (lane == 0 && nset) atomicAdd((unsigned long long *)&ctl->nset[hs], (unsigned long long)nset);
+    sync_traced(ctl, kPhaseFused, lo_b, x);
+    // long rows: a wave per vertex, 64 entries per chunk, lightest chunk first
+    const u32 nlong = min(ld_relaxed(&ctl->hlen[hs].v), (u32)a.n);
+    for (u64 i = wave_gid; i < nlong; i += nwaves) {
+        const u32 v = a.hq1[i];
+        if (v >= a.n) {
+            if (lane == 0) atomicOr(&ctl->bar.error, 4u);
+            continue;
+        }
+        const u32 lb = a.dl[v];
+        const u64 b = a.rs[v] + a.split[v], e = a.rs[v + 1];
+        u64 best = ((u64)lb << 32) | 0xFFFFFFFFull;  // (distance bits, parent)
+        for (u64 k0 = b; k0 < e; k0 += kWave) {
+            if (__float_as_uint(__fadd_rn(lo, a.w[k0])) >= (u32)(best >> 32)) break;
+            const u64 k = k0 + lane;
+            u64 c = ~0ull;
+            if (k < e) {
+                const f32 wk = a.w[k];
+                const u32 u = a.col[k];
+                const u32 du = a.dl[u];
+                if (du >= lo_b && du < hi_b) c = ((u64)__float_as_uint(__fadd_rn(__uint_as_float(du), wk)) << 32) | u;
+            }
+            best = wave_min64(best < c ? best : c);
+        }
+        const u32 bb = (u32)(best >> 32);
+        if (lane == 0 && bb < lb) {
+            a.dl[v] = bb;
+            a.lab[v] = best;
+            ++improved;
+            if (lb >= kInfBits) atomicOr((unsigned long long *)&a.reached[v >> 6], 1ull << (v & 63));
+        }
+        push_light(a, lane == 0 && bb >= hi_b && bb < hi2_b, v, t, sq);
+    }
+    stage_flush(a, sq);
+    sync_traced(ctl, kPhaseHeavy, lo_b, x);
+}
+
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void sssp_kernel(SsspArgs a) {
     SsspCtl *ctl = a.ctl;
     const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
@@ -479,6 +600,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
         sq.slot = t.slot;
     };
     u64 improved = 0;
+    u64 bucket_items = 0;            // light-queue items of the current bucket (the same in every thread)
+    u64 settled_total = 0;           // vertices settled so far (the same in every thread)
     u32 x = 0;                       // step
     f32 lo = 0.0f, hi = next_bound(0.0f, a.delta);
 
@@ -510,6 +633,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
             st_relaxed(&ctl->hhub[s].v, 0ull);
             st_relaxed(&ctl->nmin[s], 0xFFFFFFFFu);
             st_relaxed(&ctl->nnext[s], 0u);
+            st_relaxed(&ctl->nset[s], 0ull);
         }
     };
     // Light queue of step `step`: counts per tier and the packed hub counter.
@@ -554,6 +678,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
             const u64 total = queue_size(x, cnt, &hub);
             clear_ahead(x);
             if (total == 0 || ld_relaxed(&ctl->bar.error)) break;
+            bucket_items += total;
             if (gtid == 0) atomicAdd((unsigned long long *)&ctl->light_iters, 1ull);
             log_step(0, cnt[0], cnt[1], hub);
             const Target t = target(x, hi, lo);
@@ -573,11 +698,19 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
         if (gtid == 0) atomicAdd((unsigned long long *)&ctl->buckets, 1ull);
         log_step(1, 0, 0, 0);
         const f32 hi2 = next_bound(hi, a.delta);
-        {
+        const u64 unsettled = a.nact > settled_total ? a.nact - settled_total : 0;
+        if (a.pull_ratio > 0.f && (f32)bucket_items >= a.pull_ratio * (f32)unsettled) {
+            pull_step(a, x, lo, hi, hi2, improved, sq);
+            settled_total += ld_relaxed(&ctl->nset[(x + 1) % 3]);
+            if (gtid == 0) atomicAdd((unsigned long long *)&ctl->pulls, 1ull);
+            tick(1);
+            ++x;
+        } else {
             const Target t = target(x, hi2, hi);
             const int hs = (int)((x + 1) % 3);
             stage_to(t);
             sq.hs = hs;
+            u64 nset_local = 0;
             scan_open(a, [&](u64 v, u32 lbits) {
                 bool mine = false, next = false;
                 u64 b = 0, e = 0;
@@ -596,7 +729,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                 push_light(a, next, (u32)v, t, sq);
                 // the bucket's vertices are final now: the wave owns the word of v
                 const u64 sm = ballot(mine);
-                if (sm && lane == 0) a.settled[v >> 6] |= sm;
+                if (sm && lane == 0) {
+                    a.settled[v >> 6] |= sm;
+                    nset_local += (u64)__popcll(sm);
+                }
                 const u32 len = (u32)(e - b);
                 const bool medium = mine && len > kSmall && len <= kHub;
                 stage_push<2>(a, sq, medium, (u32)v);
@@ -605,7 +741,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                 walk_segments(a, t, (u32)v, dv, b, mine && len <= kSmall ? len : 0, improved, slot, sq);
             });
             stage_flush(a, sq);
+            if (nset_local) atomicAdd((unsigned long long *)&ctl->nset[hs], (unsigned long long)nset_local);
             sync_traced(ctl, kPhaseFused, __float_as_uint(lo), x);
+            settled_total += ld_relaxed(&ctl->nset[hs]);
             const u32 nmed = ld_relaxed(&ctl->hlen[hs].v);
             const u64 hh = ld_relaxed(&ctl->hhub[hs].v);
             if (nmed || hub_count(hh)) {  // second phase of the same step: longer heavy rows
@@ -625,6 +763,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
             if (queue_size(x, cnt, &hub) || ld_relaxed(&ctl->nnext[x % 3])) {
                 lo = hi;
                 hi = hi2;
+                bucket_items = 0;
                 continue;
             }
         }
@@ -650,6 +789,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
             break;
         }
         const f32 dmin = __uint_as_float(nm);
+        bucket_items = 0;
         lo = hi;
         hi = next_bound(lo, a.delta);
         while (dmin >= hi) {
@@ -814,6 +954,10 @@ void oned_graph_sssp_run(Worker_State *ws, const Oned_Graph *g, Sssp_State *s, u
     a.hq_len = s->hub_lengths + 2 * n;
     a.stamp = s->stamp;
     a.settled = s->settled;
+    // with the degree-ordered copy, the isolated vertices are the last ids
+    a.nact = (g->perm && g->notisolated) ? MIN(g->notisolated, n) : n;
+    a.pull_ratio = (f32)c.sssp_pull;
+    a.screen = (u32)ctx().sssp_screen;
     a.reached = s->reached;
     a.ctl = (SsspCtl *)s->ctl;
     a.pred = s->pred;
