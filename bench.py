@@ -132,10 +132,10 @@ def main():
     ap.add_argument("--cpu-full", action="store_true",
                     help="the full CPU baseline: OpenMP BFS from every giant root, delta-stepping from 8, "
                          "serial BFS and Dijkstra from 2 (minutes at SCALE 26)")
-    ap.add_argument("--serve-requests", type=int, default=512,
+    ap.add_argument("--serve-requests", type=int, default=2048,
                     help="requests of the serving leg (0: skip); resident graphs S16-S24")
     ap.add_argument("--serve-rate", type=float, default=1000.0, help="Poisson arrivals per second")
-    ap.add_argument("--serve-sweep", default="300,5000",
+    ap.add_argument("--serve-sweep", default="300,5000,20000,50000",
                     help="further offered rates (comma-separated, empty for none) reported in serving_sweep")
     ap.add_argument("--calibrate-mib", type=int, default=0,
                     help="also run the device copy kernel on this many MiB (counter calibration)")
@@ -408,8 +408,8 @@ def main():
             m = sess.mark()  # each run's server states are released before the next
             r = serving.poisson_run(sess, res, rate, args.serve_requests)
             sess.release(m)
-            sweep.append({k: r[k] for k in ("rate_offered", "completed", "p50_ms", "p99_ms", "req_per_s", "gteps",
-                                            "mean_batch")})
+            sweep.append({k: r[k] for k in ("rate_offered", "completed", "errors", "p50_ms", "p99_ms", "req_per_s",
+                                            "gteps", "mean_batch")})
             log(f"[bench] serving at {rate:g} req/s: {sweep[-1]}")
         out["serving_sweep"] = sweep
     else:

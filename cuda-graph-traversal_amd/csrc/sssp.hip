@@ -98,6 +98,19 @@ __device__ __forceinline__ void sync_traced(SsspCtl *ctl, u32 phase, u32 a, u32 
     }
 }
 
+// A row's bounds in one 16-byte record (one memory transaction per item):
+// row start, light entries (w < delta), row length.
+struct RowRec {
+    u64 start;
+    u32 light, len;
+};
+static_assert(sizeof(RowRec) == 16, "one 16-byte record per row");
+
+__device__ __forceinline__ RowRec load_rec(const RowRec *r, u32 v) {
+    const uint4 x = *reinterpret_cast<const uint4 *>(r + v);
+    return RowRec{((u64)x.y << 32) | x.x, x.z, x.w};
+}
+
 struct SsspArgs {
     u64 n;
     f32 dbound;  // distances at or above it set error 32 (+inf without the pruned copy)
@@ -105,6 +118,7 @@ struct SsspArgs {
     const u32 *col;
     const f32 *w;
     const u32 *split;
+    const RowRec *rows;       // [n] row records for the same split
     const u32 *perm, *iperm;  // traversal <-> original ids (null: identity)
     u64 *lab;
     u32 *dl;         // probe: distance bits per vertex (aliases the dist output)
@@ -300,9 +314,9 @@ __device__ __forceinline__ void visit_edges(const SsspArgs &a, const Target &t, 
 // Segment of u's row: the light prefix or the heavy suffix.
 template <bool LIGHT>
 __device__ __forceinline__ void segment(const SsspArgs &a, u32 u, u64 &b, u64 &e) {
-    const u64 r0 = a.rs[u], s = a.split[u];
-    b = LIGHT ? r0 : r0 + s;
-    e = LIGHT ? r0 + s : a.rs[u + 1];
+    const RowRec r = load_rec(a.rows, u);
+    b = LIGHT ? r.start : r.start + r.light;
+    e = LIGHT ? r.start + r.light : r.start + r.len;
 }
 
 // Relaxes the concatenation of the wave's 64 segments (lane j: entries
@@ -511,10 +525,7 @@ __device__ __forceinline__ void pull_step(const SsspArgs &a, u32 x, f32 lo, f32 
         }
         const bool act = open && !mine;
         u64 b = 0, e = 0;
-        if (act) {
-            b = a.rs[v] + a.split[v];
-            e = a.rs[v + 1];
-        }
+        if (act) segment<false>(a, (u32)v, b, e);
         const bool longrow = act && e - b > kPullLane;
         stage_push<2>(a, sq, longrow, (u32)v);
         u32 best = lb, bu = 0;
@@ -555,7 +566,8 @@ __device__ __forceinline__ void pull_step(const SsspArgs &a, u32 x, f32 lo, f32 
             continue;
         }
         const u32 lb = a.dl[v];
-        const u64 b = a.rs[v] + a.split[v], e = a.rs[v + 1];
+        u64 b, e;
+        segment<false>(a, v, b, e);
         u64 best = ((u64)lb << 32) | 0xFFFFFFFFull;  // (distance bits, parent)
         for (u64 k0 = b; k0 < e; k0 += kWave) {
             if (__float_as_uint(__fadd_rn(lo, a.w[k0])) >= (u32)(best >> 32)) break;
@@ -852,6 +864,12 @@ __global__ __launch_bounds__(256) void sssp_split_kernel(const u64 *__restrict__
     }
 }
 
+__global__ __launch_bounds__(256) void sssp_rows_kernel(const u64 *__restrict__ rs, const u32 *__restrict__ split,
+                                                        u64 n, RowRec *__restrict__ rows) {
+    for (u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (u64)gridDim.x * blockDim.x)
+        rows[v] = RowRec{rs[v], split[v], (u32)(rs[v + 1] - rs[v])};
+}
+
 __global__ __launch_bounds__(256) void sssp_clear_kernel(u64 *__restrict__ lab, u32 *__restrict__ dl,
                                                          u32 *__restrict__ stamp, u64 *__restrict__ settled,
                                                          u64 *__restrict__ reached, u64 n) {
@@ -893,6 +911,7 @@ Sssp_State oned_graph_sssp_create(Worker_State *ws, const Oned_Graph *g) {
     s.hub_offsets = (u64 *)arena_alloc(ws, 3 * n * sizeof(u64), "SSSP hub offsets");
     s.hub_lengths = (u32 *)arena_alloc(ws, 3 * n * sizeof(u32), "SSSP hub lengths");
     s.split = (u32 *)arena_alloc(ws, n * sizeof(u32), "SSSP light split");
+    s.rows = arena_alloc(ws, n * sizeof(RowRec), "SSSP row records");
     s.stamp = (u32 *)arena_alloc(ws, n * sizeof(u32), "SSSP stamps");
     s.settled = (u64 *)arena_alloc(ws, (n + 63) / 64 * sizeof(u64), "SSSP settled bitmap");
     s.reached = (u64 *)arena_alloc(ws, (n + 63) / 64 * sizeof(u64), "SSSP reached bitmap");
@@ -930,6 +949,9 @@ void oned_graph_sssp_run(Worker_State *ws, const Oned_Graph *g, Sssp_State *s, u
     const TraversalView tv = sssp_view(g);
     if (s->split_delta != s->delta || s->split_graph != (const void *)tv.rs) {  // light/heavy split of every row
         compute_split(tv.rs, tv.w, n, s->delta, s->split);
+        sssp_rows_kernel<<<grid_for(n, 256, (u64)c.num_cus * 32), 256, 0, c.stream>>>(tv.rs, s->split, n,
+                                                                                     (RowRec *)s->rows);
+        G500_HIP(hipGetLastError());
         s->split_delta = s->delta;
         s->split_graph = (const void *)tv.rs;
     }
@@ -941,6 +963,7 @@ void oned_graph_sssp_run(Worker_State *ws, const Oned_Graph *g, Sssp_State *s, u
     a.perm = tv.perm;
     a.iperm = tv.iperm;
     a.split = s->split;
+    a.rows = (const RowRec *)s->rows;
     a.lab = s->dist_pred;
     a.dl = (u32 *)s->dist;
     for (int b = 0; b < 2; ++b) {

@@ -52,7 +52,8 @@ class BfsState(C.Structure):
 
 class SsspState(C.Structure):
     _fields_ = [("pred", vp), ("dist", vp), ("dist_pred", vp), ("frontier", vp), ("heavy", vp),
-                ("hub_offsets", vp), ("hub_lengths", vp), ("split", vp), ("stamp", vp), ("settled", vp), ("reached", vp),
+                ("hub_offsets", vp), ("hub_lengths", vp), ("split", vp), ("rows", vp), ("stamp", vp), ("settled", vp),
+                ("reached", vp),
                 ("ctl", vp), ("split_graph", vp), ("delta", f32), ("split_delta", f32), ("max_nlocalverts", u64)]
 
 

@@ -41,7 +41,7 @@ def load_graphs(sess, spec=GRAPHS, edgefactor=16):
     return [Resident(sess, sc, s1, s2, edgefactor) for sc, s1, s2 in spec]
 
 
-def poisson_run(sess, graphs, rate, count, slots=2, delta=0.2, seed=12345):
+def poisson_run(sess, graphs, rate, count, slots=4, delta=0.2, seed=12345):
     """Submits `count` requests at `rate` per second; returns latency and
     throughput figures of the completed requests."""
     rng = np.random.default_rng(seed)

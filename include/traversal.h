@@ -94,6 +94,8 @@ struct Sssp_State {
     u64 *hub_offsets;   /* [3n] hub-tier work offsets (2 light queues, 1 heavy list) */
     u32 *hub_lengths;   /* [3n] hub-tier row-segment lengths */
     u32 *split;         /* [n] number of light entries (w < split_delta) per row */
+    void *rows;         /* [n] 16-byte row records for the same split: row start (u64), light
+                           entries (u32), row length (u32) */
     u32 *stamp;         /* [n] light-queue de-duplication */
     u64 *settled;       /* [ceil(n/64)] vertices whose distance is final */
     u64 *reached;       /* [ceil(n/64)] vertices with a finite label */
