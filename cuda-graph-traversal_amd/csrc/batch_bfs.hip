@@ -534,7 +534,7 @@ bool bfs_batch_enqueue(const Oned_Graph *g, Bfs_Batch_State *b, const u64 *roots
     a.beta = (f32)c.batch_beta;
     a.hub_degree = (u32)c.bfs_hub_degree;
     a.heavy_degree = (u32)c.batch_heavy_degree;
-    const int block = ctx().bfs_block;
+    const int block = ctx().batch_bfs_block;
     if (grid <= 0) grid = cooperative_blocks((const void *)batch_bfs_kernel, block, 0);
     if (failed()) return false;
     void *args[] = {&a};
@@ -568,7 +568,7 @@ u32 bfs_batch_error(const Bfs_Batch_State *b, hipStream_t stream) {
     return e;
 }
 
-int bfs_batch_grid() { return cooperative_blocks((const void *)batch_bfs_kernel, ctx().bfs_block, 0); }
+int bfs_batch_grid() { return cooperative_blocks((const void *)batch_bfs_kernel, ctx().batch_bfs_block, 0); }
 
 // Per-root summary of a finished batch: reached vertices and depth sum.
 __global__ __launch_bounds__(256) void batch_summary_kernel(const u64 *__restrict__ lm,

@@ -32,14 +32,16 @@ struct Context {
     const Mrg_Transition_Matrix *d_mrg_table = nullptr;  // [24][256] on the device
     u64 *d_counters = nullptr;    // 64 bytes of device counters for the validation entry points
     // tuning knobs
-    double bfs_alpha = 15.0;      // top-down -> bottom-up when m_frontier > m_unexplored / alpha
-    double bfs_beta = 18.0;       // bottom-up -> top-down when n_frontier < n / beta
+    double bfs_alpha = 60.0;      // top-down -> bottom-up when m_frontier > m_unexplored / alpha
+    double bfs_beta = 60.0;       // bottom-up -> top-down when n_frontier < n / beta
     int bfs_block = 512;          // threads per block of the persistent BFS kernel
     int bfs_hub_degree = 2048;    // frontier vertices above this degree are split grid-wide
     double batch_alpha = 15.0;    // the same two thresholds for the batched multi-root BFS
     double batch_beta = 18.0;
     int batch_heavy_degree = 1 << 20; // batched bottom-up: vertices above this degree are split grid-wide
     int sssp_block = 512;
+    int batch_bfs_block = 512;    // threads per block of the batched BFS and batched SSSP kernels
+    int batch_sssp_block = 512;
     int coop_launch = 1;          // 1: cooperative launch first; 0: plain launch of the co-resident grid
     int gen_blocks = 0;           // 0: one block per 256-edge tile (capped)
     int relabel = 1;              // 1: CSR builds add the degree-ordered traversal copy

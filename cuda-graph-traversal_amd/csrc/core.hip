@@ -299,11 +299,12 @@ i32 g500_set_param(const char *name, f64 value) {
     else if (n == "batch_alpha") c.batch_alpha = value;
     else if (n == "batch_beta") c.batch_beta = value;
     else if (n == "batch_heavy_degree") c.batch_heavy_degree = (int)value;
-    else if (n == "bfs_block" || n == "sssp_block") {
+    else if (n == "bfs_block" || n == "sssp_block" || n == "batch_bfs_block" || n == "batch_sssp_block") {
         // persistent kernels are built for at most 512 threads per block (wave multiples)
         const int v = (int)value;
         if (v != 128 && v != 256 && v != 512) return -1;
-        (n == "bfs_block" ? c.bfs_block : c.sssp_block) = v;
+        (n == "bfs_block" ? c.bfs_block : n == "sssp_block" ? c.sssp_block
+                                        : n == "batch_bfs_block" ? c.batch_bfs_block : c.batch_sssp_block) = v;
     }
     else if (n == "bfs_hub_degree") c.bfs_hub_degree = (int)value;
     else if (n == "gen_blocks") c.gen_blocks = (int)value;
@@ -327,6 +328,8 @@ i32 g500_get_param(const char *name, f64 *value) {
     else if (n == "bfs_block") *value = c.bfs_block;
     else if (n == "bfs_hub_degree") *value = c.bfs_hub_degree;
     else if (n == "sssp_block") *value = c.sssp_block;
+    else if (n == "batch_bfs_block") *value = c.batch_bfs_block;
+    else if (n == "batch_sssp_block") *value = c.batch_sssp_block;
     else if (n == "gen_blocks") *value = c.gen_blocks;
     else if (n == "coop_launch") *value = c.coop_launch;
     else if (n == "relabel") *value = c.relabel;

@@ -1325,7 +1325,7 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
     a.dw[0] = wmid;
     a.dw[1] = wlong;
     a.dw[2] = wfar;
-    const int block = ctx().sssp_block;
+    const int block = ctx().batch_sssp_block;
     const size_t dyn = q_lds_bytes(block);
     if (grid <= 0) grid = cooperative_blocks((const void *)sssp_batch_kernel, block, dyn);
     if (failed()) return false;
@@ -1350,7 +1350,7 @@ u32 sssp_batch_error(const Sssp_Batch_State *s, hipStream_t stream) {
 }
 
 int sssp_batch_grid() {
-    const int block = ctx().sssp_block;
+    const int block = ctx().batch_sssp_block;
     return cooperative_blocks((const void *)sssp_batch_kernel, block, q_lds_bytes(block));
 }
 

@@ -607,10 +607,11 @@ def test_block_shapes(arena, oracle, block):
     """Persistent kernels with smaller blocks (more blocks in the grid barrier)."""
     e, w, ref = oracle_graph(13)
     g = check_csr(arena, oracle, 1 << 13, e, w)
-    old = {k: arena.get_param(k) for k in ("bfs_block", "sssp_block")}
+    knobs = ("bfs_block", "sssp_block", "batch_bfs_block", "batch_sssp_block")
+    old = {k: arena.get_param(k) for k in knobs}
     try:
-        arena.set_param("bfs_block", block)
-        arena.set_param("sssp_block", block)
+        for k in knobs:
+            arena.set_param(k, block)
         arena.bfs_state = arena.bfs_create(g)
         s = arena.sssp_create(g)
         roots = oracle.roots(ref)
@@ -634,13 +635,15 @@ def test_autotuner_applies_candidates(arena, oracle):
     from g500 import autotune
     e, w, ref = oracle_graph(14)
     g = check_csr(arena, oracle, 1 << 14, e, w)
-    keys = ("bfs_alpha", "bfs_beta", "bfs_block", "sssp_block", "batch_alpha", "batch_heavy_degree")
+    keys = ("bfs_alpha", "bfs_beta", "bfs_block", "sssp_block", "sssp_pull", "batch_alpha", "batch_heavy_degree",
+            "batch_bfs_block", "batch_sssp_block")
     old = {k: arena.get_param(k) for k in keys}
     try:
         out = autotune.tune(arena, g, batch=True)
         p = out["params"]
         assert (p["bfs_alpha"], p["bfs_beta"]) in autotune.BFS_SWITCH and p["bfs_block"] in autotune.BLOCKS
-        assert p["sssp_delta"] in autotune.SSSP_DELTA
+        assert p["sssp_delta"] in autotune.SSSP_DELTA and p["sssp_pull"] in autotune.SSSP_PULL
+        assert p["batch_bfs_block"] in autotune.BLOCKS and p["batch_sssp_block"] in autotune.BLOCKS
         assert (p["batch_sssp_delta"], p["batch_sssp_mid_weight"], p["batch_sssp_long_weight"],
                 p["batch_sssp_far_weight"]) in autotune.BATCH_SSSP
         assert p["batch_alpha"] in autotune.BATCH_ALPHA and p["batch_heavy_degree"] in autotune.BATCH_HEAVY
