@@ -76,6 +76,11 @@ void release_library_resources();
 // the level loop uses. Blocking.
 void depth_from_pred(const Oned_Graph *g, const i64 *pred, i64 *dist, u64 root, u32 *changed);
 
+// Distances (f32, -1.0f unreached, original ids) from the graph's highest-degree
+// vertex into dist_dev[n], by the single-root SSSP on temporary arena memory.
+// Blocking; false after a failure.
+bool hub_distances(Worker_State *ws, const Oned_Graph *g, f32 *dist_dev);
+
 // Resolves, for the current device, the device address of the static scratch
 // array of the no-allocation validation entry point (once per device).
 void resolve_noalloc_scratch();

@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <array>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -147,6 +148,9 @@ struct G500_Server {
     // per graph: SSSP row splits and row records for batches of up to 16 roots
     // (delta) and of more (delta / 2)
     std::vector<std::array<Sssp_Batch_State, 2>> splits;
+    // per graph: distance of every vertex from the graph's highest-degree
+    // vertex (host copy), the key that groups SSSP requests into launches
+    std::vector<std::vector<f32>> hubd;
     u64 launches = 0, launched_requests = 0;
 };
 
@@ -162,18 +166,27 @@ u64 footprint1(const G500_Server *s, const Request *r) {
 
 // Takes the next batch under the lock: the oldest request fixes (graph, kind);
 // later requests of the pair join while the launch has room and the pool
-// admits their footprint.
+// admits their footprint. When more SSSP requests of the pair wait than a
+// launch holds, those whose roots lie closest to the oldest one's in hub
+// distance are taken (the oldest first): roots at similar hub distances share
+// more work in one launch.
 std::vector<Request *> take_batch(G500_Server *s) {
     std::vector<Request *> batch;
     if (s->pending.empty()) return batch;
     const i32 graph = s->pending.front()->graph, kind = s->pending.front()->kind;
     const size_t cap = kind == G500_BFS ? 64 : kSsspBatchRoots;
-    for (auto it = s->pending.begin(); it != s->pending.end() && batch.size() < cap;) {
-        Request *r = *it;
-        if (r->graph != graph || r->kind != kind) {
-            ++it;
-            continue;
-        }
+    std::vector<Request *> cand;
+    for (Request *r : s->pending)
+        if (r->graph == graph && r->kind == kind) cand.push_back(r);
+    if (kind == G500_SSSP && cand.size() > cap && !s->hubd[graph].empty()) {
+        const std::vector<f32> &hd = s->hubd[graph];
+        const f32 d0 = hd[cand[0]->root];
+        std::stable_sort(cand.begin() + 1, cand.end(), [&](const Request *x, const Request *y) {
+            return std::fabs(hd[x->root] - d0) < std::fabs(hd[y->root] - d0);
+        });
+    }
+    for (Request *r : cand) {
+        if (batch.size() == cap) break;
         r->bytes0 = r->out0 ? footprint0(s, r) : 0;
         r->bytes1 = footprint1(s, r);
         r->off0 = r->bytes0 ? s->pool.alloc(r->bytes0) : ~0ull;
@@ -184,7 +197,14 @@ std::vector<Request *> take_batch(G500_Server *s) {
             break;
         }
         batch.push_back(r);
-        it = s->pending.erase(it);
+    }
+    if (!batch.empty()) {
+        std::deque<Request *> rest;
+        std::vector<Request *> taken(batch);
+        std::sort(taken.begin(), taken.end());
+        for (Request *r : s->pending)
+            if (!std::binary_search(taken.begin(), taken.end(), r)) rest.push_back(r);
+        s->pending.swap(rest);
     }
     return batch;
 }
@@ -455,6 +475,19 @@ i32 g500_server_start(G500_Server *s, f32 delta) {
         }
     sync();
     if (failed()) return -1;
+    // hub distances of every graph (host copies) for grouping SSSP requests
+    s->hubd.resize(s->graphs.size());
+    for (size_t gi = 0; gi < s->graphs.size(); ++gi) {
+        const u64 n = s->graphs[gi].nlocalverts;
+        const u64 mark = s->ws->gpu_arena.used;
+        f32 *dh = (f32 *)arena_alloc(s->ws, n * sizeof(f32), "server hub distances");
+        if (!failed() && hub_distances(s->ws, &s->graphs[gi], dh)) {
+            s->hubd[gi].resize(n);
+            g500_copy_to_host(s->hubd[gi].data(), dh, n * sizeof(f32));
+        }
+        s->ws->gpu_arena.used = mark;
+        if (failed()) return -1;
+    }
     s->started = true;
     for (int k = 0; k < s->nslots; ++k) s->slots[k].th = std::thread(slot_loop, s, k);
     s->copier = std::thread(copy_loop, s);

@@ -53,6 +53,9 @@
 // (slot x % 3), writes only slot (x + 1) % 3 and clears slot (x + 2) % 3, so
 // a block acting on a decision never changes a counter that a slower block
 // still has to read for the same decision.
+#include <algorithm>
+#include <vector>
+
 #include "sssp_common.hpp"
 
 namespace g500 {
@@ -891,6 +894,48 @@ void compute_split(const u64 *rs, const f32 *w, u64 n, f32 delta, u32 *split, hi
     G500_HIP(hipGetLastError());
 }
 
+// Highest degree of the graph's rows packed with the smallest id reaching it:
+// (degree << 32) | (~id), maximised.
+__global__ __launch_bounds__(256) void hub_kernel(const u64 *__restrict__ rs, u64 n, u64 *__restrict__ best) {
+    u64 m = 0;
+    for (u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (u64)gridDim.x * blockDim.x)
+        m = max(m, (MIN(rs[v + 1] - rs[v], (u64)0xFFFFFFFFu) << 32) | (0xFFFFFFFFu - (u32)v));
+    for (int o = 32; o > 0; o >>= 1) {
+        const u32 lo = __shfl_xor((u32)m, o), hi = __shfl_xor((u32)(m >> 32), o);
+        m = max(m, ((u64)hi << 32) | lo);
+    }
+    if (lane_id() == 0 && m) atomicMax((unsigned long long *)best, (unsigned long long)m);
+}
+
+__global__ __launch_bounds__(256) void gather_dist_kernel(const f32 *__restrict__ dist, const u64 *__restrict__ ids,
+                                                          u64 count, f32 *__restrict__ out) {
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (u64)gridDim.x * blockDim.x)
+        out[i] = dist[ids[i]];
+}
+
+bool hub_distances(Worker_State *ws, const Oned_Graph *g, f32 *dist_dev) {
+    Context &c = ctx();
+    const u64 n = g->nlocalverts;
+    const u64 mark = ws->gpu_arena.used;
+    u64 *best = (u64 *)arena_alloc(ws, sizeof(u64), "hub search");
+    if (failed()) return false;
+    G500_HIP(hipMemsetAsync(best, 0, sizeof(u64), c.stream));
+    hub_kernel<<<grid_for(n, 256, (u64)c.num_cus * 16), 256, 0, c.stream>>>(g->rowstarts, n, best);
+    G500_HIP(hipGetLastError());
+    u64 h = 0;
+    g500_copy_to_host(&h, best, sizeof(h));
+    const u64 hub = 0xFFFFFFFFu - (u32)h;
+    Sssp_State s = oned_graph_sssp_create(ws, g);
+    if (!failed() && hub < n) {
+        s.delta = 0.05f;
+        oned_graph_sssp_run(ws, g, &s, hub);
+        G500_HIP(hipMemcpyAsync(dist_dev, s.dist, n * sizeof(f32), hipMemcpyDeviceToDevice, c.stream));
+        sync();
+    }
+    ws->gpu_arena.used = mark;
+    return !failed();
+}
+
 }  // namespace g500
 
 using namespace g500;
@@ -1046,6 +1091,38 @@ void g500_sssp_phase_ms(const Sssp_State *s, f64 *ms) {
     const f64 khz = ctx().wall_clock_khz;
     for (int i = 0; i < 4; ++i) ms[i] = (f64)h.ticks[i] / khz;
     ms[4] = (f64)h.scans;
+}
+
+i32 oned_graph_sssp_root_order(Worker_State *ws, const Oned_Graph *g, const u64 *roots, i32 nroots, i32 *order) {
+    Context &c = ctx();
+    if (nroots < 0 || (nroots && (!roots || !order))) {
+        fail("oned_graph_sssp_root_order: bad root list (%d roots)", nroots);
+        return -1;
+    }
+    for (i32 i = 0; i < nroots; ++i)
+        if (roots[i] >= g->nglobalverts) {
+            fail("SSSP root %llu is outside graph vertex range [0, %llu)", (unsigned long long)roots[i],
+                 (unsigned long long)g->nglobalverts);
+            return -1;
+        }
+    if (!nroots) return 0;
+    const u64 n = g->nlocalverts;
+    const u64 mark = ws->gpu_arena.used;
+    f32 *dh = (f32 *)arena_alloc(ws, n * sizeof(f32), "hub distances");
+    u64 *ids = (u64 *)arena_alloc(ws, (u64)nroots * sizeof(u64), "root ids");
+    f32 *out = (f32 *)arena_alloc(ws, (u64)nroots * sizeof(f32), "root hub distances");
+    std::vector<f32> d(nroots);
+    if (!failed() && hub_distances(ws, g, dh)) {
+        g500_copy_to_device(ids, roots, (u64)nroots * sizeof(u64));
+        gather_dist_kernel<<<grid_for((u64)nroots, 256, 64), 256, 0, c.stream>>>(dh, ids, (u64)nroots, out);
+        G500_HIP(hipGetLastError());
+        g500_copy_to_host(d.data(), out, (u64)nroots * sizeof(f32));
+    }
+    ws->gpu_arena.used = mark;
+    if (failed()) return -1;
+    for (i32 i = 0; i < nroots; ++i) order[i] = i;
+    std::stable_sort(order, order + nroots, [&](i32 x, i32 y) { return d[x] < d[y]; });
+    return 0;
 }
 
 }  // extern "C"

@@ -131,6 +131,7 @@ _SIGS = {
     "oned_graph_bfs_batch_depth": (None, [P(WorkerState), P(OnedGraph), P(BfsBatchState), i32, vp]),
     "oned_graph_bfs_batch_destroy": (None, [P(BfsBatchState)]),
     "g500_bfs_batch_level_log": (i32, [P(BfsBatchState), P(i32), P(u64), P(u64), P(f64), i32]),
+    "oned_graph_sssp_root_order": (i32, [P(WorkerState), P(OnedGraph), P(u64), i32, P(i32)]),
     "oned_graph_sssp_batch_create": (SsspBatchState, [P(WorkerState), P(OnedGraph)]),
     "oned_graph_sssp_batch_run": (None, [P(WorkerState), P(OnedGraph), P(SsspBatchState), P(u64), i32]),
     "oned_graph_sssp_batch_destroy": (None, [P(SsspBatchState)]),
@@ -353,6 +354,15 @@ class Session:
         if with_edges:
             return list(modes[:n]), list(recs[:n]), list(edges[:n]), list(ms[:n])
         return list(modes[:n]), list(recs[:n]), list(ms[:n])
+
+    def sssp_root_order(self, g, roots):
+        """Permutation of roots sorted by distance from the highest-degree vertex."""
+        r = np.ascontiguousarray(roots, np.uint64)
+        out = np.zeros(r.size, np.int32)
+        self.lib.oned_graph_sssp_root_order(C.byref(self.ws), C.byref(g), r.ctypes.data_as(P(u64)), int(r.size),
+                                            out.ctypes.data_as(P(i32)))
+        self.check()
+        return out
 
     def sssp_batch_create(self, g, delta=0.1):
         s = self.lib.oned_graph_sssp_batch_create(C.byref(self.ws), C.byref(g))

@@ -75,5 +75,7 @@ def poisson_run(sess, graphs, rate, count, slots=4, delta=0.2, seed=12345):
         "slots": slots, "launches": stats["launches"], "mean_batch": round(float(bs.mean()), 2),
         "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "wall_s": round(wall, 3), "req_per_s": round(len(done) / wall, 1),
+        # arrivals span at[-1]; the rest of the wall time drains the backlog left at the last arrival
+        "arrival_s": round(float(at[-1]) * 1e-3, 3), "drain_s": round(wall - float(at[-1]) * 1e-3, 3),
         "gteps": round(edges / wall / 1e9, 4),
     }

@@ -98,6 +98,15 @@ i32 g500_bfs_batch_level_log(const Bfs_Batch_State *b, i32 *modes, u64 *records,
                              i32 max_levels);
 
 /* ---- batched delta-stepping SSSP ---------------------------------------- */
+/* Launch grouping for batched SSSP, no counterpart in the reference: writes to
+ * order[0..nroots) the permutation that sorts roots[] (host array) by their
+ * distance from the graph's highest-degree vertex (unreached roots first,
+ * ties in input order). Shortest paths of Kronecker graphs run through the
+ * hubs, so roots at similar hub distances reach every vertex at similar
+ * distances; consecutive roots in this order, launched together, settle each
+ * vertex in the same buckets and share more work. Runs one single-root SSSP
+ * on temporary arena memory. Returns 0, or -1 after a failure. */
+i32 oned_graph_sssp_root_order(Worker_State *ws, const Oned_Graph *g, const u64 *roots, i32 nroots, i32 *order);
 /* Up to 32 roots in one persistent kernel with one bucket sequence shared by
  * all roots; items are (vertex, root mask) pairs and the 32 distance probes of
  * a vertex share one 128-byte line. Edges with w >= mid_weight are relaxed in

@@ -872,3 +872,21 @@ def test_sssp_check_matches_oracle(arena, oracle):
         arena.release(m)
         assert got == oracle.sssp_check(ref2, 0, d, p), p
     assert oracle.sssp_check(ref2, 0, d, np.array([0, 2, 1, 2], np.int64)) == 3  # the 1 <-> 2 cycle and 3 below it
+
+
+def test_sssp_root_order(arena, oracle):
+    """oned_graph_sssp_root_order sorts roots by their FP32 distance from the
+    highest-degree vertex (smallest id among ties), unreached roots first."""
+    e, w, ref = oracle_graph(12)
+    g = check_csr(arena, oracle, 1 << 12, e, w)
+    deg = np.diff(ref.rowstarts.astype(np.int64))
+    hub = int(np.flatnonzero(deg == deg.max())[0])
+    d, _ = oracle.dijkstra_f32(ref, hub)
+    roots = np.concatenate([oracle.roots(ref), np.flatnonzero(d < 0)[:2].astype(np.uint64)])
+    order = arena.sssp_root_order(g, roots)
+    assert sorted(order.tolist()) == list(range(len(roots)))
+    key = d[roots.astype(np.int64)]
+    assert np.array_equal(order, np.argsort(key, kind="stable"))
+    assert len(arena.sssp_root_order(g, roots[:0])) == 0
+    with pytest.raises(Exception):
+        arena.sssp_root_order(g, np.array([1 << 12], np.uint64))
