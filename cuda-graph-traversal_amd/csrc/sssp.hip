@@ -842,11 +842,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
     }
     // export in original ids (traversal.cu:333-340): unreached -> -1.0f / -1.
     // The probe is no longer read after the last barrier, so dist (its
-    // storage) can take the output.
-    for (u64 v = gtid; v < a.n; v += gthreads) {
-        const u64 l = a.lab[v];
+    // storage) can take the output. Lanes walk consecutive original ids, so
+    // the stores coalesce and the label reads are the scattered side.
+    for (u64 o = gtid; o < a.n; o += gthreads) {
+        const u64 l = a.lab[trav_id(a.perm, (u32)o)];
         const bool reached = (u32)(l >> 32) < kInfBits;
-        const u32 o = orig_id(a.iperm, (u32)v);
         if (reached && label_dist(l) >= a.dbound) atomicOr(&ctl->bar.error, 32u);
         a.dist[o] = reached ? label_dist(l) : -1.0f;
         a.pred[o] = reached ? (i64)orig_id(a.iperm, (u32)l) : -1;

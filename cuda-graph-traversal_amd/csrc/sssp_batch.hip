@@ -1102,21 +1102,59 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
         for (int i = 0; i < 5; ++i) st_relaxed(&ctl->ticks[i], tst[2 + i]);
         st_relaxed(&ctl->ticks[5], wall_clock64() - tst[1]);
     }
-    // export, root-major (traversal.cu:333-340): unreached -> -1.0f / -1. The
-    // probes are no longer read after the last barrier, so dist (their
-    // storage) can take the output.
-    const u64 nlab = a.n * kBatchRoots;
-    for (u64 tt = gtid; tt < nlab; tt += gthreads) {
-        const u64 v = tt / kBatchRoots;
-        const int r = (int)(tt & (kBatchRoots - 1));
-        if (r >= a.nroots) continue;
-        const u64 l = a.lab[tt];
-        const bool reached = (u32)(l >> 32) < kInfBits;
-        const u64 o = (u64)r * a.n + orig_id(a.iperm, (u32)v);
-        if (reached && label_dist(l) >= a.dbound) atomicOr(&ctl->bar.error, 32u);
-        a.dist[o] = reached ? label_dist(l) : -1.0f;
-        a.pred[o] = reached ? (i64)orig_id(a.iperm, (u32)l) : -1;
+    // the labels are exported by sssp_batch_export_kernel, launched next on the same stream
+}
+
+// Export in original ids, root-major (traversal.cu:333-340): unreached ->
+// -1.0f / -1. A block takes 64 consecutive original ids, reads their label
+// lines (the 32 roots' labels of a vertex are 256 contiguous bytes), turns
+// them root-major in LDS and writes each root's 64 distances and parents as
+// contiguous runs. Writing straight from the vertex-major labels would scatter
+// single 4- and 8-byte stores over 32 output arrays. Runs after the traversal
+// kernel, whose probe array is the dist storage.
+constexpr int kExportIds = 64;
+__global__ __launch_bounds__(256) void sssp_batch_export_kernel(const u64 *__restrict__ lab,
+                                                                const u32 *__restrict__ perm,
+                                                                const u32 *__restrict__ iperm, u64 n, int nroots,
+                                                                f32 dbound, f32 *__restrict__ dist,
+                                                                i64 *__restrict__ pred, SsspBatchCtl *ctl) {
+    __shared__ f32 sd[kBatchRoots][kExportIds + 1];
+    __shared__ i64 sp[kBatchRoots][kExportIds + 1];
+    bool over = false;
+    // load mapping: thread t takes id t / 4 and roots 8 (t % 4) .. 8 (t % 4) + 7
+    const int j = (int)(threadIdx.x >> 2), q = (int)(threadIdx.x & 3);
+    for (u64 o0 = (u64)blockIdx.x * kExportIds; o0 < n; o0 += (u64)gridDim.x * kExportIds) {
+        const u64 o = o0 + (u64)j;
+        if (o < n && 8 * q < nroots) {
+            const u64 v = perm ? perm[o] : o;
+            const uint4 *p = reinterpret_cast<const uint4 *>(lab + v * kBatchRoots + 8 * q);
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const uint4 x = p[h];
+                const u64 l2[2] = {((u64)x.y << 32) | x.x, ((u64)x.w << 32) | x.z};
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    const int r = 8 * q + 2 * h + i;
+                    const u64 l = l2[i];
+                    const bool reached = (u32)(l >> 32) < kInfBits;
+                    over = over || (reached && label_dist(l) >= dbound);
+                    sd[r][j] = reached ? label_dist(l) : -1.0f;
+                    sp[r][j] = reached ? (i64)orig_id(iperm, (u32)l) : -1;
+                }
+            }
+        }
+        __syncthreads();
+        const u64 m = min((u64)kExportIds, n - o0);
+        for (int k = (int)threadIdx.x; k < nroots * kExportIds; k += (int)blockDim.x) {
+            const int r = k / kExportIds, jj = k % kExportIds;
+            if ((u64)jj < m) {
+                dist[(u64)r * n + o0 + (u64)jj] = sd[r][jj];
+                pred[(u64)r * n + o0 + (u64)jj] = sp[r][jj];
+            }
+        }
+        __syncthreads();
     }
+    if (over) atomicOr(&ctl->bar.error, 32u);
 }
 
 // Packs each row's bounds and splits into its 32-byte record.
@@ -1331,12 +1369,22 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
     if (failed()) return false;
     void *args[] = {&a};
     s->nroots = nroots;
+    // the export's grid: as many blocks as the persistent grid holds waves / 4, at most one per 64 ids
+    const unsigned xgrid = (unsigned)std::max<u64>(1, std::min<u64>((n + kExportIds - 1) / kExportIds,
+                                                                     (u64)grid * (u64)block / 256));
+    auto export_labels = [&]() {
+        sssp_batch_export_kernel<<<xgrid, 256, 0, st>>>(a.lab, a.perm, a.iperm, n, nroots, a.dbound, a.dist, a.pred,
+                                                        a.ctl);
+        G500_HIP(hipGetLastError());
+    };
     if (library) {
         timer_start();
         launch_persistent((const void *)sssp_batch_kernel, grid, block, args, nullptr, true, dyn);
+        export_labels();
         timer_stop();
     } else {
         launch_persistent((const void *)sssp_batch_kernel, grid, block, args, st, false, dyn);
+        export_labels();
     }
     return !failed();
 }
