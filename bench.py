@@ -125,6 +125,8 @@ def main():
                     help="batched SSSP far-edge threshold (deferred over windows this wide) when not tuned")
     ap.add_argument("--no-root-order", dest="root_order", action="store_false",
                     help="SSSP launches take the roots in root-rule order instead of by hub distance")
+    ap.add_argument("--no-root-lag", dest="root_lag", action="store_false",
+                    help="batched SSSP roots share one bucket sequence (no per-root lags from hub distances)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (knobs at their defaults, delta from --delta)")
     ap.add_argument("--arena-gib", type=int, default=200)
@@ -175,7 +177,7 @@ def main():
     if not args.no_tune:
         from g500 import autotune
         t0 = time.perf_counter()
-        tuned = autotune.tune(sess, g, batch=args.mode == "batched", log=log)["params"]
+        tuned = autotune.tune(sess, g, batch=args.mode == "batched", log=log, lags=args.root_lag)["params"]
         delta_plain = tuned["sssp_delta"]
         delta_batch = tuned.get("batch_sssp_delta", delta_plain)
         long_weight = tuned.get("batch_sssp_long_weight", long_weight)
@@ -254,12 +256,21 @@ def main():
         # of a step are sorted by their distance from the highest-degree vertex. Shortest paths of a
         # Kronecker graph run through its hubs, so roots at similar hub distances reach each vertex at
         # similar distances, and the roots of one launch settle it in the same buckets and windows.
-        if args.root_order:
+        # Per-root bucket lags (once per graph, untimed, like the CSR build): the roots' distances
+        # from the highest-degree vertex. A root's distance to most vertices of a Kronecker graph is
+        # about its hub distance plus the vertex's, so a root that lags the others by its hub-distance
+        # difference settles each vertex in the same shared buckets as they do.
+        hubd = None
+        if args.root_order or args.root_lag:
             t_o = time.perf_counter()
-            rsets = [[grp[i] for i in sess.sssp_root_order(g, roots[grp])] for grp in rsets]
+            hubd = sess.sssp_hub_distances(g, roots)
             order_ms = (time.perf_counter() - t_o) * 1e3
-            log(f"[bench] SSSP launches take the roots by hub distance (oned_graph_sssp_root_order, "
-                f"{order_ms:.1f} ms, untimed)")
+            log(f"[bench] hub distances of the roots (oned_graph_sssp_hub_distances, {order_ms:.1f} ms, untimed)")
+        if args.root_order:  # launch groups of roots at similar hub distances
+            rsets = [[grp[i] for i in np.argsort(hubd[grp], kind="stable")] for grp in rsets]
+
+        def offsets_of(idx):
+            return hubd[idx] if args.root_lag else None
 
         def batched_step(grp):
             rr = roots[grp]
@@ -268,7 +279,7 @@ def main():
                 b += sess.batch(g, bb, rr[k:k + BFS_GROUP]) * 1e-3
             s = 0.0
             for k in range(0, len(rr), SSSP_GROUP):
-                s += sess.sssp_batch(g, sb, rr[k:k + SSSP_GROUP]) * 1e-3
+                s += sess.sssp_batch(g, sb, rr[k:k + SSSP_GROUP], offsets_of(grp[k:k + SSSP_GROUP])) * 1e-3
             return b, s
 
         for i in range(args.warmup):
@@ -301,7 +312,7 @@ def main():
         sess.release(mark_v)
         for k in range(0, len(rr), SSSP_GROUP):
             part = rr[k:k + SSSP_GROUP]
-            sess.sssp_batch(g, sb, part)
+            sess.sssp_batch(g, sb, part, offsets_of(rsets[0][k:k + SSSP_GROUP]))
             for i, r in enumerate(part):
                 checks["batched_sssp"] += 1
                 checks["batched_sssp_failures"] += int(
@@ -310,7 +321,7 @@ def main():
         # the 16-root SSSP batches of the batched workload (reported, not in value)
         t16, m16 = [], 0.0
         for k in range(0, len(rr), 16):
-            t16.append(sess.sssp_batch(g, sb, rr[k:k + 16]) * 1e-3)
+            t16.append(sess.sssp_batch(g, sb, rr[k:k + 16], offsets_of(rsets[0][k:k + 16])) * 1e-3)
             m16 += float(m_c[rsets[0][k:k + 16]].sum())
         sssp16 = {"launches": len(t16), "roots_per_launch": 16, "ms_per_launch": round(float(np.mean(t16)) * 1e3, 3),
                   "gteps": round(m16 / sum(t16) / 1e9, 4)}
@@ -394,7 +405,8 @@ def main():
                    "far_weight": far_weight if args.mode == "batched" else None,
                    "roots": nroots, "mode": args.mode, "tuned": tuned,
                    "sssp_root_order": ("hub distance" if args.root_order else "root rule")
-                   if args.mode == "batched" else None},
+                   if args.mode == "batched" else None,
+                   "sssp_root_lags": ("hub distance" if args.root_lag else None) if args.mode == "batched" else None},
         "bfs_gteps": round(me.sum() / tb.sum() / 1e9, 4),
         "sssp_gteps": round(me.sum() / ts.sum() / 1e9, 4),
         "plain": plain,

@@ -149,8 +149,10 @@ bool bfs_batch_enqueue(const Oned_Graph *g, Bfs_Batch_State *b, const u64 *roots
 void bfs_batch_depth_enqueue(const Oned_Graph *g, const Bfs_Batch_State *b, i32 r, i64 *depth, hipStream_t stream);
 u32 bfs_batch_error(const Bfs_Batch_State *b, hipStream_t stream);  // synchronizes the stream
 int bfs_batch_grid();
-bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roots, i32 nroots, hipStream_t stream,
-                        int grid);
+// offsets: null, or per root a distance offset (say its distance from a hub
+// vertex; negative: none) from which the roots' bucket lags are set
+bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roots, const f32 *offsets, i32 nroots,
+                        hipStream_t stream, int grid);
 u32 sssp_batch_error(const Sssp_Batch_State *s, hipStream_t stream);  // synchronizes the stream
 // Computes s's light/mid/long/far row splits and row records for its delta and
 // class weights on g (only the split arrays and row_info of s are used),

@@ -237,6 +237,7 @@ void slot_loop(G500_Server *s, int k) {
     G500_HIP(hipSetDevice(ctx().device));
     G500_Server::Slot &slot = s->slots[k];
     std::vector<u64> roots;
+    std::vector<f32> offsets;
     u64 summary[128];
     for (;;) {
         std::vector<Request *> batch;
@@ -275,7 +276,13 @@ void slot_loop(G500_Server *s, int k) {
             // block shape and grid share were fixed at start; batches of more than
             // 16 roots share more work per bucket and run at half the server's delta
             use_splits(&slot.sssp, s->splits[batch[0]->graph][roots.size() > 16 ? 1 : 0]);
-            ok = sssp_batch_enqueue(g, &slot.sssp, roots.data(), (i32)roots.size(), slot.stream, slot.grid_sssp);
+            // bucket lags from the roots' hub distances, so that the roots settle
+            // each vertex in the same buckets (scheduling only)
+            const std::vector<f32> &hd = s->hubd[batch[0]->graph];
+            offsets.clear();
+            for (u64 r : roots) offsets.push_back(hd.empty() ? -1.0f : hd[r]);
+            ok = sssp_batch_enqueue(g, &slot.sssp, roots.data(), offsets.data(), (i32)roots.size(), slot.stream,
+                                    slot.grid_sssp);
             if (ok) err = sssp_batch_error(&slot.sssp, slot.stream);
             if (ok && !err) {
                 for (size_t i = 0; i < batch.size(); ++i) {

@@ -1093,10 +1093,10 @@ void g500_sssp_phase_ms(const Sssp_State *s, f64 *ms) {
     ms[4] = (f64)h.scans;
 }
 
-i32 oned_graph_sssp_root_order(Worker_State *ws, const Oned_Graph *g, const u64 *roots, i32 nroots, i32 *order) {
+i32 oned_graph_sssp_hub_distances(Worker_State *ws, const Oned_Graph *g, const u64 *roots, i32 nroots, f32 *out) {
     Context &c = ctx();
-    if (nroots < 0 || (nroots && (!roots || !order))) {
-        fail("oned_graph_sssp_root_order: bad root list (%d roots)", nroots);
+    if (nroots < 0 || (nroots && (!roots || !out))) {
+        fail("oned_graph_sssp_hub_distances: bad root list (%d roots)", nroots);
         return -1;
     }
     for (i32 i = 0; i < nroots; ++i)
@@ -1110,16 +1110,24 @@ i32 oned_graph_sssp_root_order(Worker_State *ws, const Oned_Graph *g, const u64 
     const u64 mark = ws->gpu_arena.used;
     f32 *dh = (f32 *)arena_alloc(ws, n * sizeof(f32), "hub distances");
     u64 *ids = (u64 *)arena_alloc(ws, (u64)nroots * sizeof(u64), "root ids");
-    f32 *out = (f32 *)arena_alloc(ws, (u64)nroots * sizeof(f32), "root hub distances");
-    std::vector<f32> d(nroots);
+    f32 *dev = (f32 *)arena_alloc(ws, (u64)nroots * sizeof(f32), "root hub distances");
     if (!failed() && hub_distances(ws, g, dh)) {
         g500_copy_to_device(ids, roots, (u64)nroots * sizeof(u64));
-        gather_dist_kernel<<<grid_for((u64)nroots, 256, 64), 256, 0, c.stream>>>(dh, ids, (u64)nroots, out);
+        gather_dist_kernel<<<grid_for((u64)nroots, 256, 64), 256, 0, c.stream>>>(dh, ids, (u64)nroots, dev);
         G500_HIP(hipGetLastError());
-        g500_copy_to_host(d.data(), out, (u64)nroots * sizeof(f32));
+        g500_copy_to_host(out, dev, (u64)nroots * sizeof(f32));
     }
     ws->gpu_arena.used = mark;
-    if (failed()) return -1;
+    return failed() ? -1 : 0;
+}
+
+i32 oned_graph_sssp_root_order(Worker_State *ws, const Oned_Graph *g, const u64 *roots, i32 nroots, i32 *order) {
+    if (nroots < 0 || (nroots && (!roots || !order))) {
+        fail("oned_graph_sssp_root_order: bad root list (%d roots)", nroots);
+        return -1;
+    }
+    std::vector<f32> d(nroots);
+    if (oned_graph_sssp_hub_distances(ws, g, roots, nroots, d.data())) return -1;
     for (i32 i = 0; i < nroots; ++i) order[i] = i;
     std::stable_sort(order, order + nroots, [&](i32 x, i32 y) { return d[x] < d[y]; });
     return 0;

@@ -44,6 +44,9 @@
 // item reads its mask; the last chunk to finish clears it.
 // Counters rotate over three slots per step (written by the step's producer,
 // read by its consumer, cleared two steps ahead), as in the single-root kernel.
+#include <algorithm>
+#include <cmath>
+
 #include "sssp_common.hpp"
 
 namespace g500 {
@@ -67,7 +70,7 @@ struct SsspBatchCtl {
     CounterLine<u64> hhub[3];     // medium hub counters [slot]
     CounterLine<u32> llen[kDefer][2][2];  // deferred queue lengths [class][set][tier 0-1], set = the class's flush parity
     CounterLine<u64> lhub[kDefer][2];     // deferred queue hub counters [class][set]
-    alignas(256) u32 nmin[3];     // smallest distance bits >= hi, per slot
+    alignas(256) u32 rmin[3][kBatchRoots];  // min step: smallest open distance bits per root, per slot
     u32 nnext[3];                 // 1: a label entered [hi, hi + delta) (the bucket is not empty), per slot
     alignas(256) u64 buckets, light_iters, improvements, scans;
     u64 flushes;
@@ -140,6 +143,8 @@ struct SsspBatchArgs {
     f32 *dist;         // [nroots * n], root-major
     i64 *pred;         // [nroots * n]
     u64 roots[kBatchRoots];
+    u32 lag[kBatchRoots];   // buckets root r lags the shared bucket counter (0 for at least one root)
+    f32 uoff[kBatchRoots];  // fl(lag[r] * delta): the reduced-value offset of root r
     i32 nroots;
     f32 delta;
     f32 dbound;        // distances at or above it set error 32 (+inf without the pruned copy)
@@ -157,11 +162,37 @@ __device__ __forceinline__ void bsync(SsspBatchCtl *ctl, u32 phase, u32 a, u32 b
     }
 }
 
+// Per-root bucket state, the same in every block (each block keeps its copy
+// in LDS and updates it identically). Root r runs the bucket sequence
+// B(0) = 0, B(k + 1) = fl(B(k) + delta) (B(k) = 0 for k <= 0), lagging the
+// shared bucket counter g by lag_r buckets: its bucket index is
+// k_r = g - lag_r, and b[0..2][r] = B(k_r), B(k_r + 1), B(k_r + 2) are its
+// lo, hi and next-bucket top. Each root's own schedule is plain
+// delta-stepping, so every root ends at the same FP32 fixed point whatever its
+// lag. A root whose distances run ahead of the others' by about lag_r * delta
+// (the lags are set from the roots' distances to a hub) then settles each
+// vertex in the same shared buckets as the other roots, so a row pass serves
+// more roots at once.
+// Every write to this state follows a block barrier that every read of the
+// previous values precedes.
+// Reduced values fl(d + u_r), with u_r = fl(lag_r * delta), put the roots'
+// distances on the shared scale for the smallest-open-probe screen: rounding
+// is monotone, so d < z_r implies fl(d + u_r) <= fl(z_r + u_r).
+struct alignas(16) RootBuckets {
+    f32 b[3][kBatchRoots];
+    f32 u[kBatchRoots];
+    f32 ep[3][kBatchRoots];  // deferred classes: lo_r of the first bucket since the class's last flush
+    i32 k[kBatchRoots];
+    u32 jr[kBatchRoots];     // min step: advances until the root's bucket holds its smallest open distance
+    u32 smin[kBatchRoots];   // min step: this block's smallest open distance bits per root
+};
+
 // Where the pushes of a step go: pending-mask and queue buffer, counter slot,
-// the bound below which a lowered label is queued, and the bucket floor.
+// and the root bounds: a lowered label is queued below b[lvl + 1][r], and no
+// candidate may fall below the floor b[lvl][r] (lvl 0: current bucket, 1: next).
 struct BTarget {
-    int buf, slot;
-    f32 below, floor;
+    int buf, slot, lvl;
+    const RootBuckets *rb;
 };
 
 __device__ __forceinline__ int btier(u32 len) { return len <= kBSmall ? 0 : (len <= kBHub ? 1 : 2); }
@@ -368,8 +399,11 @@ __device__ __forceinline__ void relax_commit_push(const SsspBatchArgs &a, const 
                                                   u64 &improved, WaveQ &wq) {
     bool tried[R];
     u32 prev[R], out[R], ow[R], lw[R];
+    f32 below[R];
 #pragma unroll
     for (int h = 0; h < R; ++h) tried[h] = act[h] && pr[h].cur > pr[h].cb;
+#pragma unroll
+    for (int h = 0; h < R; ++h) below[h] = t.rb->b[t.lvl + 1][r[h]];
 #pragma unroll
     for (int h = 0; h < R; ++h)
         prev[h] = tried[h] ? atomicMin(&a.dl[(u64)pr[h].v * kBatchRoots + r[h]], pr[h].cb) : 0u;
@@ -384,7 +418,7 @@ __device__ __forceinline__ void relax_commit_push(const SsspBatchArgs &a, const 
         ow[h] = (tried[h] && pr[h].cur >= kInfBits) ? ((const u32 *)a.openw)[pr[h].v >> 5] : ~0u;
 #pragma unroll
     for (int h = 0; h < R; ++h)
-        lw[h] = (tried[h] && __uint_as_float(pr[h].cb) < t.below) ? a.split[pr[h].v] : 0u;
+        lw[h] = (tried[h] && __uint_as_float(pr[h].cb) < below[h]) ? a.split[pr[h].v] : 0u;
     u32 nimp = 0;  // lowered probes of this lane; summed per wave below (a wave-uniform counter)
 #pragma unroll
     for (int h = 0; h < R; ++h) {
@@ -393,7 +427,7 @@ __device__ __forceinline__ void relax_commit_push(const SsspBatchArgs &a, const 
         if (tried[h] && prev[h] > cb) {
             const f32 cand = __uint_as_float(cb);
             atomicMin((unsigned long long *)&a.lab[(u64)v * kBatchRoots + r[h]], ((unsigned long long)cb << 32) | u[h]);
-            atomicMin(&a.vmin[v], cb);
+            atomicMin(&a.vmin[v], __float_as_uint(cand + t.rb->u[r[h]]));
             if (prev[h] >= kInfBits) {  // v reached from root r: it has open work (again)
                 // bits are cleared only by dense scans, never during a relaxation phase, so a
                 // set bit read in this phase is current; a clear one may race a concurrent OR,
@@ -402,8 +436,8 @@ __device__ __forceinline__ void relax_commit_push(const SsspBatchArgs &a, const 
                 if (!(ow[h] & bit)) atomicOr(&((u32 *)a.openw)[v >> 5], bit);
             }
             ++nimp;
-            if (cand < t.floor) atomicOr(&a.ctl->bar.error, 16u);
-            out[h] = cand < t.below ? 1u << r[h] : 0u;
+            if (cand < t.rb->b[t.lvl][r[h]]) atomicOr(&a.ctl->bar.error, 16u);
+            out[h] = cand < below[h] ? 1u << r[h] : 0u;
         }
     }
     if (ballot(nimp != 0)) improved += wave_sum((u64)nimp);
@@ -708,20 +742,55 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
         wq.buf = t.buf;
         wq.slot = t.slot;
     };
+    // per-root bucket state: thread r < 32 of every block keeps root r's entries
+    __shared__ RootBuckets rbk;
+    const int rt = (int)threadIdx.x;  // the root this thread keeps (when < kBatchRoots)
+    if (rt < kBatchRoots) {
+        // selects, not a runtime index into the argument struct (which would copy it to scratch)
+        u32 lag = 0;
+        f32 uo = 0.0f;
+#pragma unroll
+        for (int i = 0; i < kBatchRoots; ++i)
+            if (rt == i) {
+                lag = a.lag[i];
+                uo = a.uoff[i];
+            }
+        const f32 b1 = next_bound(0.0f, a.delta), b2 = next_bound(b1, a.delta);
+        const i32 k = -(i32)lag;  // <= 0
+        rbk.k[rt] = k;
+        rbk.b[0][rt] = 0.0f;             // B(k), k <= 0
+        rbk.b[1][rt] = k + 1 >= 1 ? b1 : 0.0f;
+        rbk.b[2][rt] = k + 2 >= 2 ? b2 : (k + 2 == 1 ? b1 : 0.0f);
+        rbk.u[rt] = uo;
+#pragma unroll
+        for (int c = 0; c < kDefer; ++c) rbk.ep[c][rt] = 0.0f;
+    }
+    // one bucket further for root rt: B(k + 3) = fl(B(k + 2) + delta) once k + 2 >= 0
+    auto advance_root = [&]() {
+        const i32 k = rbk.k[rt];
+        const f32 b2 = rbk.b[2][rt];
+        rbk.b[0][rt] = rbk.b[1][rt];
+        rbk.b[1][rt] = b2;
+        rbk.b[2][rt] = k + 2 >= 0 ? next_bound(b2, a.delta) : 0.0f;
+        rbk.k[rt] = k + 1;
+    };
+    // the screen threshold of a scan: max_r fl(b[L][r] + u_r) (block-uniform)
+    auto screen_top = [&](int L) {
+        f32 z = 0.0f;
+        for (int r = 0; r < kBatchRoots; ++r) z = fmaxf(z, rbk.b[L][r] + rbk.u[r]);
+        return z;
+    };
     __syncthreads();
     u32 x = 0;
-    f32 lo = 0.0f, hi = next_bound(0.0f, a.delta);
-    // per deferred class: flushes so far (selects the counter set), whether the
-    // queue holds work, and the floor of the first bucket since the class's
-    // last flush (indexed with constants only, so they stay in registers)
+    u32 gb = 0;  // shared bucket counter
+    // per deferred class: flushes so far (selects the counter set) and whether
+    // the queue holds work (indexed with constants only, so they stay in registers)
     u32 nflush[kDefer];
     bool lpending[kDefer];
-    f32 epoch[kDefer];
 #pragma unroll
     for (int c = 0; c < kDefer; ++c) {
         nflush[c] = 0;
         lpending[c] = false;
-        epoch[c] = 0.0f;
     }
 
     const bool timer = gtid == 0;
@@ -756,7 +825,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
             st_relaxed(&ctl->hlen[s][0].v, 0u);
             st_relaxed(&ctl->hlen[s][1].v, 0u);
             st_relaxed(&ctl->hhub[s].v, 0ull);
-            st_relaxed(&ctl->nmin[s], 0xFFFFFFFFu);
+            for (int r = 0; r < kBatchRoots; ++r) st_relaxed(&ctl->rmin[s][r], 0xFFFFFFFFu);
             st_relaxed(&ctl->nnext[s], 0u);
         }
     };
@@ -775,20 +844,24 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
         *hub = ld_relaxed(&ctl->fhub[step % 3].v);
         return total + hub_count(*hub);
     };
-    auto target = [&](u32 step, f32 below, f32 floor) {
-        return BTarget{(int)((step + 1) & 1), (int)((step + 1) % 3), below, floor};
+    // pushes of step `step`; lvl 0: queued below hi_r (floor lo_r), 1: below hi2_r (floor hi_r)
+    auto target = [&](u32 step, int lvl) {
+        return BTarget{(int)((step + 1) & 1), (int)((step + 1) % 3), lvl, &rbk};
     };
     // Word-parallel pass over the vertices with open work: each lane reads one
     // word of openw, the wave walks the words that have open vertices, one
-    // lane per vertex. vmin[v], the smallest open probe of v (exact between
-    // phases), screens the vertex: only one with vmin < z reads its probe line,
-    // in the 16-byte pieces that hold a root not yet final. f(word, v, A, B,
-    // dmin, act, R) gets the roots whose open probe is in [x, y) (A) and in
-    // [y, z) (B), the smallest open probe bits >= y (vmin itself for a
-    // screened-out open vertex, all ones for a closed one), whether the line
-    // was read, and the reached roots that are not final (R, read lines only).
-    auto scan_open = [&](f32 x, f32 y, f32 z, auto &&f) {
-        const u32 zb = __float_as_uint(z);
+    // lane per vertex. vmin[v], the smallest reduced value fl(d + u_r) of v's
+    // open roots (exact between phases), screens the vertex: only one with
+    // vmin <= max_r fl(z_r + u_r) can hold a root with d < z_r, and only it
+    // reads its probe line, in the 16-byte pieces that hold a root not yet
+    // final. Per root r, with x_r = lo_r, y_r = hi_r and z_r = b[zl][r] (zl 1:
+    // hi_r, 2: hi2_r), f(word, v, A, B, dmin, act, R) gets the roots whose open
+    // probe is in [x_r, y_r) (A) and in [y_r, z_r) (B), the smallest reduced
+    // value of the open probes >= y_r (vmin itself for a screened-out open
+    // vertex, all ones for a closed one), whether the line was read, and the
+    // reached roots that are not final (R, read lines only).
+    auto scan_open = [&](int zl, auto &&f) {
+        const u32 zb = __float_as_uint(screen_top(zl));
         for (u64 w0 = wave_gid * kWave; w0 < nwords; w0 += nwaves * kWave) {
             const u64 wl = w0 + lane;
             const u64 open_l = wl < nwords ? a.openw[wl] : 0ull;
@@ -800,7 +873,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                 const u64 v = word * 64 + lane;
                 const bool isopen = (shfl64(open_l, j) >> lane) & 1;
                 const u32 vm = isopen ? a.vmin[v] : 0xFFFFFFFFu;
-                const bool act = vm < zb;
+                const bool act = isopen && vm <= zb;
                 const u32 open = act ? ~a.smask[v] : 0u;
                 u32 A = 0, B = 0, R = 0, dmin = act ? 0xFFFFFFFFu : vm;
                 if (open) {
@@ -811,15 +884,21 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                         if (!lq) continue;
                         const uint4 c = line[q];
                         const u32 bits[4] = {c.x, c.y, c.z, c.w};
+                        const float4 x4 = *reinterpret_cast<const float4 *>(&rbk.b[0][4 * q]);
+                        const float4 y4 = *reinterpret_cast<const float4 *>(&rbk.b[1][4 * q]);
+                        const float4 z4 = *reinterpret_cast<const float4 *>(&rbk.b[zl][4 * q]);
+                        const float4 u4 = *reinterpret_cast<const float4 *>(&rbk.u[4 * q]);
+                        const f32 xs[4] = {x4.x, x4.y, x4.z, x4.w}, ys[4] = {y4.x, y4.y, y4.z, y4.w};
+                        const f32 zs[4] = {z4.x, z4.y, z4.z, z4.w}, us[4] = {u4.x, u4.y, u4.z, u4.w};
 #pragma unroll
                         for (int i = 0; i < 4; ++i) {
                             if (!((lq >> i) & 1) || bits[i] >= kInfBits) continue;  // final or unreached
                             const f32 d = __uint_as_float(bits[i]);
                             const u32 bit = 1u << (4 * q + i);
                             R |= bit;
-                            if (d >= x && d < y) A |= bit;
-                            if (d >= y && d < z) B |= bit;
-                            if (d >= y) dmin = min(dmin, bits[i]);
+                            if (d >= xs[i] && d < ys[i]) A |= bit;
+                            if (d >= ys[i] && d < zs[i]) B |= bit;
+                            if (d >= ys[i]) dmin = min(dmin, __float_as_uint(d + us[i]));
                         }
                     }
                 }
@@ -834,26 +913,28 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
 #pragma unroll
     for (int i = 0; i < kBatchRoots; ++i)
         if (gtid == (u64)i) my_root = a.roots[i];
-    if (gtid < (u64)a.nroots) {
+    if (gtid < (u64)a.nroots) {  // block 0: thread gtid keeps root gtid's bucket state
         const u64 root = trav_id(a.perm, my_root);
         st_relaxed(&a.lab[root * kBatchRoots + gtid], root);  // distance 0.0f, parent = root
         st_relaxed(&a.dl[root * kBatchRoots + gtid], 0u);
-        st_relaxed(&a.vmin[root], 0u);
+        atomicMin(&a.vmin[root], __float_as_uint(rbk.u[rt]));  // reduced value of distance 0
         atomicOr((unsigned long long *)&a.openw[root >> 6], 1ull << (root & 63));
     }
     if (gtid == 0)
-        for (int i = 0; i < 3; ++i) st_relaxed(&ctl->nmin[i], 0xFFFFFFFFu);
+        for (int i = 0; i < 3; ++i)
+            for (int r = 0; r < kBatchRoots; ++r) st_relaxed(&ctl->rmin[i][r], 0xFFFFFFFFu);
     bsync(ctl, 1, 0, 0);
-    light_dest(target(0, 0.f, 0.f));
-    if (gtid < kWave) {  // step 0 queues the roots (repeated roots share one item)
-        const bool r = (i32)gtid < a.nroots;
-        push_item(a, r ? trav_id(a.perm, my_root) : 0u, r ? 1u << gtid : 0u, target(0, 0.f, 0.f), wq);
+    light_dest(target(0, 0));
+    if (gtid < kWave) {  // step 0 queues the roots without lag (repeated roots share one item)
+        // a lagging root is queued by the dense step before its bucket 0 (d = 0 in [hi_r, hi2_r) = [0, delta))
+        const bool r = (i32)gtid < a.nroots && rbk.k[rt] == 0;
+        push_item(a, r ? trav_id(a.perm, my_root) : 0u, r ? 1u << gtid : 0u, target(0, 0), wq);
     }
     q_flush(a, wq);
     bsync(ctl, 1, 1, 0);
     x = 1;
 
-    for (;;) {  // buckets [lo, hi)
+    for (;;) {  // shared buckets: root r's [lo_r, hi_r)
         for (;;) {  // light steps
             u32 cnt[2];
             u64 hub;
@@ -861,7 +942,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
             clear_ahead(x);
             if (total == 0 || ld_relaxed(&ctl->bar.error)) break;
             if (gtid == 0) atomicAdd((unsigned long long *)&ctl->light_iters, 1ull);
-            const BTarget t = target(x, hi, lo);
+            const BTarget t = target(x, 0);
             const int buf = (int)(x & 1);
             light_dest(t);
             // selects, not runtime indexes into the argument struct (which would copy it to scratch)
@@ -872,20 +953,22 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                                 buf ? a.fq_len[1] : a.fq_len[0], buf ? a.fq_done[1] : a.fq_done[0], pb, hub, t,
                                 improved, wks[0], wq);
             q_flush(a, wq);
-            bsync(ctl, 2, __float_as_uint(lo), x);
+            bsync(ctl, 2, gb, x);
             tick(0, total);
             ++x;
         }
         if (ld_relaxed(&ctl->bar.error)) break;
-        // dense step x: medium items of [lo, hi), mid and long edges deferred,
-        // next bucket [hi, hi2) queued
+        // dense step x: medium items of [lo_r, hi_r), mid, long and far edges
+        // deferred, next bucket [hi_r, hi2_r) queued
         if (gtid == 0) atomicAdd((unsigned long long *)&ctl->buckets, 1ull);
-        const f32 hi2 = next_bound(hi, a.delta);
+        if (rt < kBatchRoots) {
 #pragma unroll
-        for (int c = 0; c < kDefer; ++c)
-            if (!lpending[c]) epoch[c] = lo;
+            for (int c = 0; c < kDefer; ++c)
+                if (!lpending[c]) rbk.ep[c][rt] = rbk.b[0][rt];
+        }
+        __syncthreads();
         {
-            const BTarget t = target(x, hi2, hi);
+            const BTarget t = target(x, 1);
             const int hs = (int)((x + 1) % 3);
             light_dest(t);
             wq.hs = hs;
@@ -906,8 +989,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                 hub_append(&ctl->lhub[c][wq.ls[c]].v, a.dq[c][2], a.dq_off[c], a.dq_len[c], q && btier(len) == 2,
                            v, len, a.n, &ctl->bar.error);
             };
-            scan_open(lo, hi, hi2, [&](u64 word, u64 v, u32 H, u32 N, u32 dmin, bool act, u32 R) {
-                if (act) a.vmin[v] = dmin;  // the open roots left are those >= hi
+            scan_open(2, [&](u64 word, u64 v, u32 H, u32 N, u32 dmin, bool act, u32 R) {
+                if (act) a.vmin[v] = dmin;  // the open roots left are those >= hi_r
                 {
                     const u64 na = __popcll(ballot(act)), nh = __popcll(ballot(H != 0));
                     if (lane == 0) work_add(wks[2], 1, na, nh);
@@ -948,7 +1031,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                 if (hslot != ~0ull) a.hq_mask[hslot] = H;
             });
             q_flush(a, wq);
-            bsync(ctl, 3, __float_as_uint(lo), x);
+            bsync(ctl, 3, gb, x);
             // The deferred queues are read between the two barriers of this
             // step: the next appends to these sets come after the second one.
             u32 lc[kDefer][2];
@@ -966,7 +1049,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
             hub_tier<kSegMedium>(a, a.hqh, a.hq_off, a.hq_len, a.hq_mask, nullptr, ld_relaxed(&ctl->hhub[hs].v), t,
                                  improved, wks[1], wq);
             q_flush(a, wq);
-            bsync(ctl, 4, __float_as_uint(lo), x);
+            bsync(ctl, 4, gb, x);
             tick(2, (u64)ld_relaxed(&ctl->hlen[hs][0].v) + ld_relaxed(&ctl->hlen[hs][1].v) +
                         hub_count(ld_relaxed(&ctl->hhub[hs].v)));
             // Flush a deferred class c when the next bucket reaches past
@@ -987,7 +1070,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                 u64 items = 0;
 #pragma unroll
                 for (int c = 0; c < kDefer; ++c) {
-                    fl[c] = lpending[c] && (hi2 > epoch[c] + a.dw[c] || next_empty);
+                    // due when some root's next bucket reaches past fl(epoch_r + W_c)
+                    bool due = next_empty;
+                    for (int r = 0; r < a.nroots && !due; ++r) due = rbk.b[2][r] > rbk.ep[c][r] + a.dw[c];
+                    fl[c] = lpending[c] && due;
                     anyf = anyf || fl[c];
                     if (fl[c]) items += (u64)lc[c][0] + lc[c][1] + hub_count(lh[c]);
                 }
@@ -1017,7 +1103,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                     flush_class(std::integral_constant<int, 1>());
                     flush_class(std::integral_constant<int, 2>());
                     q_flush(a, wq);
-                    bsync(ctl, 7, __float_as_uint(lo), x);
+                    bsync(ctl, 7, gb, x);
                     tick(3, items);
 #pragma unroll
                     for (int c = 0; c < kDefer; ++c)
@@ -1035,49 +1121,88 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
             u64 hub;
             // next bucket not empty: queued light work, or labels that entered it
             if (queue_size(x, cnt, &hub) || ld_relaxed(&ctl->nnext[x % 3])) {
-                lo = hi;
-                hi = hi2;
+                // every thread of the block has read this bucket's bounds (the
+                // flush decision reads them after the last grid barrier) before
+                // they change
+                __syncthreads();
+                if (rt < kBatchRoots) advance_root();
+                __syncthreads();
+                ++gb;
                 continue;
             }
         }
         if (ld_relaxed(&ctl->bar.error)) break;
-        // min step: smallest distance >= hi over all labels
+        // min step: each root's smallest open distance (every open label is >= hi2_r now)
         clear_ahead(x);
         {
-            u32 m = 0xFFFFFFFFu;
-            scan_open(hi, hi, hi, [&](u64 word, u64 v, u32 A, u32 B, u32 dmin, bool act, u32 R) {
-                (void)word;
-                (void)v;
-                (void)A;
-                (void)B;
-                (void)act;
-                (void)R;
-                if (dmin < kInfBits) m = min(m, dmin);
-            });
-            m = wave_min(m);
-            if (lane == 0 && m != 0xFFFFFFFFu) atomicMin(&ctl->nmin[(x + 1) % 3], m);
+            if (rt < kBatchRoots) rbk.smin[rt] = 0xFFFFFFFFu;
+            __syncthreads();
+            for (u64 w0 = wave_gid * kWave; w0 < nwords; w0 += nwaves * kWave) {
+                const u64 wl = w0 + lane;
+                const u64 open_l = wl < nwords ? a.openw[wl] : 0ull;
+                u64 todo = ballot(open_l != 0);
+                while (todo) {
+                    const int j = __ffsll((unsigned long long)todo) - 1;
+                    todo &= todo - 1;
+                    const u64 v = (w0 + (u64)j) * 64 + lane;
+                    const bool isopen = (shfl64(open_l, j) >> lane) & 1;
+                    const u32 open = isopen ? ~a.smask[v] : 0u;
+                    if (!open) continue;
+                    const uint4 *line = reinterpret_cast<const uint4 *>(a.dl + v * kBatchRoots);
+                    for (int q = 0; q < kBatchRoots / 4; ++q) {
+                        const u32 lq = (open >> (4 * q)) & 0xFu;
+                        if (!lq) continue;
+                        const uint4 c = line[q];
+                        const u32 bits[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            if (((lq >> i) & 1) && bits[i] < kInfBits) atomicMin(&rbk.smin[4 * q + i], bits[i]);
+                    }
+                }
+            }
+            __syncthreads();
+            if (rt < kBatchRoots && rbk.smin[rt] != 0xFFFFFFFFu) atomicMin(&ctl->rmin[(x + 1) % 3][rt], rbk.smin[rt]);
         }
-        bsync(ctl, 5, __float_as_uint(lo), x);
+        bsync(ctl, 5, gb, x);
         ++x;
-        const u32 nm = ld_relaxed(&ctl->nmin[x % 3]);
         if (gtid == 0) atomicAdd((unsigned long long *)&ctl->scans, 1ull);
-        if (nm == 0xFFFFFFFFu) {
+        // jump: the fewest bucket advances J >= 1 after which some root's
+        // bucket [lo_r, hi_r) holds that root's smallest open distance; no
+        // root skips a bucket with open labels (each needs >= J advances)
+        if (rt < kBatchRoots) {
+            const u32 m = ld_relaxed(&ctl->rmin[x % 3][rt]);
+            u32 j = 0xFFFFFFFFu;
+            if (m < kInfBits) {
+                i32 k = rbk.k[rt];
+                f32 b1 = rbk.b[1][rt], b2 = rbk.b[2][rt];
+                j = 0;
+                do {
+                    const f32 nb = k + 2 >= 0 ? next_bound(b2, a.delta) : 0.0f;
+                    b1 = b2;
+                    b2 = nb;
+                    ++k;
+                    ++j;
+                } while (!(__uint_as_float(m) < b1));
+            }
+            rbk.jr[rt] = j;
+        }
+        __syncthreads();
+        u32 jump = 0xFFFFFFFFu;
+        for (int r = 0; r < kBatchRoots; ++r) jump = min(jump, rbk.jr[r]);
+        if (jump == 0xFFFFFFFFu) {  // no open label left
             tick(4);
             break;
         }
-        const f32 dmin = __uint_as_float(nm);
-        lo = hi;
-        hi = next_bound(lo, a.delta);
-        while (dmin >= hi) {
-            lo = hi;
-            hi = next_bound(lo, a.delta);
-        }
-        // collect step: queue the bucket [lo, hi) that holds dmin
+        if (rt < kBatchRoots)
+            for (u32 i = 0; i < jump; ++i) advance_root();
+        __syncthreads();
+        gb += jump;
+        // collect step: queue every root's bucket [lo_r, hi_r)
         clear_ahead(x);
         {
-            const BTarget t = target(x, hi, lo);
+            const BTarget t = target(x, 0);
             light_dest(t);
-            scan_open(lo, hi, hi, [&](u64 word, u64 v, u32 M, u32 B, u32 dmin, bool act, u32 R) {
+            scan_open(1, [&](u64 word, u64 v, u32 M, u32 B, u32 dmin, bool act, u32 R) {
                 (void)word;
                 (void)B;
                 (void)dmin;
@@ -1087,7 +1212,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
             });
             q_flush(a, wq);
         }
-        bsync(ctl, 6, __float_as_uint(lo), x);
+        bsync(ctl, 6, gb, x);
         if (gtid == 0) atomicAdd((unsigned long long *)&ctl->scans, 1ull);
         tick(4);
         ++x;
@@ -1267,8 +1392,28 @@ void sssp_batch_prepare(const Oned_Graph *g, Sssp_Batch_State *s, hipStream_t st
     }
 }
 
-bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roots, i32 nroots, hipStream_t stream,
-                        int grid) {
+// Bucket lags from distance offsets: the root with the largest offset runs
+// without lag, root r lags round((max - offset_r) / delta) buckets (at most
+// kMaxLag); roots without an offset (null, negative or not finite) lag none.
+// Lags only order the work: every lag gives the same distances.
+constexpr u32 kMaxLag = 1u << 16;
+void sssp_batch_lags(const f32 *offsets, i32 nroots, f32 delta, u32 *lag, f32 *uoff) {
+    f64 omax = -1.0;
+    for (i32 i = 0; offsets && i < nroots; ++i)
+        if (std::isfinite(offsets[i]) && offsets[i] >= 0.0f) omax = std::max(omax, (f64)offsets[i]);
+    for (int i = 0; i < kBatchRoots; ++i) {
+        u32 l = 0;
+        if (omax >= 0.0 && i < nroots && std::isfinite(offsets[i]) && offsets[i] >= 0.0f) {
+            const f64 q = std::floor((omax - (f64)offsets[i]) / (f64)delta + 0.5);
+            l = q >= (f64)kMaxLag ? kMaxLag : (u32)q;
+        }
+        lag[i] = l;
+        uoff[i] = (f32)((f64)l * (f64)delta);
+    }
+}
+
+bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roots, const f32 *offsets, i32 nroots,
+                        hipStream_t stream, int grid) {
     Context &c = ctx();
     if (nroots < 1 || nroots > kBatchRoots) {
         fail("batched SSSP takes 1..%d roots (got %d)", kBatchRoots, nroots);
@@ -1359,6 +1504,7 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
     for (i32 i = 0; i < nroots; ++i) a.roots[i] = roots[i];
     a.nroots = nroots;
     a.delta = s->delta;
+    sssp_batch_lags(offsets, nroots, s->delta, a.lag, a.uoff);
     a.dbound = g->rowstarts_s ? kPruneDistBound : __builtin_inff();
     a.dw[0] = wmid;
     a.dw[1] = wlong;
@@ -1446,14 +1592,32 @@ Sssp_Batch_State oned_graph_sssp_batch_create(Worker_State *ws, const Oned_Graph
 
 void oned_graph_sssp_batch_run(Worker_State *ws, const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roots,
                                i32 nroots) {
+    oned_graph_sssp_batch_run_offsets(ws, g, s, roots, nullptr, nroots);
+}
+
+void oned_graph_sssp_batch_run_offsets(Worker_State *ws, const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roots,
+                                       const f32 *offsets, i32 nroots) {
     (void)ws;
-    if (!sssp_batch_enqueue(g, s, roots, nroots, nullptr, 0)) return;
+    if (!sssp_batch_enqueue(g, s, roots, offsets, nroots, nullptr, 0)) return;
     const u32 e = sssp_batch_error(s, nullptr);
-    if (e)
+    if (e) {
+        // where the blocks stood: the fewest and most grid barriers passed, with the phase and tags of each
+        static SsspBatchCtl h;
+        g500_copy_to_host(&h, s->ctl, sizeof(h));
+        const int nb = std::min(sssp_batch_grid(), kMaxTraceBlocks);
+        u32 lo = 0xFFFFFFFFu, hi = 0;
+        int at_lo = 0, at_hi = 0;
+        for (int b = 0; b < nb; ++b) {
+            if (h.trace[b].barriers < lo) { lo = h.trace[b].barriers; at_lo = b; }
+            if (h.trace[b].barriers > hi) { hi = h.trace[b].barriers; at_hi = b; }
+        }
+        const BlockTrace &tl = h.trace[at_lo], &th = h.trace[at_hi];
         fail("batched SSSP kernel error %u (1 barrier timeout, 2 hub list overflow, 4 bad queue entry, 8 queue "
              "overflow, 16 bucket invariant, 32 distance at or above 2048 on the pruned SSSP copy: build with "
-             "sssp_prune 0)",
-             e);
+             "sssp_prune 0); barriers passed min %u (block %d phase %u bucket %u step %u) max %u (block %d phase %u "
+             "bucket %u step %u)",
+             e, lo, at_lo, tl.phase, tl.a, tl.b, hi, at_hi, th.phase, th.a, th.b);
+    }
 }
 
 void oned_graph_sssp_batch_destroy(Sssp_Batch_State *s) {

@@ -132,8 +132,11 @@ _SIGS = {
     "oned_graph_bfs_batch_destroy": (None, [P(BfsBatchState)]),
     "g500_bfs_batch_level_log": (i32, [P(BfsBatchState), P(i32), P(u64), P(u64), P(f64), i32]),
     "oned_graph_sssp_root_order": (i32, [P(WorkerState), P(OnedGraph), P(u64), i32, P(i32)]),
+    "oned_graph_sssp_hub_distances": (i32, [P(WorkerState), P(OnedGraph), P(u64), i32, P(f32)]),
     "oned_graph_sssp_batch_create": (SsspBatchState, [P(WorkerState), P(OnedGraph)]),
     "oned_graph_sssp_batch_run": (None, [P(WorkerState), P(OnedGraph), P(SsspBatchState), P(u64), i32]),
+    "oned_graph_sssp_batch_run_offsets": (None, [P(WorkerState), P(OnedGraph), P(SsspBatchState), P(u64), P(f32),
+                                                 i32]),
     "oned_graph_sssp_batch_destroy": (None, [P(SsspBatchState)]),
     "g500_sssp_batch_run_stats": (None, [P(SsspBatchState), P(u64), P(u64), P(u64)]),
     "g500_sssp_batch_phase_ms": (None, [P(SsspBatchState), P(f64)]),
@@ -364,16 +367,35 @@ class Session:
         self.check()
         return out
 
+    def sssp_hub_distances(self, g, roots):
+        """FP32 distances of the roots from the highest-degree vertex (-1 unreached)."""
+        r = np.ascontiguousarray(roots, np.uint64)
+        out = np.zeros(r.size, np.float32)
+        self.lib.oned_graph_sssp_hub_distances(C.byref(self.ws), C.byref(g), r.ctypes.data_as(P(u64)), int(r.size),
+                                               out.ctypes.data_as(P(f32)))
+        self.check()
+        return out
+
     def sssp_batch_create(self, g, delta=0.1):
         s = self.lib.oned_graph_sssp_batch_create(C.byref(self.ws), C.byref(g))
         self.check()
         s.delta = delta
         return s
 
-    def sssp_batch(self, g, s, roots):
+    def sssp_batch(self, g, s, roots, offsets=None):
+        """Batched SSSP from up to 32 roots; `offsets` (one per root, e.g. the
+        roots' hub distances) set per-root bucket lags."""
         r = np.ascontiguousarray(roots, np.uint64)
-        self.lib.oned_graph_sssp_batch_run(C.byref(self.ws), C.byref(g), C.byref(s), r.ctypes.data_as(P(u64)),
-                                           int(r.size))
+        if offsets is None:
+            self.lib.oned_graph_sssp_batch_run(C.byref(self.ws), C.byref(g), C.byref(s), r.ctypes.data_as(P(u64)),
+                                               int(r.size))
+        else:
+            o = np.ascontiguousarray(offsets, np.float32)
+            if o.size != r.size:
+                raise ValueError("one offset per root")
+            self.lib.oned_graph_sssp_batch_run_offsets(C.byref(self.ws), C.byref(g), C.byref(s),
+                                                       r.ctypes.data_as(P(u64)), o.ctypes.data_as(P(f32)),
+                                                       int(r.size))
         self.check()
         return self.lib.g500_last_run_ms()
 

@@ -9,7 +9,8 @@ that a benchmark measures. Candidates:
   plain SSSP:    delta, sssp_block, sssp_pull (the bucket share above which
                  heavy relaxations are pulled)
   batched SSSP:  (delta, mid_weight, long_weight, far_weight) candidates, then
-                 batch_sssp_block for the best of them
+                 batch_sssp_block for the best of them (with per-root bucket
+                 lags from the tuning roots' hub distances unless lags=False)
 Every candidate gives the same depths and distances (the tuner changes only
 schedules), so the choice is by time alone. Plain BFS candidates are timed on
 16 roots and plain SSSP ones on 4 (the mean of their times; a few roots whose
@@ -26,7 +27,7 @@ SSSP_DELTA = [0.025, 0.05, 0.1]
 SSSP_PULL = [0.02, 0.05, 0.2]
 # (delta, mid, long, far); far == long leaves the far class empty
 BATCH_SSSP = [(0.025, 0.075, 0.25, 0.5), (0.025, 0.05, 0.15, 0.3), (0.025, 0.075, 0.25, 0.25),
-              (0.0125, 0.05, 0.2, 0.5), (0.05, 0.1, 0.3, 0.6)]
+              (0.0125, 0.05, 0.2, 0.5), (0.05, 0.1, 0.3, 0.6), (0.05, 0.075, 0.2, 0.4)]
 
 
 def tuning_roots(sess, g, count, seed=2024):
@@ -52,18 +53,18 @@ def _best(results):
     return min(results, key=lambda r: r["ms"])
 
 
-def tune(sess, g, batch=True, roots_plain=4, roots_bfs=16, seed=2024, log=None):
+def tune(sess, g, batch=True, roots_plain=4, roots_bfs=16, seed=2024, log=None, lags=True):
     """Tunes the knobs on graph g, applies the best settings, and returns
     them with the measurements: {"params": ..., "trials": [...]}."""
     say = log or (lambda *a: None)
     mark = sess.mark()  # the tuner's traversal states are released at the end
     try:
-        return _tune(sess, g, batch, roots_plain, roots_bfs, seed, say)
+        return _tune(sess, g, batch, roots_plain, roots_bfs, seed, say, lags)
     finally:
         sess.release(mark)
 
 
-def _tune(sess, g, batch, roots_plain, roots_bfs, seed, say):
+def _tune(sess, g, batch, roots_plain, roots_bfs, seed, say, lags):
     roots = tuning_roots(sess, g, max(64, roots_plain, roots_bfs), seed)
     plain = roots[:roots_plain]
     trials = []
@@ -125,10 +126,12 @@ def _tune(sess, g, batch, roots_plain, roots_bfs, seed, say):
         trials += res
         say(f"[tune] batched bfs {best_bb}")
         sb = sess.sssp_batch_create(g)
+        width = int(sess.get_param("sssp_batch_max"))
+        offs = sess.sssp_hub_distances(g, roots[:width]) if lags else None
         res = []
         for d, wm, wl, wf in BATCH_SSSP:
             sb.delta, sb.mid_weight, sb.long_weight, sb.far_weight = d, wm, wl, wf
-            ms = sess.sssp_batch(g, sb, roots[:int(sess.get_param("sssp_batch_max"))])
+            ms = sess.sssp_batch(g, sb, roots[:width], offs)
             res.append({"kind": "batch_sssp", "delta": d, "mid_weight": wm, "long_weight": wl, "far_weight": wf,
                         "ms": ms})
         best_bs = _best(res)
@@ -136,8 +139,7 @@ def _tune(sess, g, batch, roots_plain, roots_bfs, seed, say):
         sb.long_weight, sb.far_weight = best_bs["long_weight"], best_bs["far_weight"]
         for blk in BLOCKS:  # block shape for the best schedule
             sess.set_param("batch_sssp_block", blk)
-            res.append(dict(best_bs, batch_sssp_block=blk,
-                            ms=sess.sssp_batch(g, sb, roots[:int(sess.get_param("sssp_batch_max"))])))
+            res.append(dict(best_bs, batch_sssp_block=blk, ms=sess.sssp_batch(g, sb, roots[:width], offs)))
         best_bs = _best(res)
         best_bs.setdefault("batch_sssp_block", int(old["batch_sssp_block"]))
         sess.set_param("batch_sssp_block", best_bs["batch_sssp_block"])

@@ -107,6 +107,12 @@ i32 g500_bfs_batch_level_log(const Bfs_Batch_State *b, i32 *modes, u64 *records,
  * vertex in the same buckets and share more work. Runs one single-root SSSP
  * on temporary arena memory. Returns 0, or -1 after a failure. */
 i32 oned_graph_sssp_root_order(Worker_State *ws, const Oned_Graph *g, const u64 *roots, i32 nroots, i32 *order);
+/* No counterpart in the reference: out[i] = FP32 distance of roots[i] (host
+ * array) from the graph's highest-degree vertex, -1.0f when unreached (one
+ * single-root SSSP on temporary arena memory). These are the offsets that
+ * oned_graph_sssp_batch_run_offsets turns into bucket lags. Returns 0, or -1
+ * after a failure. */
+i32 oned_graph_sssp_hub_distances(Worker_State *ws, const Oned_Graph *g, const u64 *roots, i32 nroots, f32 *out);
 /* Up to 32 roots in one persistent kernel with one bucket sequence shared by
  * all roots; items are (vertex, root mask) pairs and the 32 distance probes of
  * a vertex share one 128-byte line. Edges with w >= mid_weight are relaxed in
@@ -167,6 +173,17 @@ Sssp_Batch_State oned_graph_sssp_batch_create(Worker_State *ws, const Oned_Graph
 /* roots: host array of nroots (1..32, g500_get_param("sssp_batch_max")) vertex ids (repeats allowed). */
 void oned_graph_sssp_batch_run(Worker_State *ws, const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roots,
                                i32 nroots);
+/* No counterpart in the reference: oned_graph_sssp_batch_run with per-root
+ * bucket lags. offsets: host array of nroots distance offsets (typically
+ * oned_graph_sssp_hub_distances of the roots; a negative or non-finite entry
+ * means none). Root r runs the usual bucket sequence lagging the shared bucket
+ * counter by round((max offset - offsets[r]) / delta) buckets, so roots whose
+ * distances differ by about their offsets settle each vertex in the same
+ * buckets and share the row passes. Distances and the parent certificate are
+ * those of oned_graph_sssp_batch_run for any offsets: every label-correcting
+ * schedule reaches the same FP32 fixed point. */
+void oned_graph_sssp_batch_run_offsets(Worker_State *ws, const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roots,
+                                       const f32 *offsets, i32 nroots);
 void oned_graph_sssp_batch_destroy(Sssp_Batch_State *s);
 /* Bucket / light-step / label-improvement counts of the last batched run
  * (improvements summed over the roots). */

@@ -123,13 +123,17 @@ def sssp_plain(sess, oracle, g, h, roots, deltas):
     return out
 
 
-def sssp_batched(sess, oracle, g, h, roots, plain, certify, delta=0.05):
-    """One batched launch over `roots`: distances of roots in `plain` equal the
-    single-root kernel's bit for bit; the roots in `certify` pass the
-    certificate. Returns the launch's distances of the certified roots."""
+def sssp_batched(sess, oracle, g, h, roots, plain, certify, delta=0.05, offsets=None):
+    """One batched launch over `roots` (with per-root bucket lags from
+    `offsets`; "hub": the roots' hub distances): distances of roots in `plain`
+    equal the single-root kernel's bit for bit; the roots in `certify` pass
+    the certificate. Returns the launch's distances of the certified roots."""
     sb = sess.sssp_batch_create(g, delta)
-    ms = sess.sssp_batch(g, sb, roots)
-    say(f"batched SSSP, {len(roots)} roots, delta {delta}: {ms:.1f} ms")
+    if isinstance(offsets, str) and offsets == "hub":
+        offsets = sess.sssp_hub_distances(g, roots)
+    ms = sess.sssp_batch(g, sb, roots, offsets)
+    say(f"batched SSSP, {len(roots)} roots, delta {delta}, lags {'on' if offsets is not None else 'off'}: "
+        f"{ms:.1f} ms")
     got = {}
     for i, r in enumerate(roots):
         r = int(r)

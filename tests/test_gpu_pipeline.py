@@ -530,6 +530,52 @@ def test_sssp_batch_bitexact(arena, oracle, scale):
     sssp_batch_check(arena, oracle, g, ref, s, some)
 
 
+@pytest.mark.parametrize("scale", [10, 12, 16])
+def test_sssp_batch_lags(arena, oracle, scale):
+    """Per-root bucket lags order the work only: with the roots' hub
+    distances, random offsets (lags of up to 100 buckets), offsets without a
+    value (negative, NaN, inf) and one huge offset (lags at their cap, crossed
+    by min-step jumps), every root's distances equal the oracle's FP32 Dijkstra
+    bit for bit and the parents pass the certificate."""
+    e, w, ref = oracle_graph(scale)
+    g = arena.csr(arena.upload_tuples(e, w), 1 << scale)
+    roots = oracle.roots(ref)
+    width = int(arena.get_param("sssp_batch_max"))
+    rr = roots[:width]
+    hub = arena.sssp_hub_distances(g, rr)
+    deg = np.diff(ref.rowstarts.astype(np.int64))
+    d_hub, _ = oracle.dijkstra_f32(ref, int(np.argmax(deg)))
+    assert np.array_equal(hub.view(np.uint32), d_hub[rr.astype(np.int64)].view(np.uint32))
+    odd = np.random.default_rng(scale).uniform(0.0, 2.0, width).astype(np.float32)
+    odd[::5] = -1.0
+    odd[1::7] = np.nan
+    odd[2::9] = np.inf
+    huge = hub.copy()
+    huge[3] = 1.0e6
+    s = arena.sssp_batch_create(g)
+    for delta, wm, wl, wf in ((0.02, 0.05, 0.2, 0.5), (0.1, 0.05, 0.2, 0.4), (0.02, 0.0, 0.0, 0.0)):
+        s.delta, s.mid_weight, s.long_weight, s.far_weight = delta, wm, wl, wf
+        for name, offsets in (("hub", hub), ("random", odd), ("huge", huge)):
+            say("sssp batch lags", scale, delta, name)
+            arena.sssp_batch(g, s, rr, offsets)
+            for i, root in enumerate(rr):
+                d_ref, _ = oracle.dijkstra_f32(ref, int(root))
+                d, p = arena.sssp_batch_result(g, s, i)
+                assert np.array_equal(d.view(np.uint32), d_ref.view(np.uint32)), (delta, name, i)
+                assert oracle.sssp_check(ref, int(root), d, p) == 0, (delta, name, i)
+    # fewer roots, repeats and an isolated vertex, with lags
+    isolated = int(np.flatnonzero(deg == 0)[0])
+    some = np.array([roots[20], roots[20], isolated, roots[17]], np.uint64)
+    s.delta, s.mid_weight, s.long_weight, s.far_weight = 0.1, 0.05, 0.2, 0.5
+    arena.sssp_batch(g, s, some, np.array([0.9, 0.1, -1.0, 0.4], np.float32))
+    for i, root in enumerate(some):
+        d_ref, _ = oracle.dijkstra_f32(ref, int(root))
+        d, p = arena.sssp_batch_result(g, s, i)
+        assert np.array_equal(d.view(np.uint32), d_ref.view(np.uint32)), ("repeats", i)
+    with pytest.raises(ValueError):
+        arena.sssp_batch(g, s, some, np.zeros(3, np.float32))
+
+
 def test_sssp_batch_matches_plain(arena, oracle):
     """Batched and single-root kernels give the same distances on a larger graph."""
     e, w, ref = oracle_graph(18)
@@ -541,16 +587,18 @@ def test_sssp_batch_matches_plain(arena, oracle):
     for r in roots:
         arena.sssp(g, sp, int(r))
         plain.append(arena.sssp_result(sp)[0])
+    hub = arena.sssp_hub_distances(g, roots)
     for delta, wm, wl, wf in ((0.1, 0.05, 0.2, 0.5), (0.0125, 0.05, 0.25, 0.5)):
         sb.delta, sb.mid_weight, sb.long_weight, sb.far_weight = delta, wm, wl, wf
-        arena.sssp_batch(g, sb, roots)
-        flushes = arena.sssp_batch_phase_ms(sb)[5]
-        say("sssp batch S18", delta, wm, wl, wf, "flushes", flushes)
-        assert flushes > 0
-        for i, r in enumerate(roots):
-            d, p = arena.sssp_batch_result(g, sb, i)
-            assert np.array_equal(d.view(np.uint32), plain[i].view(np.uint32)), (delta, wm, wl, wf, i)
-            assert oracle.sssp_check(ref, int(r), d, p) == 0
+        for offsets in (None, hub):
+            arena.sssp_batch(g, sb, roots, offsets)
+            flushes = arena.sssp_batch_phase_ms(sb)[5]
+            say("sssp batch S18", delta, wm, wl, wf, "lags", offsets is not None, "flushes", flushes)
+            assert flushes > 0
+            for i, r in enumerate(roots):
+                d, p = arena.sssp_batch_result(g, sb, i)
+                assert np.array_equal(d.view(np.uint32), plain[i].view(np.uint32)), (delta, wm, wl, wf, i)
+                assert oracle.sssp_check(ref, int(r), d, p) == 0
 
 
 def test_sssp_batch_diagnostics(arena, oracle):

@@ -87,12 +87,15 @@ def test_s12_batched_all_roots(sess, oracle, s12):
         assert np.array_equal(sess.batch_depth(g, bb, i), depth), (i, int(r))
     width = int(sess.get_param("sssp_batch_max"))
     sb = sess.sssp_batch_create(g, 0.1)
-    for k in range(0, 64, width):
-        sess.sssp_batch(g, sb, roots[k:k + width])
-        for i, r in enumerate(roots[k:k + width]):
-            d, p = sess.sssp_batch_result(g, sb, i)
-            d_ref, _ = oracle.dijkstra_f32(ref, int(r))
-            assert np.array_equal(d.view(np.uint32), d_ref.view(np.uint32)), (k + i, int(r))
+    hub = sess.sssp_hub_distances(g, roots)
+    for offsets in (None, hub):  # shared buckets, then per-root lags from the hub distances
+        for k in range(0, 64, width):
+            sess.sssp_batch(g, sb, roots[k:k + width], None if offsets is None else offsets[k:k + width])
+            for i, r in enumerate(roots[k:k + width]):
+                d, p = sess.sssp_batch_result(g, sb, i)
+                d_ref, _ = oracle.dijkstra_f32(ref, int(r))
+                assert np.array_equal(d.view(np.uint32), d_ref.view(np.uint32)), (k + i, int(r))
+                assert oracle.sssp_check(ref, int(r), d, p) == 0, (k + i, int(r))
     sess.release(m)
 
 

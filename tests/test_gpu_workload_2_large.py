@@ -13,7 +13,9 @@ For each graph:
   gives the plain kernel's depths;
 - plain SSSP certified at delta 0.05 and 0.1 and equal across them; one
   32-root batched launch (and at SCALE 24, also a 16-root one) gives the plain
-  kernel's distances bit for bit, and further roots of it are certified.
+  kernel's distances bit for bit, and further roots of it are certified; the
+  same launch with per-root bucket lags from the roots' hub distances gives
+  the same distances.
 The certificate fixes the FP32 distances exactly (any label-correcting
 schedule ends at the same fixed point), so no full Dijkstra is run here.
 """
@@ -95,6 +97,11 @@ def test_sssp_plain_and_batched(big, oracle):
     s.release(m)
     certify = {int(r) for r in roots[big.nsssp:big.nsssp + 2]}
     _, got = sssp_batched(s, oracle, g, big.h, roots[:width], plain, certify, delta=0.025)
+    s.release(m)
+    # the same launch with per-root bucket lags from the hub distances: the same distances
+    same = dict(plain)
+    same.update(got)
+    sssp_batched(s, oracle, g, big.h, roots[:width], same, certify, delta=0.025, offsets="hub")
     s.release(m)
     if big.scale == 24:  # the 16-root batches of the batched workload
         plain.update(got)
