@@ -774,10 +774,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
         rbk.b[2][rt] = k + 2 >= 0 ? next_bound(b2, a.delta) : 0.0f;
         rbk.k[rt] = k + 1;
     };
-    // the screen threshold of a scan: max_r fl(b[L][r] + u_r) (block-uniform)
+    // the screen threshold of a scan: max_r fl(b[L][r] + u_r) over the roots
+    // with b[L][r] > 0 (block-uniform); a root with b[L][r] = 0 has not started,
+    // and its ranges [lo_r, hi_r) and [hi_r, b[L][r]) of the scan are empty
     auto screen_top = [&](int L) {
         f32 z = 0.0f;
-        for (int r = 0; r < kBatchRoots; ++r) z = fmaxf(z, rbk.b[L][r] + rbk.u[r]);
+        for (int r = 0; r < kBatchRoots; ++r)
+            if (rbk.b[L][r] > 0.0f) z = fmaxf(z, rbk.b[L][r] + rbk.u[r]);
         return z;
     };
     __syncthreads();

@@ -27,7 +27,8 @@ SSSP_DELTA = [0.025, 0.05, 0.1]
 SSSP_PULL = [0.02, 0.05, 0.2]
 # (delta, mid, long, far); far == long leaves the far class empty
 BATCH_SSSP = [(0.025, 0.075, 0.25, 0.5), (0.025, 0.05, 0.15, 0.3), (0.025, 0.075, 0.25, 0.25),
-              (0.0125, 0.05, 0.2, 0.5), (0.05, 0.1, 0.3, 0.6), (0.05, 0.075, 0.2, 0.4)]
+              (0.0125, 0.05, 0.2, 0.5), (0.05, 0.1, 0.3, 0.6), (0.05, 0.075, 0.2, 0.4),
+              (0.025, 0.1, 0.35, 0.35), (0.0375, 0.075, 0.25, 0.25), (0.025, 0.05, 0.2, 0.2)]
 
 
 def tuning_roots(sess, g, count, seed=2024):
