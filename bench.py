@@ -117,6 +117,10 @@ def main():
     ap.add_argument("--scale", type=int, default=26)
     ap.add_argument("--edgefactor", type=int, default=16)
     ap.add_argument("--delta", type=float, default=0.1, help="SSSP delta when not tuned")
+    ap.add_argument("--plain-delta", type=float, default=None,
+                    help="single-root SSSP delta when not tuned (default: --delta)")
+    ap.add_argument("--sssp-pull", type=float, default=None,
+                    help="single-root SSSP pull ratio when not tuned (default: the library's)")
     ap.add_argument("--long-weight", type=float, default=0.15,
                     help="batched SSSP long-edge threshold (deferred over windows this wide) when not tuned")
     ap.add_argument("--mid-weight", type=float, default=0.05,
@@ -173,6 +177,10 @@ def main():
         sess.release(m)
     tuned = None
     delta_plain = delta_batch = args.delta
+    if args.plain_delta is not None:
+        delta_plain = args.plain_delta
+    if args.sssp_pull is not None:
+        sess.set_param("sssp_pull", args.sssp_pull)
     long_weight, mid_weight, far_weight = args.long_weight, args.mid_weight, args.far_weight
     if not args.no_tune:
         from g500 import autotune
@@ -332,10 +340,19 @@ def main():
             f"sssp ms per step {np.median(ts)*1e3:.1f} ({ns} launches)")
         edges_total = 2.0 * me.sum()
         kern_time = tb.sum() + ts.sum()
+        # Shared-pass bytes of one launch, the least a batched launch moves when its roots share
+        # every row pass: the union component's rows once (BFS: 8 B of columns per undirected edge,
+        # SSSP: 16 B of columns and weights) plus the roots' own state, which does not shrink with
+        # sharing (BFS: a 64-bit root mask probed per entry, 16 B per edge, and read, written and
+        # queued per vertex, 24 B; SSSP: per root, the 4-byte target probe of both directed entries
+        # of an edge, 8 B, and the 16-byte label per reached vertex).
+        mu, nu = float(m_c.max()), float(n_c.max())
         kern["batch_bfs_kernel"] = {"bytes": float((16.0 * me + 16.0 * nv).mean()) / nb,
+                                    "shared_pass": 24.0 * mu + 24.0 * nu,
                                     "time": float(tb.mean()) / nb, "total": float(tb.sum()),
                                     "units": f"{BFS_GROUP} roots"}
         kern["sssp_batch_kernel"] = {"bytes": float((24.0 * me + 16.0 * nv).mean()) / ns,
+                                     "shared_pass": 16.0 * mu + float((8.0 * me + 16.0 * nv).mean()) / ns,
                                      "time": float(ts.mean()) / ns, "total": float(ts.sum()),
                                      "units": f"{SSSP_GROUP} roots"}
         plain = None
@@ -378,10 +395,17 @@ def main():
             out["traffic_frac"] = round(t / k["time"] / HBM_PEAK, 4)
             out["traffic_source"] = (f"profiles/{os.path.basename(tpath)}: FETCH_SIZE x2 + WRITE_SIZE per dispatch, "
                                      f"{traffic.get('commit', 'commit not recorded')}")
-        if name.startswith("batch") or name.startswith("sssp_batch"):
-            out["note"] = ("algorithmic bytes = per-root bytes (BFS 16 m_c + 16 n_c, SSSP 24 m_c + 16 n_c) summed "
-                           "over the roots of a launch; the roots share one pass over the CSR, so the figure "
-                           "can exceed what one traversal moves (traffic_frac is the bandwidth actually used)")
+        if "shared_pass" in k:
+            sp = k["shared_pass"] / k["time"]
+            out["shared_pass_bytes"] = k["shared_pass"]
+            out["shared_pass_gbs"] = round(sp / 1e9, 2)
+            out["shared_pass_frac"] = round(sp / HBM_PEAK, 4)
+            out["note"] = ("achieved/frac: per-root algorithmic bytes (BFS 16 m_c + 16 n_c, SSSP 24 m_c + 16 n_c) "
+                           "summed over the roots of a launch; the roots share the row passes, so this can exceed "
+                           "what the launch must move. shared_pass_frac: the least a launch moves with every row "
+                           "pass shared (BFS 24 m + 24 n of the union component; SSSP 16 m of the union plus "
+                           "8 m_c + 16 n_c per root), at most 1 at HBM speed; traffic_frac: bytes actually moved "
+                           "(counters of the committed profile)")
         return out
 
     other = [k for k in kern if k != dom][0]
