@@ -47,6 +47,8 @@ struct Context {
     int relabel = 1;              // 1: CSR builds add the degree-ordered traversal copy
     int sssp_prune = 1;           // 1: and the SSSP copy without two-hop-dominated entries
     int sssp_screen = 0;          // 1: single-root SSSP relaxations check the settled bitmap before the probe
+    double batch_pull = 0.2;       // batched SSSP pulls its far-class flush when the window's items >= this share
+                                   // of the vertices with entries (0: never)
     double sssp_pull = 0.05;       // single-root SSSP pulls a bucket's heavy relaxations when the bucket's light
                                   // items reach this share of the unsettled vertices (0: always push)
 };

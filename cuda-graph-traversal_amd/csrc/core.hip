@@ -313,6 +313,7 @@ i32 g500_set_param(const char *name, f64 value) {
     else if (n == "sssp_prune") c.sssp_prune = (int)value;
     else if (n == "sssp_screen") c.sssp_screen = (int)value;
     else if (n == "sssp_pull") c.sssp_pull = value;
+    else if (n == "batch_pull") c.batch_pull = value;
     else return -1;
     return 0;
 }
@@ -336,6 +337,7 @@ i32 g500_get_param(const char *name, f64 *value) {
     else if (n == "sssp_prune") *value = c.sssp_prune;
     else if (n == "sssp_screen") *value = c.sssp_screen;
     else if (n == "sssp_pull") *value = c.sssp_pull;
+    else if (n == "batch_pull") *value = c.batch_pull;
     else if (n == "num_cus") *value = c.num_cus;
     else if (n == "sssp_batch_max") *value = kSsspBatchRoots;
     else return -1;

@@ -71,6 +71,10 @@ struct SsspBatchCtl {
     CounterLine<u32> llen[kDefer][2][2];  // deferred queue lengths [class][set][tier 0-1], set = the class's flush parity
     CounterLine<u64> lhub[kDefer][2];     // deferred queue hub counters [class][set]
     alignas(256) u32 rmin[3][kBatchRoots];  // min step: smallest open distance bits per root, per slot
+    CounterLine<u32> plen;        // pull flush: vertices handed from stage A to stage B
+    alignas(256) u32 rw[kDefer][2];   // roots that some source settled in the class's window [class][set]
+    u32 wcnt[2][kBatchRoots];     // far class: sources each root settled in the window [set][root]
+    u64 pull_work[4];             // pull flushes, live (vertex, root) pairs, (vertex, entry) scan steps, stage-B vertices
     u32 nnext[3];                 // 1: a label entered [hi, hi + delta) (the bucket is not empty), per slot
     alignas(256) u64 buckets, light_iters, improvements, scans;
     u64 flushes;
@@ -147,6 +151,9 @@ struct SsspBatchArgs {
     f32 uoff[kBatchRoots];  // fl(lag[r] * delta): the reduced-value offset of root r
     i32 nroots;
     f32 delta;
+    f32 pull_ratio;    // the far class is pulled when its window items >= pull_ratio * nact (0: never)
+    u32 pull_roots;    // roots a pull may serve: those with an offset (in the hub's component)
+    u64 nact;          // vertices that can have entries: [0, nact) (the degree order puts isolated ones last)
     f32 dbound;        // distances at or above it set error 32 (+inf without the pruned copy)
     f32 dw[kDefer];    // class floor weights delta <= W_0 (mid) <= W_1 (long) <= W_2 (far)
 };
@@ -509,12 +516,181 @@ __device__ __forceinline__ void bsegment(const SsspBatchArgs &a, u32 u, u64 &b, 
 // one entry read the same column and weight and relax k labels in the same
 // 128-byte line of the target.
 
+// Pull form of the far class's flush, for windows whose items are a large
+// share of the graph. Every vertex v with a root r that is not final at v,
+// that some source of the window settled (bit r of rw) and whose probe is
+// above fl(e_r + W_c) (the smallest candidate a class entry from the window
+// can give; e_r is the class epoch of root r) scans the class segment of its
+// own row, lightest first. The SSSP rows are symmetric, so entry (v, u, w)
+// stands for the pushed relaxation of (u, v, w): when u's window mask
+// dpend[u] holds r, the candidate is fl(d_r(u) + w) with d_r(u) final. Root r
+// stops at the first entry with fl(e_r + w) >= its best candidate so far:
+// every later candidate comes from a source settled at >= e_r through an
+// entry at least as heavy, and rounding is monotone. A half wave takes one
+// vertex, one lane per root, so an entry's source line is read once for all
+// 32 roots. A vertex whose roots still scan after kPullLane entries goes to a
+// list that whole waves finish (two entries per step).
+constexpr u32 kPullLane = 32;
+
+// Lane r of a half wave holding v lowers root r's probe of v to `best`
+// (parent `par`) and queues v for the light steps when it lands below
+// b[lvl + 1][r]; `mine`: this lane commits.
+__device__ __forceinline__ void pull_commit(const SsspBatchArgs &a, const RootBuckets &rb, u32 v, bool mine, int r,
+                                            u32 p, u32 best, u32 par, const BTarget &t, u64 &improved, WaveQ &wq) {
+    bool lowered = false;
+    if (mine && best < p) {
+        const u32 prev = atomicMin(&a.dl[(u64)v * kBatchRoots + r], best);
+        if (prev > best) {
+            lowered = true;
+            const f32 d = __uint_as_float(best);
+            atomicMin((unsigned long long *)&a.lab[(u64)v * kBatchRoots + r], ((unsigned long long)best << 32) | par);
+            atomicMin(&a.vmin[v], __float_as_uint(d + rb.u[r]));
+            if (prev >= kInfBits) atomicOr(&((u32 *)a.openw)[v >> 5], 1u << (v & 31));
+            if (d < rb.b[t.lvl][r]) atomicOr(&a.ctl->bar.error, 16u);
+        }
+    }
+    const u64 lm = ballot(lowered);
+    if (!lm) return;
+    improved += (u64)__popcll(lm);
+    const u64 om = ballot(lowered && __uint_as_float(best) < rb.b[t.lvl + 1][r]);
+    const u32 half = (u32)(om >> (lane_id() & 32u));
+    push_item(a, v, (lane_id() & 31u) == 0 ? half : 0u, t, wq);
+}
+
+// Stage A: a half wave per vertex of [0, nact), at most kPullLane entries.
+template <int C>
+__device__ __forceinline__ void pull_stage_a(const SsspBatchArgs &a, const RootBuckets &rb, u32 rw, const BTarget &t,
+                                             u64 &improved, WaveQ &wq, u64 *pw) {
+    constexpr int K = kSegDef + C;
+    const u32 lane = lane_id();
+    const int r = (int)(lane & 31u), hl = (int)(lane >> 5);
+    const u64 wave_gid = wave_index(), nwaves = (u64)gridDim.x * blockDim.x / kWave;
+    const f32 er = rb.ep[C][r];
+    const u32 lowb = __float_as_uint(er + a.dw[C]);
+    const bool inw = (rw >> r) & 1u;
+    SsspBatchCtl *ctl = a.ctl;
+    for (u64 v0 = wave_gid * 2; v0 < a.nact; v0 += nwaves * 2) {
+        const u64 v = v0 + (u64)hl;
+        const bool valid = v < a.nact;
+        const u32 fin = valid ? a.smask[v] : ~0u;
+        const u32 p = valid ? a.dl[v * kBatchRoots + r] : 0u;
+        bool act = valid && inw && !((fin >> r) & 1u) && p > lowb;
+        const u64 lm = ballot(act);
+        if (!lm) continue;
+        u64 b = 0, e = 0;
+        if ((u32)(lm >> (lane & 32u))) bsegment<K>(a, (u32)v, b, e);
+        const u64 cap = min(e, b + kPullLane);
+        u32 best = p, par = 0;
+        bool cont = false;  // still scanning at the cap, with entries left
+        u64 steps = 0;
+        for (u64 k = b;; ++k) {
+            if (act && k >= cap) {
+                cont = k < e;
+                act = false;
+            }
+            const f32 wk = act ? a.w[k] : 0.0f;
+            act = act && __float_as_uint(er + wk) < best;
+            const u64 am = ballot(act);
+            if (!am) break;
+            steps += (u64)((u32)am != 0) + (u64)((u32)(am >> 32) != 0);
+            if (act) {
+                const u32 u = a.col[k];
+                if ((a.dpend[C][u] >> r) & 1u) {
+                    const u32 cb = __float_as_uint(__uint_as_float(a.dl[(u64)u * kBatchRoots + r]) + wk);
+                    if (cb < best) {
+                        best = cb;
+                        par = u;
+                    }
+                }
+            }
+        }
+        const u64 cm = ballot(cont);
+        if ((lane & 31u) == 0 && (u32)(cm >> (lane & 32u))) {
+            const u32 at = atomicAdd(&ctl->plen.v, 1u);
+            if (at < a.n) a.hqh[at] = (u32)v;
+            else atomicOr(&ctl->bar.error, 2u);
+        }
+        pull_commit(a, rb, (u32)v, valid, r, p, best, par, t, improved, wq);
+        if (lane == 0) {
+            atomicAdd((unsigned long long *)&pw[1], (unsigned long long)__popcll(lm));
+            atomicAdd((unsigned long long *)&pw[2], (unsigned long long)steps);
+            atomicAdd((unsigned long long *)&pw[3], (unsigned long long)(((u32)cm != 0) + ((u32)(cm >> 32) != 0)));
+        }
+    }
+}
+
+// Stage B: a whole wave per listed vertex, from entry kPullLane on, the two
+// halves on alternate entries; the halves' best candidates are merged at the end.
+template <int C>
+__device__ __forceinline__ void pull_stage_b(const SsspBatchArgs &a, const RootBuckets &rb, u32 rw, u32 count,
+                                             const BTarget &t, u64 &improved, WaveQ &wq, u64 *pw) {
+    constexpr int K = kSegDef + C;
+    const u32 lane = lane_id();
+    const int r = (int)(lane & 31u), hl = (int)(lane >> 5);
+    const u64 wave_gid = wave_index(), nwaves = (u64)gridDim.x * blockDim.x / kWave;
+    const f32 er = rb.ep[C][r];
+    const u32 lowb = __float_as_uint(er + a.dw[C]);
+    const bool inw = (rw >> r) & 1u;
+    for (u64 i = wave_gid; i < count; i += nwaves) {
+        const u32 v = a.hqh[i];
+        if (v >= a.nact) {
+            if (lane == 0) atomicOr(&a.ctl->bar.error, 4u);
+            continue;
+        }
+        const u32 fin = a.smask[v];
+        const u32 p = a.dl[(u64)v * kBatchRoots + r];
+        bool act = inw && !((fin >> r) & 1u) && p > lowb;
+        if (!ballot(act)) continue;
+        u64 b, e;
+        bsegment<K>(a, v, b, e);
+        u32 best = p, par = 0;
+        u64 steps = 0;
+        for (u64 k0 = b + kPullLane;; k0 += 2) {
+            const u64 k = k0 + (u64)hl;
+            if (act && k >= e) act = false;
+            const f32 wk = act ? a.w[k] : 0.0f;
+            act = act && __float_as_uint(er + wk) < best;
+            const u64 am = ballot(act);
+            if (!am) break;
+            steps += (u64)((u32)am != 0) + (u64)((u32)(am >> 32) != 0);
+            if (act) {
+                const u32 u = a.col[k];
+                if ((a.dpend[C][u] >> r) & 1u) {
+                    const u32 cb = __float_as_uint(__uint_as_float(a.dl[(u64)u * kBatchRoots + r]) + wk);
+                    if (cb < best) {
+                        best = cb;
+                        par = u;
+                    }
+                }
+            }
+        }
+        const u32 ob = __shfl_xor(best, 32), op = __shfl_xor(par, 32);
+        if (ob < best) {
+            best = ob;
+            par = op;
+        }
+        pull_commit(a, rb, v, hl == 0, r, p, best, par, t, improved, wq);
+        if (lane == 0) atomicAdd((unsigned long long *)&pw[2], (unsigned long long)steps);
+    }
+}
+
+// After a pull, clears the window masks of the class's queued sources.
+template <int C>
+__device__ __forceinline__ void pull_clear(const SsspBatchArgs &a, u32 n0, u32 n1, u64 nh) {
+    const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
+    for (u64 i = gtid; i < n0 + n1 + nh; i += gthreads) {
+        const u32 u = i < n0 ? a.dq[C][0][i] : (i < (u64)n0 + n1 ? a.dq[C][1][i - n0] : a.dq[C][2][i - n0 - n1]);
+        if (u < a.n) a.dpend[C][u] = 0;
+    }
+}
+
 // Tier 0 (segments <= 32 entries): a wave takes 64 items and walks their
 // concatenated pair ranges 64 pairs at a time. Light and long items read their
 // mask from pend and clear it; medium items carry it in the list entry.
 template <int K>
 __device__ __forceinline__ void group_tier(const SsspBatchArgs &a, u32 *pend, const u32 *lq, const u64 *hq, u32 count,
-                           const BTarget &t, u64 &improved, volatile int *slot, u64 *wk, WaveQ &wq) {
+                           const BTarget &t, u64 &improved, volatile int *slot, u64 *wk, WaveQ &wq,
+                           u32 keep = ~0u) {
     const u64 gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
     const u64 wave_gid = wave_index(), nwaves = gthreads / kWave;
@@ -526,8 +702,8 @@ __device__ __forceinline__ void group_tier(const SsspBatchArgs &a, u32 *pend, co
             if (K != kSegMedium) {
                 u = lq[i];
                 if (u < a.n) {
-                    M = pend[u];
-                    pend[u] = 0;
+                    M = pend[u] & keep;
+                    if (keep == ~0u) pend[u] = 0;
                 } else {
                     atomicOr(&a.ctl->bar.error, 4u);
                 }
@@ -606,7 +782,7 @@ __device__ __forceinline__ PairLanes pair_lanes(u32 M) {
 // Tier 1 (segments <= 1024 entries): a wave per item.
 template <int K>
 __device__ __forceinline__ void wave_tier(const SsspBatchArgs &a, u32 *pend, const u32 *lq, const u64 *hq, u32 count,
-                          const BTarget &t, u64 &improved, u64 *wk, WaveQ &wq) {
+                          const BTarget &t, u64 &improved, u64 *wk, WaveQ &wq, u32 keep = ~0u) {
     const u64 gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
     const u64 wave_gid = wave_index(), nwaves = gthreads / kWave;
@@ -618,8 +794,8 @@ __device__ __forceinline__ void wave_tier(const SsspBatchArgs &a, u32 *pend, con
                 if (lane == 0) atomicOr(&a.ctl->bar.error, 4u);
                 continue;
             }
-            M = __builtin_amdgcn_readfirstlane(pend[u]);
-            if (lane == 0) pend[u] = 0;
+            M = __builtin_amdgcn_readfirstlane(pend[u]) & keep;
+            if (lane == 0 && keep == ~0u) pend[u] = 0;
         } else {
             const u64 it = hq[i];
             u = (u32)it;
@@ -661,7 +837,8 @@ __device__ __forceinline__ void wave_tier(const SsspBatchArgs &a, u32 *pend, con
 // chunks); medium hubs from `aux`.
 template <int K>
 __device__ __forceinline__ void hub_tier(const SsspBatchArgs &a, const u32 *hv, const u64 *hoff, const u32 *hlen, u32 *aux,
-                         u32 *pend, u64 packed, const BTarget &t, u64 &improved, u64 *wk, WaveQ &wq) {
+                         u32 *pend, u64 packed, const BTarget &t, u64 &improved, u64 *wk, WaveQ &wq,
+                         u32 keep = ~0u) {
     const u64 gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
     const u64 wave_gid = wave_index(), nwaves = gthreads / kWave;
@@ -675,7 +852,7 @@ __device__ __forceinline__ void hub_tier(const SsspBatchArgs &a, const u32 *hv, 
             if (lane == 0) atomicOr(&a.ctl->bar.error, 4u);
             continue;
         }
-        const u32 M = __builtin_amdgcn_readfirstlane(K != kSegMedium ? pend[u] : aux[i]);
+        const u32 M = __builtin_amdgcn_readfirstlane(K != kSegMedium ? pend[u] & keep : aux[i]);
         const u64 local = p0 - hoff[i];
         const u32 len = hlen[i];
         u64 b, e;
@@ -708,7 +885,7 @@ __device__ __forceinline__ void hub_tier(const SsspBatchArgs &a, const u32 *hv, 
         if (K != kSegMedium && lane == 0) {
             const u32 chunks = (len + (u32)kHubChunk - 1) / (u32)kHubChunk;
             if (atomicAdd(&aux[i], 1u) + 1 == chunks) {  // every chunk has read the mask
-                pend[u] = 0;
+                if (keep == ~0u) pend[u] = 0;
                 aux[i] = 0;
             }
         }
@@ -744,6 +921,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
     };
     // per-root bucket state: thread r < 32 of every block keeps root r's entries
     __shared__ RootBuckets rbk;
+    __shared__ u32 rws[kDefer];       // dense step: roots this block deferred into each class
+    __shared__ u32 wcs[kBatchRoots];  // dense step: far-class sources this block deferred, per root
+    __shared__ u64 pws[4];            // pull flush counts of this block (see g500_sssp_batch_pull_work)
+    if (threadIdx.x < 4) pws[threadIdx.x] = 0;
     const int rt = (int)threadIdx.x;  // the root this thread keeps (when < kBatchRoots)
     if (rt < kBatchRoots) {
         // selects, not a runtime index into the argument struct (which would copy it to scratch)
@@ -969,6 +1150,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
             for (int c = 0; c < kDefer; ++c)
                 if (!lpending[c]) rbk.ep[c][rt] = rbk.b[0][rt];
         }
+        if (threadIdx.x < (u32)kDefer) rws[threadIdx.x] = 0u;
+        if (rt < kBatchRoots) wcs[rt] = 0u;
+        if (gtid == 0) st_relaxed(&ctl->plen.v, 0u);
         __syncthreads();
         {
             const BTarget t = target(x, 1);
@@ -986,6 +1170,19 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                     const u32 old = a.dpend[c][v];
                     a.dpend[c][v] = old | H;
                     q = old == 0;
+                    if (c == kDefer - 1) atomicOr(&rws[c], H);  // the window's roots (pull flushes read them)
+                }
+                if (c == kDefer - 1) {  // far-class sources per root of the window (pull flushes read them)
+                    const u32 hf = (H && len) ? H : 0u;
+                    u32 any = hf;
+#pragma unroll
+                    for (int o = 32; o >= 1; o >>= 1) any |= __shfl_xor(any, o);
+                    while (any) {
+                        const int r = __ffs(any) - 1;
+                        any &= any - 1;
+                        const u64 bm = ballot((hf >> r) & 1u);
+                        if (lane == 0) atomicAdd(&wcs[r], (u32)__popcll(bm));
+                    }
                 }
                 q_push32<kQDef0 + 2 * c>(a, wq, q && btier(len) == 0, v);
                 q_push32<kQDef0 + 2 * c + 1>(a, wq, q && btier(len) == 1, v);
@@ -1033,7 +1230,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                                              (u32)v, len, a.n, &ctl->bar.error);
                 if (hslot != ~0ull) a.hq_mask[hslot] = H;
             });
-            q_flush(a, wq);
+            q_flush(a, wq);  // ends with a block barrier: rws is complete
+            if (threadIdx.x == 0 && rws[kDefer - 1])
+                atomicOr(&ctl->rw[kDefer - 1][wq.ls[kDefer - 1]], rws[kDefer - 1]);
+            if (rt < kBatchRoots && wcs[rt]) atomicAdd(&ctl->wcnt[wq.ls[kDefer - 1]][rt], wcs[rt]);
             bsync(ctl, 3, gb, x);
             // The deferred queues are read between the two barriers of this
             // step: the next appends to these sets come after the second one.
@@ -1081,6 +1281,23 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                     if (fl[c]) items += (u64)lc[c][0] + lc[c][1] + hub_count(lh[c]);
                 }
                 if (anyf) {
+                    // the far class is pulled when its window's items are a large share of the graph
+                    constexpr int cf = kDefer - 1;
+                    // A pull serves the roots at their peak: those with an offset (in the hub's
+                    // component) that settled at least pull_ratio * nact sources in the window. A
+                    // root that has reached few vertices (just started, or in another component)
+                    // would keep every other vertex scanning its whole segment; it is pushed.
+                    const u32 rwf = ld_relaxed(&ctl->rw[cf][wq.ls[cf]]);
+                    u32 elig = 0;  // roots that settled at least pull_ratio * nact sources in the window
+                    if (fl[cf] && a.pull_ratio > 0.0f) {
+                        const f32 need = a.pull_ratio * (f32)a.nact;
+                        for (int r = 0; r < a.nroots; ++r)
+                            if (((a.pull_roots >> r) & 1u) && (f32)ld_relaxed(&ctl->wcnt[wq.ls[cf]][r]) >= need)
+                                elig |= 1u << r;
+                    }
+                    const bool pull = elig != 0u;
+                    // the window's other roots are pushed, their masks kept for the pull (cleared after it)
+                    const u32 keep = pull ? ~elig : ~0u;
                     if (gtid == 0) {  // a class's other set was last read by its previous flush
 #pragma unroll
                         for (int c = 0; c < kDefer; ++c)
@@ -1089,24 +1306,39 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                                 st_relaxed(&ctl->llen[c][o][0].v, 0u);
                                 st_relaxed(&ctl->llen[c][o][1].v, 0u);
                                 st_relaxed(&ctl->lhub[c][o].v, 0ull);
+                                st_relaxed(&ctl->rw[c][o], 0u);
+                                if (c == kDefer - 1)
+                                    for (int r = 0; r < kBatchRoots; ++r) st_relaxed(&ctl->wcnt[o][r], 0u);
                             }
                         atomicAdd((unsigned long long *)&ctl->flushes, 1ull);
+                        if (pull) pws[0] += 1;
                     }
                     auto flush_class = [&](auto cls) {
                         constexpr int c = decltype(cls)::value;
-                        if (!fl[c]) return;
+                        if (!fl[c] || (c == cf && pull && (rwf & keep) == 0u)) return;
+                        const u32 kc = c == cf ? keep : ~0u;
                         group_tier<kSegDef + c>(a, a.dpend[c], a.dq[c][0], nullptr, lc[c][0], t, improved, slot,
-                                                wks[3 + c], wq);
+                                                wks[3 + c], wq, kc);
                         wave_tier<kSegDef + c>(a, a.dpend[c], a.dq[c][1], nullptr, lc[c][1], t, improved, wks[3 + c],
-                                               wq);
+                                               wq, kc);
                         hub_tier<kSegDef + c>(a, a.dq[c][2], a.dq_off[c], a.dq_len[c], a.dq_done[c], a.dpend[c], lh[c],
-                                              t, improved, wks[3 + c], wq);
+                                              t, improved, wks[3 + c], wq, kc);
                     };
                     flush_class(std::integral_constant<int, 0>());
                     flush_class(std::integral_constant<int, 1>());
                     flush_class(std::integral_constant<int, 2>());
+                    static_assert(kDefer == 3, "one flush_class call per class");
+                    if (pull) pull_stage_a<cf>(a, rbk, elig, t, improved, wq, pws);
                     q_flush(a, wq);
                     bsync(ctl, 7, gb, x);
+                    if (pull) {
+                        pull_stage_b<cf>(a, rbk, elig, min(ld_relaxed(&ctl->plen.v), (u32)a.n), t, improved, wq,
+                                         pws);
+                        q_flush(a, wq);
+                        bsync(ctl, 8, gb, x);
+                        pull_clear<cf>(a, lc[cf][0], lc[cf][1], hub_count(lh[cf]));
+                        bsync(ctl, 9, gb, x);
+                    }
                     tick(3, items);
 #pragma unroll
                     for (int c = 0; c < kDefer; ++c)
@@ -1222,6 +1454,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
     }
     if (lane == 0 && improved) atomicAdd((unsigned long long *)&ctl->improvements, (unsigned long long)improved);
     __syncthreads();
+    if (threadIdx.x < 4 && pws[threadIdx.x])
+        atomicAdd((unsigned long long *)&ctl->pull_work[threadIdx.x], (unsigned long long)pws[threadIdx.x]);
     if (threadIdx.x < 3 * (3 + kDefer) && wks[threadIdx.x / 3][threadIdx.x % 3])
         atomicAdd((unsigned long long *)&ctl->work[threadIdx.x / 3][threadIdx.x % 3],
                   (unsigned long long)wks[threadIdx.x / 3][threadIdx.x % 3]);
@@ -1398,9 +1632,11 @@ void sssp_batch_prepare(const Oned_Graph *g, Sssp_Batch_State *s, hipStream_t st
 // Bucket lags from distance offsets: the root with the largest offset runs
 // without lag, root r lags round((max - offset_r) / delta) buckets (at most
 // kMaxLag); roots without an offset (null, negative or not finite) lag none.
-// Lags only order the work: every lag gives the same distances.
+// Lags only order the work: every lag gives the same distances. with_offset:
+// the roots that have an offset (the ones pulled flushes may serve).
 constexpr u32 kMaxLag = 1u << 16;
-void sssp_batch_lags(const f32 *offsets, i32 nroots, f32 delta, u32 *lag, f32 *uoff) {
+void sssp_batch_lags(const f32 *offsets, i32 nroots, f32 delta, u32 *lag, f32 *uoff, u32 *with_offset) {
+    *with_offset = 0;
     f64 omax = -1.0;
     for (i32 i = 0; offsets && i < nroots; ++i)
         if (std::isfinite(offsets[i]) && offsets[i] >= 0.0f) omax = std::max(omax, (f64)offsets[i]);
@@ -1409,6 +1645,7 @@ void sssp_batch_lags(const f32 *offsets, i32 nroots, f32 delta, u32 *lag, f32 *u
         if (omax >= 0.0 && i < nroots && std::isfinite(offsets[i]) && offsets[i] >= 0.0f) {
             const f64 q = std::floor((omax - (f64)offsets[i]) / (f64)delta + 0.5);
             l = q >= (f64)kMaxLag ? kMaxLag : (u32)q;
+            *with_offset |= 1u << i;
         }
         lag[i] = l;
         uoff[i] = (f32)((f64)l * (f64)delta);
@@ -1507,7 +1744,9 @@ bool sssp_batch_enqueue(const Oned_Graph *g, Sssp_Batch_State *s, const u64 *roo
     for (i32 i = 0; i < nroots; ++i) a.roots[i] = roots[i];
     a.nroots = nroots;
     a.delta = s->delta;
-    sssp_batch_lags(offsets, nroots, s->delta, a.lag, a.uoff);
+    a.pull_ratio = (f32)c.batch_pull;
+    a.nact = (g->perm && g->notisolated) ? std::min<u64>(g->notisolated, n) : n;
+    sssp_batch_lags(offsets, nroots, s->delta, a.lag, a.uoff, &a.pull_roots);
     a.dbound = g->rowstarts_s ? kPruneDistBound : __builtin_inff();
     a.dw[0] = wmid;
     a.dw[1] = wlong;
@@ -1660,6 +1899,12 @@ void g500_sssp_batch_work(const Sssp_Batch_State *s, u64 *out) {
     g500_copy_to_host(&h, s->ctl, sizeof(h));
     for (int k = 0; k < 3 + kDefer; ++k)
         for (int j = 0; j < 3; ++j) out[k * 3 + j] = h.work[k][j];
+}
+
+void g500_sssp_batch_pull_work(const Sssp_Batch_State *s, u64 *out) {
+    static SsspBatchCtl h;
+    g500_copy_to_host(&h, s->ctl, sizeof(h));
+    for (int i = 0; i < 4; ++i) out[i] = h.pull_work[i];
 }
 
 void g500_sssp_batch_run_stats(const Sssp_Batch_State *s, u64 *buckets, u64 *light_iterations, u64 *relaxations) {

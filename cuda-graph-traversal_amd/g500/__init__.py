@@ -141,6 +141,7 @@ _SIGS = {
     "g500_sssp_batch_run_stats": (None, [P(SsspBatchState), P(u64), P(u64), P(u64)]),
     "g500_sssp_batch_phase_ms": (None, [P(SsspBatchState), P(f64)]),
     "g500_sssp_batch_work": (None, [P(SsspBatchState), P(u64)]),
+    "g500_sssp_batch_pull_work": (None, [P(SsspBatchState), P(u64)]),
     "g500_sssp_batch_step_log": (i32, [P(SsspBatchState), P(i32), P(u64), P(f64), i32]),
     "g500_server_create": (vp, [P(WorkerState), i32, u64]),
     "g500_server_add_graph": (i32, [vp, P(OnedGraph)]),
@@ -415,6 +416,11 @@ class Session:
         self.lib.g500_sssp_batch_work(C.byref(s), out)
         return {k: tuple(int(x) for x in out[3 * i:3 * i + 3])
                 for i, k in enumerate(("light", "medium", "scan", "mid", "long", "far"))}
+
+    def sssp_batch_pull_work(self, s):
+        out = (u64 * 4)()
+        self.lib.g500_sssp_batch_pull_work(C.byref(s), out)
+        return dict(zip(("pulls", "live_pairs", "scan_steps", "long_vertices"), (int(x) for x in out)))
 
     def sssp_batch_stats(self, s):
         b, li, r = u64(), u64(), u64()

@@ -9,8 +9,10 @@ that a benchmark measures. Candidates:
   plain SSSP:    delta, sssp_block, sssp_pull (the bucket share above which
                  heavy relaxations are pulled)
   batched SSSP:  (delta, mid_weight, long_weight, far_weight) candidates, then
-                 batch_sssp_block for the best of them (with per-root bucket
-                 lags from the tuning roots' hub distances unless lags=False)
+                 batch_sssp_block and batch_pull (the window share above which
+                 far-class flushes are pulled) for the best of them (with
+                 per-root bucket lags from the tuning roots' hub distances
+                 unless lags=False)
 Every candidate gives the same depths and distances (the tuner changes only
 schedules), so the choice is by time alone. Plain BFS candidates are timed on
 16 roots and plain SSSP ones on 4 (the mean of their times; a few roots whose
@@ -28,7 +30,9 @@ SSSP_PULL = [0.02, 0.05, 0.2]
 # (delta, mid, long, far); far == long leaves the far class empty
 BATCH_SSSP = [(0.025, 0.075, 0.25, 0.5), (0.025, 0.05, 0.15, 0.3), (0.025, 0.075, 0.25, 0.25),
               (0.0125, 0.05, 0.2, 0.5), (0.05, 0.1, 0.3, 0.6), (0.05, 0.075, 0.2, 0.4),
-              (0.025, 0.1, 0.35, 0.35), (0.0375, 0.075, 0.25, 0.25), (0.025, 0.05, 0.2, 0.2)]
+              (0.025, 0.1, 0.35, 0.35), (0.0375, 0.075, 0.25, 0.25), (0.025, 0.05, 0.2, 0.2),
+              (0.025, 0.05, 0.15, 0.15), (0.025, 0.05, 0.1, 0.1)]
+BATCH_PULL = [0.0, 0.1, 0.2, 0.3]
 
 
 def tuning_roots(sess, g, count, seed=2024):
@@ -70,7 +74,7 @@ def _tune(sess, g, batch, roots_plain, roots_bfs, seed, say, lags):
     plain = roots[:roots_plain]
     trials = []
     old = {k: sess.get_param(k) for k in ("bfs_alpha", "bfs_beta", "bfs_block", "sssp_block", "batch_alpha",
-                                          "batch_heavy_degree", "batch_bfs_block", "batch_sssp_block")}
+                                          "batch_heavy_degree", "batch_bfs_block", "batch_sssp_block", "batch_pull")}
 
     b = sess.bfs_create(g)
     res = []
@@ -144,13 +148,21 @@ def _tune(sess, g, batch, roots_plain, roots_bfs, seed, say, lags):
         best_bs = _best(res)
         best_bs.setdefault("batch_sssp_block", int(old["batch_sssp_block"]))
         sess.set_param("batch_sssp_block", best_bs["batch_sssp_block"])
+        if lags:  # pulled flushes serve only roots with an offset
+            for share in BATCH_PULL:
+                sess.set_param("batch_pull", share)
+                res.append(dict(best_bs, batch_pull=share, ms=sess.sssp_batch(g, sb, roots[:width], offs)))
+            best_bs = _best(res)
+        best_bs.setdefault("batch_pull", float(old["batch_pull"]))
+        sess.set_param("batch_pull", best_bs["batch_pull"])
         trials += res
         say(f"[tune] batched sssp {best_bs}")
         params.update({"batch_alpha": best_bb["batch_alpha"], "batch_heavy_degree": best_bb["batch_heavy_degree"],
                        "batch_bfs_block": best_bb["batch_bfs_block"], "batch_sssp_block": best_bs["batch_sssp_block"],
                        "batch_sssp_delta": best_bs["delta"], "batch_sssp_mid_weight": best_bs["mid_weight"],
-                       "batch_sssp_long_weight": best_bs["long_weight"], "batch_sssp_far_weight": best_bs["far_weight"]})
+                       "batch_sssp_long_weight": best_bs["long_weight"], "batch_sssp_far_weight": best_bs["far_weight"],
+                       "batch_pull": best_bs["batch_pull"]})
     else:
-        for k in ("batch_alpha", "batch_heavy_degree", "batch_bfs_block", "batch_sssp_block"):
+        for k in ("batch_alpha", "batch_heavy_degree", "batch_bfs_block", "batch_sssp_block", "batch_pull"):
             sess.set_param(k, old[k])
     return {"params": params, "trials": trials, "tuning_roots": len(roots)}

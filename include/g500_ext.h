@@ -200,6 +200,11 @@ void g500_sssp_batch_phase_ms(const Sssp_Batch_State *s, f64 ms[9]);
  * 64-vertex words walked, vertices that read their probe line, vertices
  * settling a root. */
 void g500_sssp_batch_work(const Sssp_Batch_State *s, u64 out[18]);
+/* Pull-form far-class flushes of the last batched run (g500_set_param("batch_pull", share): a flush whose
+ * window items are at least share x the vertices with entries is pulled by the targets): out[0] pulled
+ * flushes, out[1] live (vertex, root) pairs, out[2] (vertex, entry) scan steps, out[3] vertices finished by
+ * whole waves. */
+void g500_sssp_batch_pull_work(const Sssp_Batch_State *s, u64 out[4]);
 /* Per-phase log of the last batched run (first 4096 phases): kind (0 light
  * step, 1 dense scan, 2 medium relaxations, 3 deferred flush, 4 min + collect
  * steps), items processed and device milliseconds. Returns the entries. */

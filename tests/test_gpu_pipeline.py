@@ -576,6 +576,41 @@ def test_sssp_batch_lags(arena, oracle, scale):
         arena.sssp_batch(g, s, some, np.zeros(3, np.float32))
 
 
+@pytest.mark.parametrize("scale", [10, 12, 16])
+def test_sssp_batch_pull(arena, oracle, scale):
+    """Pulled far-class flushes (every far flush pulled, or those of large
+    windows) give the oracle's distances bit for bit and parents that pass
+    the certificate; launches without offsets push every flush."""
+    e, w, ref = oracle_graph(scale)
+    g = arena.csr(arena.upload_tuples(e, w), 1 << scale)
+    roots = oracle.roots(ref)
+    width = int(arena.get_param("sssp_batch_max"))
+    rr = roots[:width]
+    hub = arena.sssp_hub_distances(g, rr)
+    d_ref = [oracle.dijkstra_f32(ref, int(r))[0] for r in rr]
+    s = arena.sssp_batch_create(g)
+    old = arena.get_param("batch_pull")
+    try:
+        for share in (1e-9, 0.05):
+            arena.set_param("batch_pull", share)
+            for delta, wm, wl, wf in ((0.02, 0.05, 0.2, 0.2), (0.1, 0.05, 0.2, 0.4), (0.02, 0.0, 0.0, 0.0)):
+                s.delta, s.mid_weight, s.long_weight, s.far_weight = delta, wm, wl, wf
+                for offsets in (None, hub):
+                    arena.sssp_batch(g, s, rr, offsets)
+                    pw = arena.sssp_batch_pull_work(s)
+                    say("sssp batch pull", scale, share, delta, "lags", offsets is not None, pw)
+                    if offsets is None:  # without offsets no root is known to share the hub's component
+                        assert pw["pulls"] == 0
+                    elif share < 1e-6 and wf > delta:
+                        assert pw["pulls"] > 0
+                    for i, root in enumerate(rr):
+                        d, p = arena.sssp_batch_result(g, s, i)
+                        assert np.array_equal(d.view(np.uint32), d_ref[i].view(np.uint32)), (share, delta, i)
+                        assert oracle.sssp_check(ref, int(root), d, p) == 0, (share, delta, i)
+    finally:
+        arena.set_param("batch_pull", old)
+
+
 def test_sssp_batch_matches_plain(arena, oracle):
     """Batched and single-root kernels give the same distances on a larger graph."""
     e, w, ref = oracle_graph(18)

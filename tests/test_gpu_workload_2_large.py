@@ -14,8 +14,8 @@ For each graph:
 - plain SSSP certified at delta 0.05 and 0.1 and equal across them; one
   32-root batched launch (and at SCALE 24, also a 16-root one) gives the plain
   kernel's distances bit for bit, and further roots of it are certified; the
-  same launch with per-root bucket lags from the roots' hub distances gives
-  the same distances.
+  same launch with per-root bucket lags from the roots' hub distances, and
+  with pulled far-class flushes, gives the same distances.
 The certificate fixes the FP32 distances exactly (any label-correcting
 schedule ends at the same fixed point), so no full Dijkstra is run here.
 """
@@ -102,6 +102,15 @@ def test_sssp_plain_and_batched(big, oracle):
     same = dict(plain)
     same.update(got)
     sssp_batched(s, oracle, g, big.h, roots[:width], same, certify, delta=0.025, offsets="hub")
+    s.release(m)
+    # and with the far-class flushes of large windows pulled by their targets
+    old = s.get_param("batch_pull")
+    s.set_param("batch_pull", 0.05)
+    try:
+        sb, _ = sssp_batched(s, oracle, g, big.h, roots[:width], same, certify, delta=0.025, offsets="hub")
+        say(f"S{big.scale} ef{big.ef}: pulled flushes {s.sssp_batch_pull_work(sb)}")
+    finally:
+        s.set_param("batch_pull", old)
     s.release(m)
     if big.scale == 24:  # the 16-root batches of the batched workload
         plain.update(got)
