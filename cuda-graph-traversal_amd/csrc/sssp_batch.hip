@@ -351,7 +351,11 @@ __device__ __forceinline__ void q_flush(const SsspBatchArgs &a, WaveQ &q) {
     __syncthreads();  // the LDS buffers and counts are reused by the next phase
 }
 
-__device__ __forceinline__ void push_item(const SsspBatchArgs &a, u32 v, u32 mask, const BTarget &t, WaveQ &wq) {
+// Queues v for the light step of target t with root mask `mask`; lw is v's
+// light-entry count (split[v]), read by the caller together with its other
+// loads of v (any value when mask is 0).
+__device__ __forceinline__ void push_item_lw(const SsspBatchArgs &a, u32 v, u32 mask, u32 lw, const BTarget &t,
+                                             WaveQ &wq) {
     const u64 wm = ballot(mask != 0);
     if (!wm) return;
     SsspBatchCtl *ctl = a.ctl;
@@ -360,17 +364,18 @@ __device__ __forceinline__ void push_item(const SsspBatchArgs &a, u32 v, u32 mas
         st_relaxed(&ctl->nnext[t.slot], 1u);
     // a vertex without light entries has nothing to do in light steps
     bool q = false;
-    u32 lw = 0;
-    if (mask) {
-        lw = a.split[v];
-        if (lw) q = atomicOr(&(t.buf ? a.pend[1] : a.pend[0])[v], mask) == 0;
-    }
+    if (!mask) lw = 0;
+    if (lw) q = atomicOr(&(t.buf ? a.pend[1] : a.pend[0])[v], mask) == 0;
     if (!ballot(q)) return;
     q_push32<kQLight0>(a, wq, q && btier(lw) == 0, v);
     q_push32<kQLight0 + 1>(a, wq, q && btier(lw) == 1, v);
     hub_append(&ctl->fhub[t.slot].v, t.buf ? a.fq[1][2] : a.fq[0][2], t.buf ? a.fq_off[1] : a.fq_off[0],
                t.buf ? a.fq_len[1] : a.fq_len[0], q && btier(lw) == 2, v, lw,
                a.n, &ctl->bar.error);
+}
+
+__device__ __forceinline__ void push_item(const SsspBatchArgs &a, u32 v, u32 mask, const BTarget &t, WaveQ &wq) {
+    push_item_lw(a, v, mask, mask ? a.split[v] : 0u, t, wq);
 }
 
 // A relaxation in two parts, so that a lane can have several relaxations in
@@ -704,6 +709,9 @@ __device__ __forceinline__ void group_tier(const SsspBatchArgs &a, u32 *pend, co
                 if (u < a.n) {
                     M = pend[u] & keep;
                     if (keep == ~0u) pend[u] = 0;
+                    // the row record does not depend on the mask: its load goes out with the mask's
+                    bsegment<K>(a, u, b, e);
+                    if (!M) b = e = 0;
                 } else {
                     atomicOr(&a.ctl->bar.error, 4u);
                 }
@@ -715,8 +723,8 @@ __device__ __forceinline__ void group_tier(const SsspBatchArgs &a, u32 *pend, co
                     atomicOr(&a.ctl->bar.error, 4u);
                     M = 0;
                 }
+                if (M) bsegment<K>(a, u, b, e);
             }
-            if (M) bsegment<K>(a, u, b, e);
         }
         const u32 pairs = (u32)(e - b) * (u32)__popc(M);
         const u32 incl = wave_inclusive_scan(pairs);
@@ -805,9 +813,9 @@ __device__ __forceinline__ void wave_tier(const SsspBatchArgs &a, u32 *pend, con
                 continue;
             }
         }
-        if (!M) continue;
         u64 b, e;
-        bsegment<K>(a, u, b, e);
+        bsegment<K>(a, u, b, e);  // before the mask test, so that its load goes out with the mask's
+        if (!M) continue;
         if (lane == 0) work_add(wk, 1, e - b, (e - b) * (u64)__popc(M));
         const PairLanes p = pair_lanes(M);
         const f32 du = p.active ? __uint_as_float(a.dl[(u64)u * kBatchRoots + p.r]) : 0.f;
@@ -1056,9 +1064,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                 const u64 word = w0 + (u64)j;
                 const u64 v = word * 64 + lane;
                 const bool isopen = (shfl64(open_l, j) >> lane) & 1;
+                // both loads go out together: the settled mask is read for every
+                // open vertex, not only for one the screen lets through
                 const u32 vm = isopen ? a.vmin[v] : 0xFFFFFFFFu;
+                const u32 sm = isopen ? a.smask[v] : 0xFFFFFFFFu;
                 const bool act = isopen && vm <= zb;
-                const u32 open = act ? ~a.smask[v] : 0u;
+                const u32 open = act ? ~sm : 0u;
                 u32 A = 0, B = 0, R = 0, dmin = act ? 0xFFFFFFFFu : vm;
                 if (open) {
                     const uint4 *line = reinterpret_cast<const uint4 *>(a.dl + v * kBatchRoots);
@@ -1163,11 +1174,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
             for (int c = 0; c < kDefer; ++c) wq.ls[c] = (int)(nflush[c] & 1);
             // defers class c's segment of v (the lane owns v in this pass: no
             // atomic needed): ORs H into its pending mask, queues v once
-            auto defer = [&](auto cls, u32 v, u32 H, u32 len) {
+            // (`old`: the class's pending mask of v, read with v's other loads)
+            auto defer = [&](auto cls, u32 v, u32 H, u32 len, u32 old) {
                 constexpr int c = decltype(cls)::value;
                 bool q = false;
                 if (H && len) {
-                    const u32 old = a.dpend[c][v];
                     a.dpend[c][v] = old | H;
                     q = old == 0;
                     if (c == kDefer - 1) atomicOr(&rws[c], H);  // the window's roots (pull flushes read them)
@@ -1196,19 +1207,31 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                     if (lane == 0) work_add(wks[2], 1, na, nh);
                 }
                 if (!ballot(H | N)) return;
+                // Every load of v goes out here, before the first store: the
+                // compiler cannot move a load above a store to another array
+                // (they may alias), so loads interleaved with the updates
+                // below would each cost a round trip of their own.
+                RowInfo r = {};
+                u32 dp[kDefer] = {}, sm = 0, lw = 0;
+                if (H) {
+                    r = load_row_info(a.ri, (u32)v);
+#pragma unroll
+                    for (int c = 0; c < kDefer; ++c) dp[c] = a.dpend[c][v];
+                    sm = a.smask[v];
+                }
+                if (N) lw = a.split[v];
                 // the bucket's roots of v are final now; v closes when no reached root is open
                 bool closed = false;
                 if (H) {
-                    a.smask[v] |= H;
+                    a.smask[v] = sm | H;
                     closed = (R & ~H) == 0;
                 }
                 const u64 cm = ballot(closed);
                 if (cm && lane == 0) atomicAnd((unsigned long long *)&a.openw[word], ~(unsigned long long)cm);
-                push_item(a, (u32)v, N, t, wq);
+                push_item_lw(a, (u32)v, N, lw, t, wq);
                 u64 b = 0, e = 0;
                 u64 cut[kDefer + 1];  // class c is [cut[c], cut[c + 1])
                 if (H) {
-                    const RowInfo r = load_row_info(a.ri, (u32)v);
                     b = r.start + r.light;
                     e = r.start + r.cut[0];
 #pragma unroll
@@ -1218,9 +1241,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
 #pragma unroll
                     for (int c = 0; c <= kDefer; ++c) cut[c] = 0;
                 }
-                defer(std::integral_constant<int, 0>(), (u32)v, H, (u32)(cut[1] - cut[0]));
-                defer(std::integral_constant<int, 1>(), (u32)v, H, (u32)(cut[2] - cut[1]));
-                defer(std::integral_constant<int, 2>(), (u32)v, H, (u32)(cut[3] - cut[2]));
+                defer(std::integral_constant<int, 0>(), (u32)v, H, (u32)(cut[1] - cut[0]), dp[0]);
+                defer(std::integral_constant<int, 1>(), (u32)v, H, (u32)(cut[2] - cut[1]), dp[1]);
+                defer(std::integral_constant<int, 2>(), (u32)v, H, (u32)(cut[3] - cut[2]), dp[2]);
                 static_assert(kDefer == 3, "one defer call per class");
                 const u32 len = (u32)(e - b);
                 const bool item = H && len > 0;
