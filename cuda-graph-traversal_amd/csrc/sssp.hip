@@ -469,16 +469,21 @@ __device__ __forceinline__ void scan_open(const SsspArgs &a, F &&f) {
     const u32 lane = lane_id();
     const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
     const u64 nwords = (a.n + 63) / 64;
-    for (u64 w0 = wave_gid * kWave; w0 < nwords; w0 += nwaves * kWave) {
-        const u64 w = w0 + lane;
+    // Words are dealt to waves round-robin (lane l of wave g reads word
+    // base + l * nwaves + g): open vertices cluster in the degree order, and
+    // runs of 64 consecutive words per wave would leave most waves idle while
+    // a few walk dense runs.
+    for (u64 base = 0; base < nwords; base += nwaves * kWave) {
+        const u64 w = base + (u64)lane * nwaves + wave_gid;
         const u64 open_l = w < nwords ? (a.reached[w] & ~a.settled[w]) : 0ull;
         u64 todo = ballot(open_l != 0);
         // software pipelined: the next word's labels load while f runs
+        auto word_of = [&](int j) { return base + (u64)j * nwaves + wave_gid; };
         auto take = [&](int &j, u32 &lbits) {
             j = todo ? __ffsll((unsigned long long)todo) - 1 : -1;
             todo &= todo - 1;
             const u64 open = shfl64(open_l, j < 0 ? 0 : j);
-            const u64 v = (w0 + (u64)(j < 0 ? 0 : j)) * 64 + lane;
+            const u64 v = word_of(j < 0 ? 0 : j) * 64 + lane;
             lbits = (j >= 0 && ((open >> lane) & 1)) ? a.dl[v] : kInfBits;
         };
         int j;
@@ -488,7 +493,7 @@ __device__ __forceinline__ void scan_open(const SsspArgs &a, F &&f) {
             int jn;
             u32 ln;
             take(jn, ln);
-            f((w0 + (u64)j) * 64 + lane, lbits);
+            f(word_of(j) * 64 + lane, lbits);
             j = jn;
             lbits = ln;
         }

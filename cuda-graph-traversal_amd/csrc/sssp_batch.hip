@@ -1054,20 +1054,44 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
     // reached roots that are not final (R, read lines only).
     auto scan_open = [&](int zl, auto &&f) {
         const u32 zb = __float_as_uint(screen_top(zl));
-        for (u64 w0 = wave_gid * kWave; w0 < nwords; w0 += nwaves * kWave) {
-            const u64 wl = w0 + lane;
+        // Words are dealt to waves round-robin (lane l of wave g reads word
+        // base + l * nwaves + g): open vertices cluster in the degree order, and
+        // runs of 64 consecutive words per wave would leave most waves idle
+        // while a few walk dense runs.
+        for (u64 base = 0; base < nwords; base += nwaves * kWave) {
+            const u64 wl = base + (u64)lane * nwaves + wave_gid;
             const u64 open_l = wl < nwords ? a.openw[wl] : 0ull;
             u64 todo = ballot(open_l != 0);
-            while (todo) {
-                const int j = __ffsll((unsigned long long)todo) - 1;
-                todo &= todo - 1;
-                const u64 word = w0 + (u64)j;
+            // The walk is software-pipelined: the screen loads of the next open
+            // word (vmin and the settled mask, both read for every open vertex)
+            // go out before the current word's line is read and f runs, so
+            // that their round trip overlaps the current word's. f touches
+            // only the current word's vertices, whose loads are its own.
+            int jn = -1;
+            u32 vmn = 0xFFFFFFFFu, smn = 0xFFFFFFFFu;
+            bool openn = false;
+            auto prefetch = [&]() {
+                jn = todo ? __ffsll((unsigned long long)todo) - 1 : -1;
+                vmn = smn = 0xFFFFFFFFu;
+                openn = false;
+                if (jn >= 0) {
+                    todo &= todo - 1;
+                    const u64 vn = (base + (u64)jn * nwaves + wave_gid) * 64 + lane;
+                    openn = (shfl64(open_l, jn) >> lane) & 1;
+                    if (openn) {
+                        vmn = a.vmin[vn];
+                        smn = a.smask[vn];
+                    }
+                }
+            };
+            prefetch();
+            while (jn >= 0) {
+                const int j = jn;
+                const u32 vm = vmn, sm = smn;
+                const bool isopen = openn;
+                prefetch();
+                const u64 word = base + (u64)j * nwaves + wave_gid;
                 const u64 v = word * 64 + lane;
-                const bool isopen = (shfl64(open_l, j) >> lane) & 1;
-                // both loads go out together: the settled mask is read for every
-                // open vertex, not only for one the screen lets through
-                const u32 vm = isopen ? a.vmin[v] : 0xFFFFFFFFu;
-                const u32 sm = isopen ? a.smask[v] : 0xFFFFFFFFu;
                 const bool act = isopen && vm <= zb;
                 const u32 open = act ? ~sm : 0u;
                 u32 A = 0, B = 0, R = 0, dmin = act ? 0xFFFFFFFFu : vm;
