@@ -144,7 +144,7 @@ struct G500_Server {
     std::vector<Slot> slots;
     std::thread copier;
     hipStream_t copy_stream = nullptr;
-    f32 delta = 0.1f;
+    f32 delta = 0.05f;
     // per graph: SSSP row splits and row records for batches of up to 16 roots
     // (delta) and of more (delta / 2)
     std::vector<std::array<Sssp_Batch_State, 2>> splits;

@@ -41,7 +41,7 @@ def load_graphs(sess, spec=GRAPHS, edgefactor=16):
     return [Resident(sess, sc, s1, s2, edgefactor) for sc, s1, s2 in spec]
 
 
-def poisson_run(sess, graphs, rate, count, slots=4, delta=0.2, seed=12345):
+def poisson_run(sess, graphs, rate, count, slots=4, delta=0.05, seed=12345):
     """Submits `count` requests at `rate` per second; returns latency and
     throughput figures of the completed requests."""
     rng = np.random.default_rng(seed)
@@ -72,7 +72,7 @@ def poisson_run(sess, graphs, rate, count, slots=4, delta=0.2, seed=12345):
         "graphs": [f"S{r.scale}s{r.seed[0]}{r.seed[1]}" for r in graphs],
         "rate_offered": rate, "requests": count, "completed": len(done),
         "errors": int(sum(1 for c in done if c.status)),
-        "slots": slots, "launches": stats["launches"], "mean_batch": round(float(bs.mean()), 2),
+        "slots": slots, "delta": delta, "launches": stats["launches"], "mean_batch": round(float(bs.mean()), 2),
         "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "wall_s": round(wall, 3), "req_per_s": round(len(done) / wall, 1),
         # arrivals span at[-1]; the rest of the wall time drains the backlog left at the last arrival
