@@ -72,6 +72,9 @@ struct SsspBatchCtl {
     CounterLine<u64> lhub[kDefer][2];     // deferred queue hub counters [class][set]
     alignas(256) u32 rmin[3][kBatchRoots];  // min step: smallest open distance bits per root, per slot
     CounterLine<u32> plen;        // pull flush: vertices handed from stage A to stage B
+    CounterLine<u32> grab[2];     // light steps of long lists: next 64-item chunk [step parity]
+    CounterLine<u32> mgrab;       // the same for medium items
+    CounterLine<u32> fgrab[kDefer];  // and for each deferred class's flush
     alignas(256) u32 rw[kDefer][2];   // roots that some source settled in the class's window [class][set]
     u32 wcnt[2][kBatchRoots];     // far class: sources each root settled in the window [set][root]
     u64 pull_work[4];             // pull flushes, live (vertex, root) pairs, (vertex, entry) scan steps, stage-B vertices
@@ -695,15 +698,27 @@ __device__ __forceinline__ void pull_clear(const SsspBatchArgs &a, u32 n0, u32 n
 template <int K>
 __device__ __forceinline__ void group_tier(const SsspBatchArgs &a, u32 *pend, const u32 *lq, const u64 *hq, u32 count,
                            const BTarget &t, u64 &improved, volatile int *slot, u64 *wk, WaveQ &wq,
-                           u32 keep = ~0u) {
+                           u32 keep = ~0u, u32 *grab = nullptr) {
     const u64 gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
     const u64 wave_gid = wave_index(), nwaves = gthreads / kWave;
-    for (u64 i0 = wave_gid * kWave; i0 < count; i0 += nwaves * kWave) {
+    // Items per wave: 64, or fewer when 64-item chunks would leave waves idle
+    // (a short list then spreads over the whole grid). A list of several
+    // chunks per wave is dealt dynamically when the caller gives a counter
+    // (zero at the start, grab): the cost of a chunk varies with its rows, and
+    // a fixed stride leaves the waves with the heaviest chunks running alone.
+    const u64 per = count >= nwaves * kWave ? (u64)kWave : max((u64)1, ((u64)count + nwaves - 1) / nwaves);
+    const bool dyn = grab != nullptr && (u64)count >= 4 * nwaves * kWave;
+    auto next_chunk = [&]() -> u64 {
+        u32 c = 0;
+        if (lane == 0) c = atomicAdd(grab, 1u);
+        return (u64)__builtin_amdgcn_readfirstlane(c) * kWave;
+    };
+    for (u64 i0 = dyn ? next_chunk() : wave_gid * per; i0 < count; i0 = dyn ? next_chunk() : i0 + nwaves * per) {
         const u64 i = i0 + lane;
         u32 u = 0, M = 0;
         u64 b = 0, e = 0;
-        if (i < count) {
+        if (lane < per && i < count) {
             if (K != kSegMedium) {
                 u = lq[i];
                 if (u < a.n) {
@@ -1166,13 +1181,16 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
             light_dest(t);
             // selects, not runtime indexes into the argument struct (which would copy it to scratch)
             u32 *pb = buf ? a.pend[1] : a.pend[0];
-            group_tier<kSegLight>(a, pb, buf ? a.fq[1][0] : a.fq[0][0], nullptr, cnt[0], t, improved, slot, wks[0], wq);
+            group_tier<kSegLight>(a, pb, buf ? a.fq[1][0] : a.fq[0][0], nullptr, cnt[0], t, improved, slot, wks[0], wq,
+                                  ~0u, &ctl->grab[x & 1].v);
             wave_tier<kSegLight>(a, pb, buf ? a.fq[1][1] : a.fq[0][1], nullptr, cnt[1], t, improved, wks[0], wq);
             hub_tier<kSegLight>(a, buf ? a.fq[1][2] : a.fq[0][2], buf ? a.fq_off[1] : a.fq_off[0],
                                 buf ? a.fq_len[1] : a.fq_len[0], buf ? a.fq_done[1] : a.fq_done[0], pb, hub, t,
                                 improved, wks[0], wq);
             q_flush(a, wq);
             bsync(ctl, 2, gb, x);
+            // every wave has finished this step's chunks; the counter's next user is step x + 2
+            if (gtid == 0) st_relaxed(&ctl->grab[x & 1].v, 0u);
             tick(0, total);
             ++x;
         }
@@ -1293,13 +1311,14 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
             }
             tick(1);
             group_tier<kSegMedium>(a, nullptr, nullptr, a.hq[0], min(ld_relaxed(&ctl->hlen[hs][0].v), (u32)a.n), t,
-                                   improved, slot, wks[1], wq);
+                                   improved, slot, wks[1], wq, ~0u, &ctl->mgrab.v);
             wave_tier<kSegMedium>(a, nullptr, nullptr, a.hq[1], min(ld_relaxed(&ctl->hlen[hs][1].v), (u32)a.n), t,
                                   improved, wks[1], wq);
             hub_tier<kSegMedium>(a, a.hqh, a.hq_off, a.hq_len, a.hq_mask, nullptr, ld_relaxed(&ctl->hhub[hs].v), t,
                                  improved, wks[1], wq);
             q_flush(a, wq);
             bsync(ctl, 4, gb, x);
+            if (gtid == 0) st_relaxed(&ctl->mgrab.v, 0u);  // its next user is the next dense step
             tick(2, (u64)ld_relaxed(&ctl->hlen[hs][0].v) + ld_relaxed(&ctl->hlen[hs][1].v) +
                         hub_count(ld_relaxed(&ctl->hhub[hs].v)));
             // Flush a deferred class c when the next bucket reaches past
@@ -1365,7 +1384,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                         if (!fl[c] || (c == cf && pull && (rwf & keep) == 0u)) return;
                         const u32 kc = c == cf ? keep : ~0u;
                         group_tier<kSegDef + c>(a, a.dpend[c], a.dq[c][0], nullptr, lc[c][0], t, improved, slot,
-                                                wks[3 + c], wq, kc);
+                                                wks[3 + c], wq, kc, &ctl->fgrab[c].v);
                         wave_tier<kSegDef + c>(a, a.dpend[c], a.dq[c][1], nullptr, lc[c][1], t, improved, wks[3 + c],
                                                wq, kc);
                         hub_tier<kSegDef + c>(a, a.dq[c][2], a.dq_off[c], a.dq_len[c], a.dq_done[c], a.dpend[c], lh[c],
@@ -1378,6 +1397,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                     if (pull) pull_stage_a<cf>(a, rbk, elig, t, improved, wq, pws);
                     q_flush(a, wq);
                     bsync(ctl, 7, gb, x);
+                    if (gtid == 0)  // their next users are the next flushes
+                        for (int c = 0; c < kDefer; ++c) st_relaxed(&ctl->fgrab[c].v, 0u);
                     if (pull) {
                         pull_stage_b<cf>(a, rbk, elig, min(ld_relaxed(&ctl->plen.v), (u32)a.n), t, improved, wq,
                                          pws);
