@@ -370,12 +370,15 @@ __device__ __forceinline__ void process_tier(const SsspArgs &a, const u32 *list,
     const u32 lane = lane_id();
     const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
     if (tier == 0) {
-        for (u64 i0 = wave_gid * kWave; i0 < count; i0 += nwaves * kWave) {
+        // 64 vertices per wave, or fewer when 64-vertex chunks would leave
+        // waves idle (a short list then spreads over the whole grid)
+        const u64 per = count >= nwaves * kWave ? (u64)kWave : max((u64)1, ((u64)count + nwaves - 1) / nwaves);
+        for (u64 i0 = wave_gid * per; i0 < count; i0 += nwaves * per) {
             const u64 i = i0 + lane;
             u64 b = 0, e = 0;
             u32 u = 0;
             f32 du = 0.f;
-            if (i < count) {
+            if (lane < per && i < count) {
                 u = list[i];
                 if (u < a.n) {
                     segment<LIGHT>(a, u, b, e);
