@@ -1558,19 +1558,35 @@ __global__ __launch_bounds__(256) void sssp_batch_export_kernel(const u64 *__res
         if (o < n && 8 * q < nroots) {
             const u64 v = perm ? perm[o] : o;
             const uint4 *p = reinterpret_cast<const uint4 *>(lab + v * kBatchRoots + 8 * q);
+            u64 l8[8];
 #pragma unroll
             for (int h = 0; h < 4; ++h) {
                 const uint4 x = p[h];
-                const u64 l2[2] = {((u64)x.y << 32) | x.x, ((u64)x.w << 32) | x.z};
+                l8[2 * h] = ((u64)x.y << 32) | x.x;
+                l8[2 * h + 1] = ((u64)x.w << 32) | x.z;
+            }
+            // A vertex's parent is often the same for many roots: a parent equal
+            // to the previous root's reuses its original id, and only the others
+            // are looked up (all at once, not one after the other).
+            u32 oid[8];
+            bool fresh[8];
 #pragma unroll
-                for (int i = 0; i < 2; ++i) {
-                    const int r = 8 * q + 2 * h + i;
-                    const u64 l = l2[i];
-                    const bool reached = (u32)(l >> 32) < kInfBits;
-                    over = over || (reached && label_dist(l) >= dbound);
-                    sd[r][j] = reached ? label_dist(l) : -1.0f;
-                    sp[r][j] = reached ? (i64)orig_id(iperm, (u32)l) : -1;
-                }
+            for (int i = 0; i < 8; ++i)
+                fresh[i] = i == 0 || (u32)l8[i] != (u32)l8[i - 1] || (u32)(l8[i - 1] >> 32) >= kInfBits;
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                oid[i] = fresh[i] && (u32)(l8[i] >> 32) < kInfBits ? orig_id(iperm, (u32)l8[i]) : 0u;
+#pragma unroll
+            for (int i = 1; i < 8; ++i)
+                if (!fresh[i]) oid[i] = oid[i - 1];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int r = 8 * q + i;
+                const u64 l = l8[i];
+                const bool reached = (u32)(l >> 32) < kInfBits;
+                over = over || (reached && label_dist(l) >= dbound);
+                sd[r][j] = reached ? label_dist(l) : -1.0f;
+                sp[r][j] = reached ? (i64)oid[i] : -1;
             }
         }
         __syncthreads();
