@@ -30,11 +30,13 @@ constexpr int kSteps = 4;  // bottom-up chunks in flight per wave
 
 struct BatchCtl {
     GridBarrier bar;
+    // counters that many waves append to sit on lines of their own, so that
+    // their atomics do not queue behind each other's
     alignas(256) u64 nrec;         // discovery-log records so far
-    u64 rcount[3];                 // records produced for a level, per slot
-    u32 ntouched[3];               // top-down touched vertices, per slot
-    u64 hub[3];                    // packed hub counters (record indices), per slot
-    u64 fedges[3];                 // sum of degrees of a level's records, per slot
+    alignas(256) u64 rcount[3];    // records produced for a level, per slot
+    alignas(256) u32 ntouched[3];  // top-down touched vertices, per slot
+    alignas(256) u64 hub[3];       // packed hub counters (record indices), per slot
+    alignas(256) u64 fedges[3];    // sum of degrees of a level's records, per slot
     u64 alive[3];                  // OR of a level's masks, per slot
     u64 bhub;                      // packed counter of the heavy-vertex list (bottom-up)
     u32 levels;
@@ -132,6 +134,79 @@ __device__ __forceinline__ void td_edge(const BatchArgs &a, bool act, u32 u, u64
     }
     const u32 s = wave_append(&a.ctl->ntouched[slot], first);
     if (first) a.touched[s] = u;
+}
+
+// Top-down edge of a dense level (see compact_dense): the OR alone, without
+// waiting for its old value and without a touched list.
+__device__ __forceinline__ void td_edge_dense(const BatchArgs &a, bool act, u32 u, u64 mask) {
+    if (act) {
+        const u64 nb = mask & ~a.seen[u];
+        if (nb) atomicOr((unsigned long long *)&a.next[u], (unsigned long long)nb);
+    }
+}
+
+// Compaction of a dense top-down level, one whose records' edges reach a
+// large share of the graph (at SCALE 26 the level after the roots: 3.9e3
+// records, 5.4e7 edges, 8.7e6 vertices reached). Instead of a touched list,
+// whose appends would take one atomic on a single counter per 64 edges, every
+// vertex's next mask is read in one coalesced pass, and a block makes one
+// atomic per pass step for the records of all its waves.
+__device__ void compact_dense(const BatchArgs &a, int slot, int buf) {
+    __shared__ u32 wcnt[1024 / kWave];
+    __shared__ u64 bbase;
+    const u32 lane = lane_id();
+    const int w = (int)(threadIdx.x / kWave), nw = (int)(blockDim.x / kWave);
+    u64 edges = 0, alive = 0;
+    for (u64 u0 = (u64)blockIdx.x * blockDim.x; u0 < a.n; u0 += (u64)gridDim.x * blockDim.x) {
+        const u64 u = u0 + threadIdx.x;
+        u64 nb = 0;
+        if (u < a.n) {
+            nb = a.next[u];
+            if (nb) {
+                a.next[u] = 0;
+                const u64 sn = a.seen[u];
+                nb &= ~sn;
+                if (nb) a.seen[u] = sn | nb;
+            }
+        }
+        const bool want = nb != 0;
+        const u64 m = ballot(want);
+        if (lane == 0) wcnt[w] = (u32)__popcll(m);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            u32 tot = 0;
+            for (int i = 0; i < nw; ++i) {
+                const u32 c = wcnt[i];
+                wcnt[i] = tot;
+                tot += c;
+            }
+            u64 base = 0;
+            if (tot) {
+                base = atomicAdd((unsigned long long *)&a.ctl->nrec, (unsigned long long)tot);
+                atomicAdd((unsigned long long *)&a.ctl->rcount[slot], (unsigned long long)tot);
+            }
+            bbase = base;
+        }
+        __syncthreads();
+        const u64 idx = bbase + wcnt[w] + mask_rank(m);
+        u32 deg = 0;
+        if (want) {
+            if (idx < a.cap) {
+                a.lv[idx] = (u32)u;
+                a.lm[idx] = nb;
+            } else {
+                atomicOr(&a.ctl->bar.error, 2u);
+            }
+            deg = (u32)(a.rs[u + 1] - a.rs[u]);
+            edges += deg;
+            alive |= nb;
+        }
+        const u64 h = (u64)buf * a.n;
+        hub_append(&a.ctl->hub[slot], a.hubs + h, a.hoff + h, a.hlen + h, want && deg > a.hub_degree && idx < a.cap,
+                   (u32)idx, deg, a.n, &a.ctl->bar.error);
+        __syncthreads();  // wcnt and bbase are reused by the next step
+    }
+    flush_level(a, slot, edges, alive);
 }
 
 // Touched vertices of `slot` -> records of the next level.
@@ -413,6 +488,8 @@ __global__ __launch_bounds__(512) void batch_bfs_kernel(BatchArgs a) {
             batch_sync(ctl, 5, L);
             bu_prev = true;
         } else {
+            // a level whose edges reach a large share of the graph is dense (grid-uniform)
+            const bool dense = fedges >= a.n / 16;
             // hub records: chunks of the flattened hub edge space over the grid
             const u64 hp = ld_relaxed(&ctl->hub[cur]);
             const u64 nh = hub_count(hp), ht = hub_total(hp);
@@ -430,7 +507,10 @@ __global__ __launch_bounds__(512) void batch_bfs_kernel(BatchArgs a) {
 #pragma unroll
                 for (u64 j = 0; j < kHubChunk; j += kWave) {
                     const u64 k = b + j + lane;
-                    td_edge(a, k < e, k < e ? a.col[k] : 0, mask, nxt);
+                    if (dense)
+                        td_edge_dense(a, k < e, k < e ? a.col[k] : 0, mask);
+                    else
+                        td_edge(a, k < e, k < e ? a.col[k] : 0, mask, nxt);
                 }
             }
             for (u64 i = lo + wave_gid; i < hi; i += nwaves) {  // a wave per other record
@@ -440,11 +520,17 @@ __global__ __launch_bounds__(512) void batch_bfs_kernel(BatchArgs a) {
                 const u64 mask = a.lm[i];
                 for (u64 k0 = b; k0 < e; k0 += kWave) {
                     const u64 k = k0 + lane;
-                    td_edge(a, k < e, k < e ? a.col[k] : 0, mask, nxt);
+                    if (dense)
+                        td_edge_dense(a, k < e, k < e ? a.col[k] : 0, mask);
+                    else
+                        td_edge(a, k < e, k < e ? a.col[k] : 0, mask, nxt);
                 }
             }
             batch_sync(ctl, 6, L);
-            compact_touched(a, nxt, (int)((L + 1) & 1));
+            if (dense)
+                compact_dense(a, nxt, (int)((L + 1) & 1));
+            else
+                compact_touched(a, nxt, (int)((L + 1) & 1));
             batch_sync(ctl, 7, L);
             bu_prev = false;
         }
