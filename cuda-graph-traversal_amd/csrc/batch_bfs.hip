@@ -27,6 +27,7 @@ namespace g500 {
 
 constexpr int kBatchLevelLog = 256;
 constexpr int kSteps = 4;  // bottom-up chunks in flight per wave
+constexpr u32 kRecStage = 256;  // bottom-up records a wave stages in LDS before appending them
 
 struct BatchCtl {
     GridBarrier bar;
@@ -257,6 +258,47 @@ __device__ void bu_groups(const BatchArgs &a, u32 L, u64 alive_cur, int nxt, u64
     const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x, gthreads = (u64)gridDim.x * blockDim.x;
     const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
     const u64 ngroups = (a.n + kWave - 1) / kWave;
+    // The wave's records are staged in LDS and appended in runs of up to
+    // kRecStage: one pair of counter atomics per run instead of one per
+    // 64-vertex group (a large level has records in almost every group).
+    __shared__ u32 stv[512 / kWave][kRecStage];
+    __shared__ u64 stm[512 / kWave][kRecStage];
+    volatile u32 *sv = stv[threadIdx.x / kWave];
+    volatile u64 *sm = stm[threadIdx.x / kWave];
+    u32 sn = 0;  // staged records (wave-uniform)
+    const int hbuf = (int)((L + 1) & 1);
+    auto stage_flush = [&]() {
+        if (!sn) return;
+        u64 rb = 0;
+        if (lane == 0) {
+            rb = atomicAdd((unsigned long long *)&a.ctl->nrec, (unsigned long long)sn);
+            atomicAdd((unsigned long long *)&a.ctl->rcount[nxt], (unsigned long long)sn);
+        }
+        rb = shfl64(rb, 0);
+        const u64 h = (u64)hbuf * a.n;
+        for (u32 i0 = 0; i0 < sn; i0 += kWave) {
+            const u32 i = i0 + lane;
+            const bool want = i < sn;
+            const u64 idx = rb + i;
+            u32 deg = 0, v = 0;
+            if (want) {
+                v = sv[i];
+                const u64 m = sm[i];
+                if (idx < a.cap) {
+                    a.lv[idx] = v;
+                    a.lm[idx] = m;
+                } else {
+                    atomicOr(&a.ctl->bar.error, 2u);
+                }
+                deg = (u32)(a.rs[v + 1] - a.rs[v]);
+                edges += deg;
+                alive |= m;
+            }
+            hub_append(&a.ctl->hub[nxt], a.hubs + h, a.hoff + h, a.hlen + h,
+                       want && deg > a.hub_degree && idx < a.cap, (u32)idx, deg, a.n, &a.ctl->bar.error);
+        }
+        sn = 0;
+    };
     for (u64 grp = wave_gid; grp < ngroups; grp += nwaves) {
         const u64 u = grp * kWave + lane;
         u64 need = 0, base = 0;
@@ -326,8 +368,18 @@ __device__ void bu_groups(const BatchArgs &a, u32 L, u64 alive_cur, int nxt, u64
         }
         const u64 word = ballot(nb != 0);
         if (lane == 0) fbn[grp] = word;
-        append_record(a, nb != 0, (u32)u, nb, nxt, (int)((L + 1) & 1), edges, alive);
+        if (word) {
+            const u32 k = (u32)__popcll(word);
+            if (sn + k > kRecStage) stage_flush();
+            if (nb) {
+                const u32 pos = sn + mask_rank(word);
+                sv[pos] = (u32)u;
+                sm[pos] = nb;
+            }
+            sn += k;
+        }
     }
+    stage_flush();
 }
 
 // Heavy vertices with missing roots: their edge lists in 256-edge chunks over
