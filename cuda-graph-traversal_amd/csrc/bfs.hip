@@ -25,10 +25,12 @@ constexpr int kMaxLevelLog = 256;
 
 struct BfsCtl {
     GridBarrier bar;
+    // counters that many waves add to sit on lines of their own, so that
+    // their atomics do not queue behind each other's
     alignas(256) u32 qlen[3];
-    u64 hcnt[3];   // packed hub counters (count << 38 | padded edge total)
-    u64 fcount[3];
-    u64 fedges[3];
+    alignas(256) u64 hcnt[3];   // packed hub counters (count << 38 | padded edge total)
+    alignas(256) u64 fcount[3];
+    alignas(256) u64 fedges[3];
     u32 levels;
     u32 depth_changed;
     alignas(256) i32 mode_log[kMaxLevelLog];
@@ -71,8 +73,22 @@ __device__ __forceinline__ u64 ld64(const u64 *p) {
 
 // Claims v for parent u when this lane is active; appends winners to the next
 // queue (or the hub queue) with one atomic per wave. All lanes must call it.
+// A wave's top-down queue appends are staged in LDS (sq, sn entries) and
+// written out in runs: one atomic on the queue counter per run instead of one
+// per 64 edges with a new vertex.
+constexpr u32 kTdStage = 256;
+__device__ __forceinline__ void td_flush(u32 *qn, BfsCtl *ctl, int nxt, volatile u32 *sq, u32 &sn, u64 cap) {
+    if (!sn) return;
+    u32 base = 0;
+    if (lane_id() == 0) base = atomicAdd(&ctl->qlen[nxt], sn);
+    base = __shfl(base, 0);
+    for (u32 i = lane_id(); i < sn; i += kWave)
+        if ((u64)base + i < cap) qn[base + i] = sq[i];
+    sn = 0;
+}
+
 __device__ __forceinline__ void td_visit(const BfsArgs &a, u32 u, u32 v, bool active, u32 *qn, int nb,
-                                         BfsCtl *ctl, int nxt, u32 &cnt, u64 &edges) {
+                                         BfsCtl *ctl, int nxt, u32 &cnt, u64 &edges, volatile u32 *sq, u32 &sn) {
     bool won = false;
     if (active) {
         const u64 bit = 1ull << (v & 63);
@@ -89,8 +105,13 @@ __device__ __forceinline__ void td_visit(const BfsArgs &a, u32 u, u32 v, bool ac
         edges += deg;
     }
     const bool hub = won && deg > a.hub_degree;
-    const u32 s1 = wave_append(&ctl->qlen[nxt], won && !hub);
-    if (won && !hub) qn[s1] = v;
+    const u64 m = ballot(won && !hub);
+    if (m) {
+        const u32 k = (u32)__popcll(m);
+        if (sn + k > kTdStage) td_flush(qn, ctl, nxt, sq, sn, a.n);
+        if (won && !hub) sq[sn + mask_rank(m)] = v;
+        sn += k;
+    }
     hub_append(&ctl->hcnt[nxt], a.hubs[nb], a.hoff[nb], a.hlen[nb], hub, v, deg, a.n, &ctl->bar.error);
 }
 
@@ -100,6 +121,9 @@ __global__ __launch_bounds__(512) void bfs_kernel(BfsArgs a) {
     const u64 gthreads = (u64)gridDim.x * blockDim.x;
     const u32 lane = lane_id();
     const u64 wave_gid = gtid / kWave, nwaves = gthreads / kWave;
+    __shared__ u32 tq[512 / kWave][kTdStage];
+    volatile u32 *sq = tq[threadIdx.x / kWave];
+    u32 sn = 0;  // staged queue entries of this wave (wave-uniform)
 
     if (gtid == 0) {  // level 0: the root
         const u64 r = trav_id(a.perm, a.root);
@@ -206,7 +230,7 @@ __global__ __launch_bounds__(512) void bfs_kernel(BfsArgs a) {
                 for (u64 j = 0; j < kHubChunk; j += kWave) {
                     const u64 k = b + j + lane;
                     const bool act = k < e;
-                    td_visit(a, u, act ? a.col[k] : 0, act, qn, (int)((L + 1) & 1), ctl, nxt, cnt, edges);
+                    td_visit(a, u, act ? a.col[k] : 0, act, qn, (int)((L + 1) & 1), ctl, nxt, cnt, edges, sq, sn);
                 }
             }
             for (u64 i = wave_gid; i < nq; i += nwaves) {  // one wave per frontier vertex
@@ -215,9 +239,10 @@ __global__ __launch_bounds__(512) void bfs_kernel(BfsArgs a) {
                 for (u64 k0 = b; k0 < e; k0 += kWave) {
                     const u64 k = k0 + lane;
                     const bool act = k < e;
-                    td_visit(a, u, act ? a.col[k] : 0, act, qn, (int)((L + 1) & 1), ctl, nxt, cnt, edges);
+                    td_visit(a, u, act ? a.col[k] : 0, act, qn, (int)((L + 1) & 1), ctl, nxt, cnt, edges, sq, sn);
                 }
             }
+            td_flush(qn, ctl, nxt, sq, sn, a.n);
         } else {
             for (u64 w = wave_gid; w < a.nwords; w += nwaves) {
                 const u64 vis = a.visited[w];
