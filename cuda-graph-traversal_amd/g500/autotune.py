@@ -31,7 +31,8 @@ SSSP_PULL = [0.02, 0.05, 0.2]
 BATCH_SSSP = [(0.025, 0.075, 0.25, 0.5), (0.025, 0.05, 0.15, 0.3), (0.025, 0.075, 0.25, 0.25),
               (0.0125, 0.05, 0.2, 0.5), (0.05, 0.1, 0.3, 0.6), (0.05, 0.075, 0.2, 0.4),
               (0.025, 0.1, 0.35, 0.35), (0.0375, 0.075, 0.25, 0.25), (0.025, 0.05, 0.2, 0.2),
-              (0.025, 0.05, 0.15, 0.15), (0.025, 0.05, 0.1, 0.1)]
+              (0.025, 0.05, 0.15, 0.15), (0.025, 0.05, 0.1, 0.1), (0.02, 0.05, 0.15, 0.15),
+              (0.02, 0.04, 0.15, 0.15), (0.02, 0.05, 0.2, 0.2), (0.0175, 0.05, 0.15, 0.15)]
 BATCH_PULL = [0.0, 0.1, 0.2, 0.3]
 
 
