@@ -566,7 +566,14 @@ __device__ __forceinline__ void pull_step(const SsspArgs &a, u32 x, f32 lo, f32 
         push_light(a, act && !longrow && best >= hi_b && best < hi2_b, (u32)v, t, sq);
     }
     stage_flush(a, sq);
-    if (lane == 0 && nset) atomicAdd((unsigned long long *)&ctl->nset[hs], (unsigned long long)nset);
+    {  // the block sums its waves' settled counts and makes one atomic (every thread is here)
+        __shared__ u64 bset;
+        if (threadIdx.x == 0) bset = 0;
+        __syncthreads();
+        if (lane == 0 && nset) atomicAdd((unsigned long long *)&bset, (unsigned long long)nset);
+        __syncthreads();
+        if (threadIdx.x == 0 && bset) atomicAdd((unsigned long long *)&ctl->nset[hs], (unsigned long long)bset);
+    }
     sync_traced(ctl, kPhaseFused, lo_b, x);
     // long rows: a wave per vertex, 64 entries per chunk, lightest chunk first
     const u32 nlong = min(ld_relaxed(&ctl->hlen[hs].v), (u32)a.n);
@@ -764,7 +771,14 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void s
                 walk_segments(a, t, (u32)v, dv, b, mine && len <= kSmall ? len : 0, improved, slot, sq);
             });
             stage_flush(a, sq);
-            if (nset_local) atomicAdd((unsigned long long *)&ctl->nset[hs], (unsigned long long)nset_local);
+            {  // the block sums its waves' settled counts and makes one atomic (every thread is here)
+                __shared__ u64 bset;
+                if (threadIdx.x == 0) bset = 0;
+                __syncthreads();
+                if (nset_local) atomicAdd((unsigned long long *)&bset, (unsigned long long)nset_local);
+                __syncthreads();
+                if (threadIdx.x == 0 && bset) atomicAdd((unsigned long long *)&ctl->nset[hs], (unsigned long long)bset);
+            }
             sync_traced(ctl, kPhaseFused, __float_as_uint(lo), x);
             settled_total += ld_relaxed(&ctl->nset[hs]);
             const u32 nmed = ld_relaxed(&ctl->hlen[hs].v);
